@@ -1,0 +1,153 @@
+/*
+ * mpqp_hip.h — C ABI of libmpqp_hip.so, the MI355X (gfx950) engine for the
+ * batched feasibility / optimality checks of candidate active sets in
+ * combinatorial multiparametric QP.
+ *
+ * Drop-in boundary: the Python host of willowfordcourt/mpQPCudaSolver loads
+ * `mpqp_cuda.so` with ctypes and binds exactly two symbols, `warm_up` and
+ * `mpqp`, plus the `GPUResults` struct (main.py:12-50, called at main.py:60
+ * and main.py:144).  This header declares the same two symbols with the same
+ * argument order, C types and return struct, so a ctypes host written for
+ * mpqp_cuda.so loads libmpqp_hip.so unchanged.  Everything after the
+ * "extensions" banner is additive: the reference has no counterpart and never
+ * calls it.
+ *
+ * Conventions (same as the reference call site, main.py:87-141):
+ *   - matrices are column-major (Fortran order) float64, A[col * M + row];
+ *   - index vectors are int32;
+ *   - no function throws; errors are reported through mpqp_last_error()
+ *     and a NULL / negative return (the reference aborted instead).
+ */
+#ifndef MPQP_HIP_H
+#define MPQP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result struct, field-for-field the ctypes.Structure of main.py:12-22.
+ * x86-64 offsets 0,4,8,12,16,24,32,40; size 48.
+ *   active_sets: n_final_lps rows of n_x int32, each row the ascending
+ *   constraint indices of one final active set, padded with -1
+ *   (decoded by utils.py:109-115).  Owned by the library; valid until the
+ *   next mpqp() call on this thread or mpqp_free_results(). */
+typedef struct GPUResults {
+    int n_batches;       /* kernel batches launched (levels x chunks) */
+    int n_splits;        /* always 1: level-wise frontiers replace splits */
+    int n_feasible_lps;  /* candidates whose feasibility LP is feasible */
+    int n_final_lps;     /* rows of active_sets */
+    double time_feas;    /* seconds in feasibility LPs */
+    double time_opti;    /* seconds in optimality LPs */
+    double time_GPU;     /* seconds from first launch to verdicts on host */
+    int *active_sets;
+} GPUResults;
+
+/* Replaces the reference's warm_up() (bound at main.py:60, no argtypes /
+ * restype): creates the HIP context and stream on device 0 and loads the
+ * kernels.  Returns 0 on success (the reference ignores the return). */
+int warm_up(void);
+
+/* Replaces mpqp() of mpqp_cuda.so (argtypes main.py:28-49, call main.py:144).
+ *   AxT          n_x x n_c, A[:n_c,:n_x]^T column-major (main.py:100,130).
+ *   n_c,n_x,n_t  constraint / variable / parameter counts.
+ *   k_min,k_max  active-set cardinalities reported / enumerated.
+ *   extra_batches  accepted, unused (the reference passes 0, main.py:141).
+ *   A1,xb1,b1,M1,N1  feasibility LP  [A -F -rowsum I; 0 A_t -rowsum I],
+ *                    rhs [b; b_t], slack basis  (main.py:91-98,121).
+ *   A2,xb2,b2,non_b2,M2,N2,n_eqs  optimality LP (main.py:103-128).
+ *   reserved_memory  MB of device memory to leave free (main.py:52,68-69).
+ * Returns NULL on error (see mpqp_last_error). */
+GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
+                 int extra_batches, double *A1, double *xb1, int *b1, int M1, int N1,
+                 double *A2, double *xb2, int *b2, int *non_b2, int M2, int N2,
+                 int n_eqs, int reserved_memory);
+
+/* ------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference ABI).
+ * ---------------------------------------------------------------------- */
+
+enum { MPQP_FP64 = 0, MPQP_MIXED = 1 };          /* MpqpOptions.precision */
+enum { MPQP_PLAIN = 0, MPQP_PERSISTENT = 1 };    /* MpqpOptions.batching */
+
+/* Per-LP status codes. */
+enum {
+    MPQP_ST_NOT_RUN = 0,
+    MPQP_ST_OPTIMAL = 1,    /* simplex terminated by its pricing / stop rule */
+    MPQP_ST_UNBOUNDED = 2,  /* empty ratio test (primal or dual) */
+    MPQP_ST_ITER_CAP = 3,   /* iteration cap reached: undecided */
+    MPQP_ST_STALL = 4,      /* phase-1 stall exit */
+    MPQP_ST_SINGULAR = 5    /* initial KKT basis singular (LICQ fails) */
+};
+
+/* Tolerances and control constants.  The reference's host passes the six
+ * tolerances in (batched_simplex.cu:991,1262); their values were never
+ * published, so the defaults here are this library's choice. */
+typedef struct MpqpOptions {
+    int precision;          /* MPQP_FP64 | MPQP_MIXED */
+    int batching;           /* MPQP_PLAIN | MPQP_PERSISTENT */
+    int record;             /* 0: none; 1: per-candidate records; 2: + final bases */
+    int output_flag;        /* >0 prints per-level progress to stdout */
+    int max_iter_feas;      /* 500   (batched_simplex.cu:1162) */
+    int max_iter_phase1;    /* 10000 (batched_simplex.cu:1474) */
+    int max_iter_phase2;    /* 800   (batched_simplex.cu:1633) */
+    int stall_perturb;      /* 20    (batched_simplex.cu:1550) */
+    int stall_exit;         /* 300   (batched_simplex.cu:1561) */
+    int opt_threads;        /* threads per optimality workgroup; 0 = auto */
+    int warm_start;         /* 1: children start from the parent's final basis */
+    int reserved0;
+    double tol_plus;           /* primal ratio test: column entry > tol_plus */
+    double tol_minus;          /* dual ratio test: row entry < tol_minus */
+    double tol_minus_1;        /* feasibility pricing: stop if min d > tol_minus_1 */
+    double tol_minus_optimal;  /* phase-2 pricing: stop if min d > tol_minus_optimal */
+    double dual_feasible_tol;  /* phase-1 verdict |z| < dual_feasible_tol */
+    double dual_stop_tol;      /* phase-1 stop: no x_B < dual_stop_tol */
+    double feas_z_tol;         /* feasibility verdict |z| <= feas_z_tol (1e-12) */
+    double licq_tol;           /* |pivot| <= licq_tol on a KKT-basis build: singular */
+    double final_tol;          /* final active set if margin eps* > final_tol */
+    double perturb_step;       /* rhs perturbation (i+1)*perturb_step (1e-6) */
+    double certify_tol;        /* mixed precision: FP64 certification tolerance */
+    double reserved1;
+} MpqpOptions;
+
+/* One record per candidate active set of the last mpqp() call, in
+ * enumeration order (level, then lexicographic). */
+typedef struct MpqpRecord {
+    int level;          /* |S| */
+    int parent;         /* record index of S \ {max S}, -1 for the empty set */
+    int feas_status, feas_iters, feasible;
+    int p1_status, p1_iters, kkt_feasible, p1_warm;
+    int p2_status, p2_iters, final_set;
+    double z_feas;      /* feasibility LP objective (sum of slacks of S) */
+    double z1;          /* phase-1 objective (sum of artificials) */
+    double z2;          /* phase-2 objective = -eps* */
+    int repivots;       /* mixed precision: FP64 re-pivots after certification */
+    int reserved;
+} MpqpRecord;
+
+void mpqp_default_options(MpqpOptions *opt);
+int mpqp_set_options(const MpqpOptions *opt);   /* 0 ok, <0 invalid */
+int mpqp_get_options(MpqpOptions *opt);
+
+/* Frees a result returned by mpqp() (the reference has no free symbol;
+ * Python hosts that never call it leak nothing beyond the last result). */
+void mpqp_free_results(GPUResults *res);
+
+/* Records of the last mpqp() call (requires MpqpOptions.record >= 1).
+ * Returns the number of candidates; copies at most max_records of them.
+ * Any output pointer may be NULL.
+ *   sets:      max_records x k_max int32, -1 padded
+ *   feas_basis max_records x M1, p1_basis / p2_basis max_records x M2
+ *              (record >= 2; rows of candidates that did not run are -1). */
+int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis,
+                     int *p1_basis, int *p2_basis, int max_records);
+
+/* Copies the last error message; returns its length (0 if none). */
+int mpqp_last_error(char *buf, int len);
+
+/* Library identification: "hip-gfx950" for libmpqp_hip.so. */
+const char *mpqp_backend(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPQP_HIP_H */

@@ -1,0 +1,14 @@
+"""MI355X (gfx950) engine for the batched active-set checks of combinatorial mpQP.
+
+The product is the C-ABI library `libmpqp_hip.so` (hand-written HIP kernels);
+this package holds the ctypes host that mirrors the reference's binding
+(main.py:11-60), the problem generators and the LP assembly (main.py:87-138).
+"""
+from .problem import (  # noqa: F401
+    MPQPProb, LPData, assemble_lps, synthetic_mpqp, precision_stress_mpqp,
+    uneven_mpqps, config_problem, CONFIGS, n_candidates,
+)
+from .host import (  # noqa: F401
+    GPUResults, MpqpOptions, MpqpRecord, MpqpLib, MpqpRun, library_path,
+    load_library, decode_active_sets, calculate_mpqp_with_HIP,
+)

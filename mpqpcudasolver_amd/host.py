@@ -1,0 +1,311 @@
+"""ctypes host for libmpqp_hip.so — the counterpart of the reference's binding.
+
+The reference binds `mpqp_cuda.so` inside `calculate_mpqp_with_CUDA`
+(main.py:11-60): a `GPUResults` ctypes.Structure (main.py:12-22), 20
+positional argtypes (main.py:28-49), `warm_up()` (main.py:60) and the call at
+main.py:144, whose `active_sets` are decoded by utils.py:109-115.  `MpqpLib`
+declares the same struct and argtypes, so a caller can switch the library path
+and nothing else; the extra symbols (options, per-candidate records) are the
+library's extensions.
+
+The product library is `libmpqp_hip.so` next to this file.  `load_library()`
+raises if it has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .problem import LPData, MPQPProb, assemble_lps
+
+__all__ = [
+    "GPUResults",
+    "MpqpOptions",
+    "MpqpRecord",
+    "MpqpLib",
+    "MpqpRun",
+    "library_path",
+    "load_library",
+    "decode_active_sets",
+    "calculate_mpqp_with_HIP",
+    "STATUS_NAMES",
+]
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+
+MPQP_FP64, MPQP_MIXED = 0, 1
+MPQP_PLAIN, MPQP_PERSISTENT = 0, 1
+STATUS_NAMES = {0: "not_run", 1: "optimal", 2: "unbounded", 3: "iter_cap", 4: "stall", 5: "singular"}
+
+
+class GPUResults(ctypes.Structure):
+    """Field-for-field the struct of main.py:12-22."""
+
+    _fields_ = [
+        ("n_batches", ctypes.c_int),
+        ("n_splits", ctypes.c_int),
+        ("n_feasible_lps", ctypes.c_int),
+        ("n_final_lps", ctypes.c_int),
+        ("time_feas", ctypes.c_double),
+        ("time_opti", ctypes.c_double),
+        ("time_GPU", ctypes.c_double),
+        ("active_sets", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class MpqpOptions(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("batching", ctypes.c_int),
+        ("record", ctypes.c_int),
+        ("output_flag", ctypes.c_int),
+        ("max_iter_feas", ctypes.c_int),
+        ("max_iter_phase1", ctypes.c_int),
+        ("max_iter_phase2", ctypes.c_int),
+        ("stall_perturb", ctypes.c_int),
+        ("stall_exit", ctypes.c_int),
+        ("opt_threads", ctypes.c_int),
+        ("warm_start", ctypes.c_int),
+        ("reserved0", ctypes.c_int),
+        ("tol_plus", ctypes.c_double),
+        ("tol_minus", ctypes.c_double),
+        ("tol_minus_1", ctypes.c_double),
+        ("tol_minus_optimal", ctypes.c_double),
+        ("dual_feasible_tol", ctypes.c_double),
+        ("dual_stop_tol", ctypes.c_double),
+        ("feas_z_tol", ctypes.c_double),
+        ("licq_tol", ctypes.c_double),
+        ("final_tol", ctypes.c_double),
+        ("perturb_step", ctypes.c_double),
+        ("certify_tol", ctypes.c_double),
+        ("reserved1", ctypes.c_double),
+    ]
+
+
+class MpqpRecord(ctypes.Structure):
+    _fields_ = [
+        ("level", ctypes.c_int),
+        ("parent", ctypes.c_int),
+        ("feas_status", ctypes.c_int),
+        ("feas_iters", ctypes.c_int),
+        ("feasible", ctypes.c_int),
+        ("p1_status", ctypes.c_int),
+        ("p1_iters", ctypes.c_int),
+        ("kkt_feasible", ctypes.c_int),
+        ("p1_warm", ctypes.c_int),
+        ("p2_status", ctypes.c_int),
+        ("p2_iters", ctypes.c_int),
+        ("final_set", ctypes.c_int),
+        ("z_feas", ctypes.c_double),
+        ("z1", ctypes.c_double),
+        ("z2", ctypes.c_double),
+        ("repivots", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+    ]
+
+
+RECORD_DTYPE = np.dtype([
+    ("level", np.int32), ("parent", np.int32),
+    ("feas_status", np.int32), ("feas_iters", np.int32), ("feasible", np.int32),
+    ("p1_status", np.int32), ("p1_iters", np.int32), ("kkt_feasible", np.int32), ("p1_warm", np.int32),
+    ("p2_status", np.int32), ("p2_iters", np.int32), ("final_set", np.int32),
+    ("z_feas", np.float64), ("z1", np.float64), ("z2", np.float64),
+    ("repivots", np.int32), ("reserved", np.int32),
+])
+assert RECORD_DTYPE.itemsize == ctypes.sizeof(MpqpRecord)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+# argtypes of main.py:28-49, in order.
+MPQP_ARGTYPES = [
+    _dp,            # AxT
+    ctypes.c_int,   # n_c
+    ctypes.c_int,   # n_x
+    ctypes.c_int,   # n_t
+    ctypes.c_int,   # k_min
+    ctypes.c_int,   # k_max
+    ctypes.c_int,   # extra_batches
+    _dp,            # A1
+    _dp,            # xb1
+    _ip,            # b1
+    ctypes.c_int,   # M1
+    ctypes.c_int,   # N1
+    _dp,            # A2
+    _dp,            # xb2
+    _ip,            # b2
+    _ip,            # non_b2
+    ctypes.c_int,   # M2
+    ctypes.c_int,   # N2
+    ctypes.c_int,   # n_eqs
+    ctypes.c_int,   # reserved_memory
+]
+
+
+def library_path() -> str:
+    return os.path.join(_DIR, "libmpqp_hip.so")
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+@dataclass
+class MpqpRun:
+    """Decoded result of one mpqp() call."""
+
+    n_batches: int
+    n_splits: int
+    n_feasible_lps: int
+    n_final_lps: int
+    time_feas: float
+    time_opti: float
+    time_GPU: float
+    active_sets: list
+    records: np.ndarray | None = None
+    sets: np.ndarray | None = None
+    feas_basis: np.ndarray | None = None
+    p1_basis: np.ndarray | None = None
+    p2_basis: np.ndarray | None = None
+
+
+def decode_active_sets(ptr, n_sets: int, num_x: int) -> list:
+    """utils.py:109-115: rows of num_x ints, each ended by the first -1."""
+    out = []
+    for lp_idx in range(n_sets):
+        active_set = []
+        for ele_idx in range(num_x):
+            v = ptr[lp_idx * num_x + ele_idx]
+            if v == -1:
+                break
+            active_set.append(int(v))
+        out.append(active_set)
+    return out
+
+
+class MpqpLib:
+    """A loaded library exporting the mpqp C ABI (include/mpqp_hip.h)."""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise RuntimeError(f"mpqp library not found: {path} (build it with __graft_entry__.build())")
+        self.path = path
+        self.lib = ctypes.cdll.LoadLibrary(path)
+        L = self.lib
+        L.mpqp.argtypes = MPQP_ARGTYPES
+        L.mpqp.restype = ctypes.POINTER(GPUResults)
+        L.warm_up.restype = ctypes.c_int
+        L.mpqp_default_options.argtypes = [ctypes.POINTER(MpqpOptions)]
+        L.mpqp_default_options.restype = None
+        L.mpqp_set_options.argtypes = [ctypes.POINTER(MpqpOptions)]
+        L.mpqp_set_options.restype = ctypes.c_int
+        L.mpqp_get_options.argtypes = [ctypes.POINTER(MpqpOptions)]
+        L.mpqp_get_options.restype = ctypes.c_int
+        L.mpqp_free_results.argtypes = [ctypes.POINTER(GPUResults)]
+        L.mpqp_free_results.restype = None
+        L.mpqp_get_records.argtypes = [ctypes.POINTER(MpqpRecord), _ip, _ip, _ip, _ip, ctypes.c_int]
+        L.mpqp_get_records.restype = ctypes.c_int
+        L.mpqp_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.mpqp_last_error.restype = ctypes.c_int
+        L.mpqp_backend.restype = ctypes.c_char_p
+
+    @property
+    def backend(self) -> str:
+        return self.lib.mpqp_backend().decode()
+
+    def warm_up(self) -> int:
+        return self.lib.warm_up()
+
+    def last_error(self) -> str:
+        buf = ctypes.create_string_buffer(1024)
+        self.lib.mpqp_last_error(buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    def default_options(self) -> MpqpOptions:
+        o = MpqpOptions()
+        self.lib.mpqp_default_options(ctypes.byref(o))
+        return o
+
+    def get_options(self) -> MpqpOptions:
+        o = MpqpOptions()
+        self.lib.mpqp_get_options(ctypes.byref(o))
+        return o
+
+    def set_options(self, opts: MpqpOptions | None = None, **kw) -> MpqpOptions:
+        o = opts if opts is not None else self.default_options()
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        rc = self.lib.mpqp_set_options(ctypes.byref(o))
+        if rc != 0:
+            raise ValueError(self.last_error())
+        return o
+
+    def mpqp_raw(self, lp: LPData, k_min: int, k_max: int, reserved_memory: int = 0,
+                 extra_batches: int = 0):
+        """The call of main.py:144 with the arrays of `lp`; returns the
+        POINTER(GPUResults) exactly as the reference's caller receives it."""
+        self._keep = lp  # arrays must outlive the call
+        return self.lib.mpqp(
+            _ptr(lp.A0, _dp), lp.n_c, lp.n_x, lp.n_t, k_min, k_max, extra_batches,
+            _ptr(lp.A1, _dp), _ptr(lp.xb1, _dp), _ptr(lp.b1, _ip), lp.M1, lp.N1,
+            _ptr(lp.A2, _dp), _ptr(lp.xb2, _dp), _ptr(lp.b2, _ip), _ptr(lp.non_b2, _ip),
+            lp.M2, lp.N2, lp.n_eqs, reserved_memory)
+
+    def run(self, lp: LPData, k_max: int, k_min: int = 0, reserved_memory: int = 0,
+            records: bool = False, bases: bool = False) -> MpqpRun:
+        res = self.mpqp_raw(lp, k_min, k_max, reserved_memory)
+        if not res:
+            raise RuntimeError(f"mpqp failed: {self.last_error()}")
+        r = res.contents
+        run = MpqpRun(
+            n_batches=r.n_batches, n_splits=r.n_splits, n_feasible_lps=r.n_feasible_lps,
+            n_final_lps=r.n_final_lps, time_feas=r.time_feas, time_opti=r.time_opti,
+            time_GPU=r.time_GPU,
+            active_sets=decode_active_sets(r.active_sets, r.n_final_lps, lp.n_x),
+        )
+        if records:
+            n = self.lib.mpqp_get_records(None, None, None, None, None, 0)
+            kw = max(k_max, 1)
+            recs = np.zeros(n, dtype=RECORD_DTYPE)
+            sets = np.full((n, kw), -1, dtype=np.intc)
+            fb = p1 = p2 = None
+            if bases:
+                fb = np.full((n, lp.M1), -1, dtype=np.intc)
+                p1 = np.full((n, lp.M2), -1, dtype=np.intc)
+                p2 = np.full((n, lp.M2), -1, dtype=np.intc)
+            self.lib.mpqp_get_records(
+                recs.ctypes.data_as(ctypes.POINTER(MpqpRecord)), _ptr(sets, _ip),
+                _ptr(fb, _ip) if bases else None, _ptr(p1, _ip) if bases else None,
+                _ptr(p2, _ip) if bases else None, n)
+            run.records, run.sets = recs, sets
+            run.feas_basis, run.p1_basis, run.p2_basis = fb, p1, p2
+        self.lib.mpqp_free_results(res)
+        return run
+
+
+def load_library(path: str | None = None) -> MpqpLib:
+    """Loads libmpqp_hip.so (the HIP engine).  Raises if it is missing."""
+    return MpqpLib(path or library_path())
+
+
+def calculate_mpqp_with_HIP(problems, k_max: int | None = None, lib: MpqpLib | None = None,
+                            reserved_memory: int = 0):
+    """Counterpart of calculate_mpqp_with_CUDA (main.py:11-173) up to the
+    GPU call: assembles each problem's LPs, calls mpqp() and decodes the
+    final active sets.  Returns one (active_sets, MpqpRun) per problem.
+    Critical-region reconstruction (utils.py:23-144, Gurobi) is out of scope."""
+    lib = lib or load_library()
+    lib.warm_up()
+    out = []
+    for prog in problems:
+        lp = assemble_lps(prog)
+        km = prog.num_x if k_max is None else k_max
+        run = lib.run(lp, k_max=km, k_min=0, reserved_memory=reserved_memory)
+        prog.n_active_sets = run.n_final_lps
+        out.append((run.active_sets, run))
+    return out

@@ -1,0 +1,1134 @@
+/*
+ * mpqp_oracle.c — CPU FP64 restatement of the feasibility and optimality
+ * checks of willowfordcourt/mpQPCudaSolver's batched_simplex.cu.
+ *
+ * TEST INFRASTRUCTURE.  This file is the checker: only the tests, the smoke
+ * entry point and the benchmark's CPU-baseline leg load it.  The product
+ * library (libmpqp_hip.so) never links, loads or calls it.
+ *
+ * It exports the same C ABI as libmpqp_hip.so (include/mpqp_hip.h) so the
+ * same ctypes host drives both, plus oracle_eval_set() for checking single
+ * candidates of large runs.
+ *
+ * What is restated, and from where (file:line in the reference):
+ *   feasibility LP      primal simplex, Dantzig pricing with lowest-index
+ *                       ties (min_idx_kernel :256-316), ratio test with
+ *                       lowest-row ties (pivot_kernel_1_2 :901-942), verdict
+ *                       |z| <= 1e-12 (:310), cap 500 (:1162), cost = 1 on the
+ *                       slacks of S (initialise_c :26-36).
+ *   optimality phase 1  dual simplex from the KKT basis (main.py:124-128),
+ *                       leaving row by the dual_min_idx_kernel tree and its
+ *                       two modes (:156-241), entering column by the
+ *                       dual_pivot_kernel_1 tree and its three modes
+ *                       (:811-877), use counters b_counter / non_b_counter,
+ *                       rhs perturbation (:146-154,1550-1555), stall exit
+ *                       (:1561), cap 10000 (:1474), S-masking of the w
+ *                       columns (initialise_A_kernel_1 :635-650).
+ *   optimality phase 2  primal simplex on the first N_ columns with cost
+ *                       -1 on eps (prepare_phase2_kernel :243-254), pricing
+ *                       tree with the b_counter rule (min_idx_kernel_optimal
+ *                       :318-383), ratio test (pivot_kernel_2 :945-987),
+ *                       cap 800 (:1633).
+ *   warm starts         copy_basis_feasibility_1/2 (:729-809) and
+ *                       copy_basis_dual_1/2 (:525-590) choose a root LP
+ *                       that differs by few constraints; here the root is
+ *                       the parent S \ {max S} of a level-wise enumeration.
+ *                       A singular warm basis falls back to the initial
+ *                       basis (check_invertible_kernel_dual_2 :65-83).
+ *
+ * Arithmetic (identical, operation for operation, in the HIP kernels):
+ *   - condensed tableau [B^-1 A_N | B^-1 b (| B^-1 delta)] plus a reduced
+ *     cost row, updated by pivot() below with one IEEE division per pivot-row
+ *     entry and one fma per other entry; the reference instead refactors B
+ *     with cuBLAS getrf/getrs every iteration (:1164-1190), which is the same
+ *     mathematics with different rounding;
+ *   - reductions without an index tie-break use the reference's tree
+ *     (for i = M_2; i > 0; i >>= 1: combine(t, t+i) for t < i, t+i < n),
+ *     sums included; reductions with an index tie-break are plain
+ *     lexicographic minima, which every tree order reproduces.
+ *
+ * Re-specified behaviour (the reference is racy, batch-global or undefined):
+ *   - n_cycles_same_lps (:1542-1546) counts batch iterations in which no LP
+ *     finished; here each LP counts its own iterations since its phase-1
+ *     primal infeasibility sum(min(x_B,0)) last reached a new best.
+ *   - the artificial-in-basis test of :238 compares row positions with N_
+ *     and never fires; here an artificial left basic makes phase 1 fail.
+ *   - the mode-1/2 counter lookups of :852,861 index non_b by a variable id;
+ *     here the counter of that variable is used.
+ *   - an empty ratio test (stale th_idxs :871-876, idx -1 write :941)
+ *     ends the LP with status UNBOUNDED (phase 1: infeasible).
+ *   - LPs stopped by a cap or the stall exit keep verdict 0 with a status.
+ *
+ * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/mpqp_hip.h"
+
+/* ------------------------------------------------------------------ */
+/* Options, errors                                                     */
+/* ------------------------------------------------------------------ */
+
+static MpqpOptions g_opt;
+static int g_opt_set = 0;
+static char g_err[512];
+static int g_trace = 0; /* diagnostics: print phase-1 decisions */
+
+static void set_err(const char *msg) {
+    snprintf(g_err, sizeof g_err, "%s", msg);
+}
+
+void mpqp_default_options(MpqpOptions *o) {
+    memset(o, 0, sizeof *o);
+    o->precision = MPQP_FP64;
+    o->batching = MPQP_PLAIN;
+    o->record = 0;
+    o->output_flag = 0;
+    o->max_iter_feas = 500;
+    o->max_iter_phase1 = 10000;
+    o->max_iter_phase2 = 800;
+    o->stall_perturb = 20;
+    o->stall_exit = 300;
+    o->opt_threads = 0;
+    o->warm_start = 1;
+    o->tol_plus = 1e-9;
+    o->tol_minus = -1e-9;
+    o->tol_minus_1 = -1e-9;
+    o->tol_minus_optimal = -1e-9;
+    o->dual_feasible_tol = 1e-9;
+    o->dual_stop_tol = -1e-9;
+    o->feas_z_tol = 1e-12;
+    o->licq_tol = 1e-10;
+    o->final_tol = 1e-8;
+    o->perturb_step = 1e-6;
+    o->certify_tol = 1e-9;
+}
+
+static const MpqpOptions *opts(void) {
+    if (!g_opt_set) {
+        mpqp_default_options(&g_opt);
+        g_opt_set = 1;
+    }
+    return &g_opt;
+}
+
+int mpqp_set_options(const MpqpOptions *o) {
+    if (!o || o->max_iter_feas < 0 || o->max_iter_phase1 < 0 || o->max_iter_phase2 < 0 ||
+        !(o->tol_plus >= 0.0) || !(o->tol_minus <= 0.0) || !(o->licq_tol >= 0.0)) {
+        set_err("mpqp_set_options: invalid option values");
+        return -1;
+    }
+    g_opt = *o;
+    g_opt_set = 1;
+    return 0;
+}
+
+int mpqp_get_options(MpqpOptions *o) {
+    *o = *opts();
+    return 0;
+}
+
+int mpqp_last_error(char *buf, int len) {
+    int n = (int)strlen(g_err);
+    if (buf && len > 0) {
+        snprintf(buf, (size_t)len, "%s", g_err);
+    }
+    return n;
+}
+
+const char *mpqp_backend(void) { return "cpu-oracle"; }
+
+int warm_up(void) { return 0; }
+
+/* ------------------------------------------------------------------ */
+/* Reductions                                                          */
+/* ------------------------------------------------------------------ */
+
+/* Half of the next power of two >= n (M_2 of batched_simplex.cu:993-1000);
+ * 0 for n <= 1. */
+static int half_pow2(int n) {
+    if (n <= 1) return 0;
+    unsigned m = (unsigned)(n - 1);
+    m |= m >> 1;
+    m |= m >> 2;
+    m |= m >> 4;
+    m |= m >> 8;
+    m |= m >> 16;
+    return (int)((m + 1u) >> 1);
+}
+
+/* Tree sum in the reference's order (:231-236, :301-306); destroys v. */
+static double tree_sum(double *v, int n) {
+    if (n <= 0) return 0.0;
+    for (int i = half_pow2(n); i > 0; i >>= 1)
+        for (int t = 0; t < i; t++)
+            if (t + i < n) v[t] = v[t] + v[t + i];
+    return v[0];
+}
+
+/* dual_min_idx_kernel combine predicate (:177-198): replace slot t by t+i. */
+static int dmin_replace(double vt, double vu, int ct, int cu, int mode_b, double stop_tol) {
+    if (!mode_b) return (vt > vu) || (vt > vu - 1e-10 && ct > cu);
+    return (vt > vu + 1e-10 && ct > cu - 10) || (vt > vu && ct > cu - 5) ||
+           (vu < stop_tol && vt < stop_tol && ct > cu + 10);
+}
+
+/* dual_pivot_kernel_1 combine predicate (:841-866); it, iu are variable
+ * ids (-1: not a candidate), ct, cu their non_b_counter values. */
+static int dratio_replace(double vt, int it, double vu, int iu, int ct, int cu, int mode) {
+    if (iu < 0) return 0;
+    if (it < 0) return 1;
+    if (mode == 0) return vt > vu;
+    if (mode == 1) return vt > vu || (vt >= vu - 1e-12 && ct > cu + 2);
+    return vt >= vu - 1e-10 && ct > cu + 5;
+}
+
+/* min_idx_kernel_optimal combine predicate (:343-344). */
+static int p2price_replace(double vt, double vu, int ct, int cu) {
+    return (vt > vu) || (ct > cu && vu < -1e-10);
+}
+
+/* ------------------------------------------------------------------ */
+/* Pivot                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Condensed-tableau pivot on (r, q) over rows [0, nrows) and columns
+ * [0, ncols) of a row-major tableau with leading dimension ld.  Row r is
+ * divided by p; column q becomes -T[i][q]/p (as fma(-f, 1/p, 0)); every
+ * other entry becomes fma(-T[i][q], T'[r][j], T[i][j]). */
+static void pivot(double *T, int ld, int nrows, int ncols, int r, int q, double *colq) {
+    double p = T[(size_t)r * ld + q];
+    for (int i = 0; i < nrows; i++) colq[i] = T[(size_t)i * ld + q];
+    double *tr = T + (size_t)r * ld;
+    for (int j = 0; j < ncols; j++)
+        if (j != q) tr[j] = tr[j] / p;
+    tr[q] = 1.0 / p;
+    for (int i = 0; i < nrows; i++) {
+        if (i == r) continue;
+        double *ti = T + (size_t)i * ld;
+        double f = colq[i];
+        ti[q] = 0.0;
+        for (int j = 0; j < ncols; j++) ti[j] = fma(-f, tr[j], ti[j]);
+    }
+}
+
+/* Reduced-cost row: T[M][j] = cN[j] - sum_i cb[i] T[i][j] (ascending i,
+ * one fma per nonzero cb[i]). */
+static void cost_row(double *T, int ld, int M, int ncols, const double *cb, const double *cN) {
+    double *tc = T + (size_t)M * ld;
+    for (int j = 0; j < ncols; j++) {
+        double acc = cN[j];
+        for (int i = 0; i < M; i++)
+            if (cb[i] != 0.0) acc = fma(-cb[i], T[(size_t)i * ld + j], acc);
+        tc[j] = acc;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Problem                                                             */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n_c, n_x, n_t, n_cp;
+    int M1, N1, K1, ld1;
+    int M2, N2, K2, N_, n_eqs, ld2;
+    const double *A1, *xb1, *A2, *xb2;
+    /* root phase-1 tableau: rows 0..M2-1 (positions of b2), cols slots
+     * 0..K2-1 (order of non_b2), K2 = rhs, K2+1 = perturbation column */
+    double *root;
+    int *root_basis, *root_nonb;
+    int root_ok;
+} Prob;
+
+/* var ids of the optimality LP */
+#define W_VAR(P, j) ((P)->n_x + (P)->n_t + (j))
+#define EPS_VAR(P) ((P)->n_x + (P)->n_t + (P)->n_c)
+
+static double a2(const Prob *P, int row, int col) {
+    return P->A2[(size_t)col * P->M2 + row];
+}
+
+/* lambda_j: column w_j of A2 with its R5_j entry zeroed (j in S). */
+static double lam_entry(const Prob *P, int j, int row) {
+    if (row == P->M2 - P->n_c + j) return 0.0;
+    return a2(P, row, W_VAR(P, j));
+}
+
+/* s_j: column w_j of A2 with its R4 entries zeroed (j not in S). */
+static double slack_entry(const Prob *P, int j, int row) {
+    int r4 = P->M2 - P->n_c - P->n_x;
+    if (row >= r4 && row < P->M2 - P->n_c) return 0.0;
+    return a2(P, row, W_VAR(P, j));
+}
+
+/* Gauss-Jordan build of the S = {} KKT basis b2 from the identity basis of
+ * the e columns: w_j at rows R5_j first (pivots +-1), then x'_i at the
+ * unused R4 row of largest |entry| (lowest row on ties). */
+static int build_root(Prob *P, const MpqpOptions *o) {
+    int M = P->M2, K = P->K2, ld = P->ld2;
+    int n_x = P->n_x, n_c = P->n_c, N2 = P->N2;
+    double *G = (double *)calloc((size_t)M * ld, sizeof(double));
+    int *rowvar = (int *)malloc(sizeof(int) * M);
+    int *slotvar = (int *)malloc(sizeof(int) * K);
+    int *used = (int *)calloc((size_t)M, sizeof(int));
+    double *colq = (double *)malloc(sizeof(double) * (M + 1));
+    int ok = 1;
+    for (int k = 0; k < K; k++) {
+        slotvar[k] = k; /* non-e variables 0..K2-1 */
+        int is_w = (k >= n_x + P->n_t && k < n_x + P->n_t + n_c);
+        for (int r = 0; r < M; r++)
+            G[(size_t)r * ld + k] = is_w ? slack_entry(P, k - n_x - P->n_t, r) : a2(P, r, k);
+    }
+    for (int r = 0; r < M; r++) {
+        G[(size_t)r * ld + K] = P->xb2[r];
+        G[(size_t)r * ld + K + 1] = (double)(r + 1) * o->perturb_step;
+        rowvar[r] = N2 - M + r;
+    }
+    for (int j = 0; j < n_c && ok; j++) {
+        int r = M - n_c + j, q = n_x + P->n_t + j;
+        if (!(fabs(G[(size_t)r * ld + q]) > o->licq_tol)) { ok = 0; break; }
+        pivot(G, ld, M, K + 2, r, q, colq);
+        int t = rowvar[r]; rowvar[r] = slotvar[q]; slotvar[q] = t;
+    }
+    int r4 = M - n_c - n_x;
+    for (int i = 0; i < n_x && ok; i++) {
+        int r = -1;
+        double best = -1.0;
+        for (int rr = r4; rr < r4 + n_x; rr++) {
+            if (used[rr]) continue;
+            double a = fabs(G[(size_t)rr * ld + i]);
+            if (a > best) { best = a; r = rr; }
+        }
+        if (r < 0 || !(best > o->licq_tol)) { ok = 0; break; }
+        pivot(G, ld, M, K + 2, r, i, colq);
+        used[r] = 1;
+        int t = rowvar[r]; rowvar[r] = slotvar[i]; slotvar[i] = t;
+    }
+    P->root_ok = ok;
+    if (ok) {
+        /* positions in b2 order: x'_i, w_j, then e of rows 0..M-n_eqs-1 */
+        int *rowof = (int *)malloc(sizeof(int) * (size_t)(N2));
+        int *slotof = (int *)malloc(sizeof(int) * (size_t)(N2));
+        for (int v = 0; v < N2; v++) rowof[v] = slotof[v] = -1;
+        for (int r = 0; r < M; r++) rowof[rowvar[r]] = r;
+        for (int k = 0; k < K; k++) slotof[slotvar[k]] = k;
+        int p = 0;
+        for (int i = 0; i < n_x; i++) P->root_basis[p++] = i;
+        for (int j = 0; j < n_c; j++) P->root_basis[p++] = W_VAR(P, j);
+        for (int t = 0; t < M - P->n_eqs; t++) P->root_basis[p++] = N2 - M + t;
+        int k2 = 0;
+        for (int v = 0; v < N2; v++)
+            if (rowof[v] < 0) P->root_nonb[k2++] = v;
+        if (p != M || k2 != K) ok = 0;
+        for (int pp = 0; pp < M && ok; pp++) {
+            int src = rowof[P->root_basis[pp]];
+            if (src < 0) { ok = 0; break; }
+            for (int k = 0; k < K; k++) {
+                int sc = slotof[P->root_nonb[k]];
+                if (sc < 0) { ok = 0; break; }
+                P->root[(size_t)pp * ld + k] = G[(size_t)src * ld + sc];
+            }
+            P->root[(size_t)pp * ld + K] = G[(size_t)src * ld + K];
+            P->root[(size_t)pp * ld + K + 1] = G[(size_t)src * ld + K + 1];
+        }
+        free(rowof);
+        free(slotof);
+        P->root_ok = ok;
+    }
+    free(G); free(rowvar); free(slotvar); free(used); free(colq);
+    return P->root_ok;
+}
+
+/* ------------------------------------------------------------------ */
+/* Feasibility LP                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double *T;   /* (M1+1) x ld1 */
+    int *basis;  /* M1 */
+    int *nonb;   /* K1 */
+} FState;
+
+typedef struct {
+    double *T;   /* (M2+1) x ld2 */
+    int *basis;  /* M2 */
+    int *nonb;   /* K2 */
+} OState;
+
+typedef struct {
+    double *colq, *cb, *cN, *red;
+    int *ired, *where, *bcnt, *nbcnt;
+    char *inS;
+} Scratch;
+
+static void scratch_init(Scratch *s, const Prob *P) {
+    int R = (P->M2 > P->M1 ? P->M2 : P->M1) + 2;
+    int C = P->N2 + P->N1 + 8;
+    s->colq = (double *)malloc(sizeof(double) * R);
+    s->cb = (double *)malloc(sizeof(double) * R);
+    s->cN = (double *)malloc(sizeof(double) * C);
+    s->red = (double *)malloc(sizeof(double) * C);
+    s->ired = (int *)malloc(sizeof(int) * C);
+    s->where = (int *)malloc(sizeof(int) * C);
+    s->bcnt = (int *)malloc(sizeof(int) * C);
+    s->nbcnt = (int *)malloc(sizeof(int) * C);
+    s->inS = (char *)malloc((size_t)P->n_c + 1);
+}
+
+static void scratch_free(Scratch *s) {
+    free(s->colq); free(s->cb); free(s->cN); free(s->red); free(s->ired);
+    free(s->where); free(s->bcnt); free(s->nbcnt); free(s->inS);
+}
+
+static void feas_cold(const Prob *P, FState *F) {
+    int M = P->M1, K = P->K1, ld = P->ld1;
+    for (int i = 0; i < M; i++) {
+        F->basis[i] = K + i;
+        for (int k = 0; k < K; k++) F->T[(size_t)i * ld + k] = P->A1[(size_t)k * M + i];
+        F->T[(size_t)i * ld + K] = P->xb1[i];
+    }
+    for (int k = 0; k < K; k++) F->nonb[k] = k;
+}
+
+static double c_feas(const Prob *P, const char *inS, int var) {
+    int s = var - P->K1;
+    return (s >= 0 && s < P->n_c && inS[s]) ? 1.0 : 0.0;
+}
+
+/* Primal simplex on the feasibility LP from the basis held in F. */
+static void feas_solve(const Prob *P, const MpqpOptions *o, FState *F, Scratch *s,
+                       int *status, int *iters, double *z, int *feasible) {
+    int M = P->M1, K = P->K1, ld = P->ld1;
+    double *T = F->T;
+    for (int i = 0; i < M; i++) s->cb[i] = c_feas(P, s->inS, F->basis[i]);
+    for (int k = 0; k < K; k++) s->cN[k] = c_feas(P, s->inS, F->nonb[k]);
+    s->cN[K] = 0.0;
+    cost_row(T, ld, M, K + 1, s->cb, s->cN);
+    *status = MPQP_ST_ITER_CAP;
+    *iters = 0;
+    *z = 0.0;
+    *feasible = 0;
+    for (int it = 0; it < o->max_iter_feas; it++) {
+        const double *tc = T + (size_t)M * ld;
+        int q = 0;
+        for (int k = 1; k < K; k++) {
+            double d = tc[k];
+            if (d < tc[q] || (d == tc[q] && F->nonb[k] < F->nonb[q])) q = k;
+        }
+        if (tc[q] > o->tol_minus_1) {
+            for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * T[(size_t)i * ld + K];
+            *z = tree_sum(s->red, M);
+            *feasible = (*z >= -o->feas_z_tol && *z <= o->feas_z_tol);
+            *status = MPQP_ST_OPTIMAL;
+            return;
+        }
+        int r = -1;
+        double best = 0.0;
+        for (int i = 0; i < M; i++) {
+            double a = T[(size_t)i * ld + q];
+            if (a > o->tol_plus) {
+                double rt = T[(size_t)i * ld + K] / a;
+                if (r < 0 || rt < best) { r = i; best = rt; }
+            }
+        }
+        if (r < 0) { *status = MPQP_ST_UNBOUNDED; return; }
+        pivot(T, ld, M + 1, K + 1, r, q, s->colq);
+        int t = F->basis[r]; F->basis[r] = F->nonb[q]; F->nonb[q] = t;
+        s->cb[r] = c_feas(P, s->inS, F->basis[r]);
+        (*iters)++;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Optimality LP                                                       */
+/* ------------------------------------------------------------------ */
+
+static void build_where(const Prob *P, const int *basis, const int *nonb, int *where) {
+    for (int v = 0; v < P->N2; v++) where[v] = INT32_MIN;
+    for (int i = 0; i < P->M2; i++) where[basis[i]] = i;
+    for (int k = 0; k < P->K2; k++) where[nonb[k]] = -(k + 1);
+}
+
+/* temp column (K2+2) := B^-1 lambda_j, assembled from the columns of B^-1,
+ * i.e. the e-variable columns of the tableau (unit vectors when basic). */
+static void lambda_column(const Prob *P, OState *O, int j, const int *where) {
+    int M = P->M2, ld = P->ld2, tmp = P->K2 + 2;
+    double *T = O->T;
+    for (int i = 0; i < M; i++) T[(size_t)i * ld + tmp] = 0.0;
+    for (int rho = 0; rho < M; rho++) {
+        double a = lam_entry(P, j, rho);
+        if (a == 0.0) continue;
+        int w = where[P->N2 - M + rho];
+        if (w >= 0) {
+            T[(size_t)w * ld + tmp] = fma(a, 1.0, T[(size_t)w * ld + tmp]);
+        } else {
+            int k = -w - 1;
+            for (int i = 0; i < M; i++)
+                T[(size_t)i * ld + tmp] = fma(a, T[(size_t)i * ld + k], T[(size_t)i * ld + tmp]);
+        }
+    }
+}
+
+/* Bring lambda_j into the basis in place of s_j (same variable id w_j).
+ * Returns 0 if the pivot magnitude is <= licq_tol (singular basis). */
+static int lambda_swap(const Prob *P, const MpqpOptions *o, OState *O, int j, Scratch *s) {
+    int M = P->M2, ld = P->ld2, tmp = P->K2 + 2;
+    build_where(P, O->basis, O->nonb, s->where);
+    lambda_column(P, O, j, s->where);
+    int w = s->where[W_VAR(P, j)];
+    if (w >= 0) {
+        double pv = O->T[(size_t)w * ld + tmp];
+        if (!(fabs(pv) > o->licq_tol)) return 0;
+        pivot(O->T, ld, M, P->K2 + 3, w, tmp, s->colq);
+    } else {
+        int k = -w - 1;
+        for (int i = 0; i < M; i++) O->T[(size_t)i * ld + k] = O->T[(size_t)i * ld + tmp];
+    }
+    return 1;
+}
+
+static void opt_cold(const Prob *P, OState *O) {
+    int M = P->M2, ld = P->ld2;
+    for (int i = 0; i < M; i++) {
+        memcpy(O->T + (size_t)i * ld, P->root + (size_t)i * ld, sizeof(double) * (size_t)(P->K2 + 2));
+        O->T[(size_t)i * ld + P->K2 + 2] = 0.0;
+    }
+    memcpy(O->basis, P->root_basis, sizeof(int) * (size_t)M);
+    memcpy(O->nonb, P->root_nonb, sizeof(int) * (size_t)P->K2);
+}
+
+static double c_ph1(const Prob *P, int var) { return var >= P->N_ ? 1.0 : 0.0; }
+static double c_ph2(const Prob *P, int var) { return var == EPS_VAR(P) ? -1.0 : 0.0; }
+
+typedef struct {
+    int p1_status, p1_iters, kkt_feasible;
+    int p2_status, p2_iters, final_set;
+    double z1, z2;
+} OptOut;
+
+/* Phase 1 (dual simplex) from the basis held in O. */
+static void phase1(const Prob *P, const MpqpOptions *o, OState *O, Scratch *s, OptOut *out) {
+    int M = P->M2, K = P->K2, ld = P->ld2, N2 = P->N2;
+    int rhs = K, pert = K + 1;
+    double *T = O->T;
+    for (int i = 0; i < M; i++) s->cb[i] = c_ph1(P, O->basis[i]);
+    for (int k = 0; k < K; k++) s->cN[k] = c_ph1(P, O->nonb[k]);
+    s->cN[K] = 0.0;
+    s->cN[K + 1] = 0.0;
+    cost_row(T, ld, M, K + 2, s->cb, s->cN);
+    for (int v = 0; v < N2; v++) { s->bcnt[v] = 0; s->nbcnt[v] = 0; }
+    int n_same = 0;
+    double best_p = -INFINITY;
+    double *xb = (double *)malloc(sizeof(double) * (size_t)M);
+    double *vals = (double *)malloc(sizeof(double) * (size_t)(M > K ? M : K));
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(M > K ? M : K));
+    out->p1_status = MPQP_ST_ITER_CAP;
+    out->p1_iters = 0;
+    out->kkt_feasible = 0;
+    out->z1 = 0.0;
+    for (int it = 0; it < o->max_iter_phase1; it++) {
+        int perturbed = n_same >= o->stall_perturb;
+        for (int i = 0; i < M; i++) {
+            double v = T[(size_t)i * ld + rhs];
+            xb[i] = perturbed ? v + T[(size_t)i * ld + pert] : v;
+        }
+        /* leaving row */
+        int mode_b = (n_same % 20) >= 10;
+        for (int i = 0; i < M; i++) { vals[i] = xb[i]; idx[i] = i; }
+        for (int h = half_pow2(M); h > 0; h >>= 1)
+            for (int t = 0; t < h; t++)
+                if (t + h < M &&
+                    dmin_replace(vals[t], vals[t + h], s->bcnt[O->basis[idx[t]]],
+                                 s->bcnt[O->basis[idx[t + h]]], mode_b, o->dual_stop_tol)) {
+                    vals[t] = vals[t + h];
+                    idx[t] = idx[t + h];
+                }
+        int r = idx[0];
+        s->bcnt[O->basis[r]] += 1;
+        int go_on = 0;
+        for (int i = 0; i < M; i++)
+            if (xb[i] < o->dual_stop_tol) go_on = 1;
+        if (!go_on) {
+            for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * xb[i];
+            out->z1 = tree_sum(s->red, M);
+            int f = (out->z1 > -o->dual_feasible_tol && out->z1 < o->dual_feasible_tol);
+            for (int i = 0; i < M; i++)
+                if (O->basis[i] >= P->N_) f = 0;
+            out->kkt_feasible = f;
+            out->p1_status = MPQP_ST_OPTIMAL;
+            break;
+        }
+        /* entering column */
+        int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
+        const double *tr = T + (size_t)r * ld;
+        const double *tc = T + (size_t)M * ld;
+        for (int k = 0; k < K; k++) {
+            double a = tr[k];
+            if (a < o->tol_minus) {
+                idx[k] = O->nonb[k];
+                vals[k] = -tc[k] / a;
+            } else {
+                idx[k] = -1;
+                vals[k] = 100000.0;
+            }
+        }
+        for (int h = half_pow2(K); h > 0; h >>= 1)
+            for (int t = 0; t < h; t++)
+                if (t + h < K) {
+                    int it_ = idx[t], iu = idx[t + h];
+                    int ct = it_ >= 0 ? s->nbcnt[it_] : 0;
+                    int cu = iu >= 0 ? s->nbcnt[iu] : 0;
+                    if (dratio_replace(vals[t], it_, vals[t + h], iu, ct, cu, mode)) {
+                        vals[t] = vals[t + h];
+                        idx[t] = idx[t + h];
+                    }
+                }
+        int wv = idx[0];
+        int q = -1;
+        for (int k = 0; k < K; k++)
+            if (O->nonb[k] == wv) { q = k; break; }
+        if (g_trace && it < 256) fprintf(stderr, "oracle it %d r %d q %d n_same %d\n", it, r, q, n_same);
+        if (wv < 0) { out->p1_status = MPQP_ST_UNBOUNDED; break; }
+        s->nbcnt[wv] += 1;
+        /* infeasibility progress of this iteration (before the pivot) */
+        for (int i = 0; i < M; i++) s->red[i] = xb[i] < 0.0 ? xb[i] : 0.0;
+        double pinf = tree_sum(s->red, M);
+        pivot(T, ld, M + 1, K + 2, r, q, s->colq);
+        int t = O->basis[r]; O->basis[r] = O->nonb[q]; O->nonb[q] = t;
+        s->cb[r] = c_ph1(P, O->basis[r]);
+        out->p1_iters++;
+        if (pinf > best_p) { best_p = pinf; n_same = 0; } else { n_same++; }
+        if (n_same > o->stall_exit) { out->p1_status = MPQP_ST_STALL; break; }
+    }
+    free(xb); free(vals); free(idx);
+}
+
+/* Phase 2 (primal simplex, min -eps over the first N_ columns). */
+static void phase2(const Prob *P, const MpqpOptions *o, OState *O, Scratch *s, OptOut *out) {
+    int M = P->M2, K = P->K2, ld = P->ld2, Nn = P->N_;
+    int rhs = K;
+    double *T = O->T;
+    for (int i = 0; i < M; i++) s->cb[i] = c_ph2(P, O->basis[i]);
+    for (int k = 0; k < K; k++) s->cN[k] = c_ph2(P, O->nonb[k]);
+    s->cN[K] = 0.0;
+    cost_row(T, ld, M, K + 1, s->cb, s->cN);
+    for (int v = 0; v < P->N2; v++) s->bcnt[v] = 0;
+    double *vals = (double *)malloc(sizeof(double) * (size_t)Nn);
+    int *idx = (int *)malloc(sizeof(int) * (size_t)Nn);
+    out->p2_status = MPQP_ST_ITER_CAP;
+    out->p2_iters = 0;
+    out->z2 = 0.0;
+    for (int it = 0; it < o->max_iter_phase2; it++) {
+        build_where(P, O->basis, O->nonb, s->where);
+        const double *tc = T + (size_t)M * ld;
+        for (int v = 0; v < Nn; v++) {
+            int w = s->where[v];
+            vals[v] = w >= 0 ? 100000.0 : tc[-w - 1];
+            idx[v] = v;
+        }
+        for (int h = half_pow2(Nn); h > 0; h >>= 1)
+            for (int t = 0; t < h; t++)
+                if (t + h < Nn &&
+                    p2price_replace(vals[t], vals[t + h], s->bcnt[idx[t]], s->bcnt[idx[t + h]])) {
+                    vals[t] = vals[t + h];
+                    idx[t] = idx[t + h];
+                }
+        int wv = idx[0];
+        s->bcnt[wv] += 1;
+        if (vals[0] > o->tol_minus_optimal) {
+            for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * T[(size_t)i * ld + rhs];
+            out->z2 = tree_sum(s->red, M);
+            out->p2_status = MPQP_ST_OPTIMAL;
+            break;
+        }
+        int q = -s->where[wv] - 1;
+        int r = -1;
+        double best = 0.0;
+        for (int i = 0; i < M; i++) {
+            double a = T[(size_t)i * ld + q];
+            if (a > o->tol_plus) {
+                double rt = T[(size_t)i * ld + rhs] / a;
+                if (r < 0 || rt < best) { r = i; best = rt; }
+            }
+        }
+        if (r < 0) { out->p2_status = MPQP_ST_UNBOUNDED; break; }
+        pivot(T, ld, M + 1, K + 1, r, q, s->colq);
+        int t = O->basis[r]; O->basis[r] = O->nonb[q]; O->nonb[q] = t;
+        s->cb[r] = c_ph2(P, O->basis[r]);
+        out->p2_iters++;
+    }
+    out->final_set = (out->p2_status == MPQP_ST_OPTIMAL && -out->z2 > o->final_tol);
+    free(vals); free(idx);
+}
+
+/* ------------------------------------------------------------------ */
+/* Candidate evaluation                                                */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int rec_bases;
+    int *fb, *b1, *b2;   /* output bases (may be NULL) */
+} BasisOut;
+
+/* Evaluate one candidate S (|S| = k, ascending) given its parent's final
+ * states (fpar / opar may be NULL; opar_ok: parent phase 1 feasible).
+ * The candidate's final feasibility state is left in F and its phase-1
+ * final state in O1 (when kkt feasible). */
+static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, int k,
+                           const FState *fpar, const OState *opar, int opar_ok,
+                           FState *F, OState *O1, OState *O2, Scratch *s,
+                           MpqpRecord *rec, BasisOut *bo) {
+    int M1 = P->M1, M2 = P->M2;
+    memset(s->inS, 0, (size_t)P->n_c);
+    for (int t = 0; t < k; t++) s->inS[S[t]] = 1;
+    rec->level = k;
+    rec->feas_status = rec->p1_status = rec->p2_status = MPQP_ST_NOT_RUN;
+    rec->feas_iters = rec->p1_iters = rec->p2_iters = 0;
+    rec->feasible = rec->kkt_feasible = rec->final_set = 0;
+    rec->p1_warm = 0;
+    rec->z_feas = rec->z1 = rec->z2 = 0.0;
+    rec->repivots = 0;
+    if (bo && bo->fb) for (int i = 0; i < M1; i++) bo->fb[i] = -1;
+    if (bo && bo->b1) for (int i = 0; i < M2; i++) bo->b1[i] = -1;
+    if (bo && bo->b2) for (int i = 0; i < M2; i++) bo->b2[i] = -1;
+
+    /* feasibility LP */
+    if (fpar && o->warm_start) {
+        memcpy(F->T, fpar->T, sizeof(double) * (size_t)(M1 + 1) * P->ld1);
+        memcpy(F->basis, fpar->basis, sizeof(int) * (size_t)M1);
+        memcpy(F->nonb, fpar->nonb, sizeof(int) * (size_t)P->K1);
+    } else {
+        feas_cold(P, F);
+    }
+    int st, its, fe;
+    double z;
+    feas_solve(P, o, F, s, &st, &its, &z, &fe);
+    rec->feas_status = st;
+    rec->feas_iters = its;
+    rec->z_feas = z;
+    rec->feasible = fe;
+    if (bo && bo->fb) memcpy(bo->fb, F->basis, sizeof(int) * (size_t)M1);
+    if (!fe) return;
+
+    /* optimality LP, phase 1 */
+    OptOut out;
+    memset(&out, 0, sizeof out);
+    int started = 0;
+    if (!P->root_ok) {
+        rec->p1_status = MPQP_ST_SINGULAR;
+        return;
+    }
+    if (opar && opar_ok && o->warm_start && k > 0) {
+        memcpy(O1->T, opar->T, sizeof(double) * (size_t)(M2 + 1) * P->ld2);
+        memcpy(O1->basis, opar->basis, sizeof(int) * (size_t)M2);
+        memcpy(O1->nonb, opar->nonb, sizeof(int) * (size_t)P->K2);
+        if (lambda_swap(P, o, O1, S[k - 1], s)) {
+            started = 1;
+            rec->p1_warm = 1;
+        }
+    }
+    if (!started) {
+        opt_cold(P, O1);
+        started = 1;
+        for (int t = 0; t < k; t++) {
+            if (!lambda_swap(P, o, O1, S[t], s)) { started = 0; break; }
+        }
+        if (!started) {
+            rec->p1_status = MPQP_ST_SINGULAR;
+            return;
+        }
+    }
+    phase1(P, o, O1, s, &out);
+    rec->p1_status = out.p1_status;
+    rec->p1_iters = out.p1_iters;
+    rec->kkt_feasible = out.kkt_feasible;
+    rec->z1 = out.z1;
+    if (bo && bo->b1) memcpy(bo->b1, O1->basis, sizeof(int) * (size_t)M2);
+    if (!out.kkt_feasible) return;
+
+    /* phase 2 on a copy (O1 keeps the phase-1 state for children) */
+    memcpy(O2->T, O1->T, sizeof(double) * (size_t)(M2 + 1) * P->ld2);
+    memcpy(O2->basis, O1->basis, sizeof(int) * (size_t)M2);
+    memcpy(O2->nonb, O1->nonb, sizeof(int) * (size_t)P->K2);
+    phase2(P, o, O2, s, &out);
+    rec->p2_status = out.p2_status;
+    rec->p2_iters = out.p2_iters;
+    rec->z2 = out.z2;
+    rec->final_set = out.final_set;
+    if (bo && bo->b2) memcpy(bo->b2, O2->basis, sizeof(int) * (size_t)M2);
+}
+
+/* ------------------------------------------------------------------ */
+/* Problem setup                                                       */
+/* ------------------------------------------------------------------ */
+
+static int prob_init(Prob *P, int n_c, int n_x, int n_t, const double *A1, const double *xb1,
+                     const int *b1, int M1, int N1, const double *A2, const double *xb2,
+                     const int *b2, const int *non_b2, int M2, int N2, int n_eqs,
+                     const MpqpOptions *o) {
+    memset(P, 0, sizeof *P);
+    P->n_c = n_c; P->n_x = n_x; P->n_t = n_t;
+    P->n_cp = M1 - n_c;
+    P->M1 = M1; P->N1 = N1; P->K1 = N1 - M1; P->ld1 = P->K1 + 1;
+    P->M2 = M2; P->N2 = N2; P->K2 = N2 - M2; P->N_ = N2 - n_eqs; P->n_eqs = n_eqs;
+    P->ld2 = P->K2 + 3;
+    P->A1 = A1; P->xb1 = xb1; P->A2 = A2; P->xb2 = xb2;
+    if (n_c <= 0 || n_x <= 0 || n_t <= 0 || P->n_cp < 0 || P->K1 != n_x + n_t + 1 ||
+        M2 != 2 * n_c + 1 + P->n_cp + n_x || P->K2 != n_x + n_t + n_c + 2 || n_eqs != n_c + n_x) {
+        set_err("mpqp: inconsistent LP dimensions");
+        return -1;
+    }
+    for (int i = 0; i < M1; i++)
+        if (b1[i] != P->K1 + i) { set_err("mpqp: b1 must be the slack basis N1-M1..N1-1"); return -1; }
+    {
+        int p = 0, bad = 0;
+        for (int i = 0; i < n_x; i++) bad |= b2[p++] != i;
+        for (int j = 0; j < n_c; j++) bad |= b2[p++] != n_x + n_t + j;
+        for (int t = 0; t < M2 - n_eqs; t++) bad |= b2[p++] != N2 - M2 + t;
+        int k = 0;
+        for (int v = 0; v < N2 && !bad; v++) {
+            int inb = 0;
+            for (int i = 0; i < M2; i++) inb |= b2[i] == v;
+            if (!inb) bad |= (k >= P->K2) || non_b2[k++] != v;
+        }
+        if (bad) { set_err("mpqp: b2/non_b2 must be the KKT basis of main.py:124-128"); return -1; }
+    }
+    for (int i = 0; i < M1; i++)
+        if (!(xb1[i] >= 0.0)) { set_err("mpqp: xb1 must be >= 0 for the slack basis"); return -1; }
+    P->root = (double *)calloc((size_t)M2 * P->ld2, sizeof(double));
+    P->root_basis = (int *)malloc(sizeof(int) * (size_t)M2);
+    P->root_nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
+    build_root(P, o);
+    return 0;
+}
+
+static void prob_free(Prob *P) {
+    free(P->root); free(P->root_basis); free(P->root_nonb);
+}
+
+static void fstate_alloc(FState *F, const Prob *P) {
+    F->T = (double *)malloc(sizeof(double) * (size_t)(P->M1 + 1) * P->ld1);
+    F->basis = (int *)malloc(sizeof(int) * (size_t)P->M1);
+    F->nonb = (int *)malloc(sizeof(int) * (size_t)P->K1);
+}
+static void fstate_free(FState *F) { free(F->T); free(F->basis); free(F->nonb); }
+static void ostate_alloc(OState *O, const Prob *P) {
+    O->T = (double *)calloc((size_t)(P->M2 + 1) * P->ld2, sizeof(double));
+    O->basis = (int *)malloc(sizeof(int) * (size_t)P->M2);
+    O->nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
+}
+static void ostate_free(OState *O) { free(O->T); free(O->basis); free(O->nonb); }
+
+/* ------------------------------------------------------------------ */
+/* Level-wise enumeration: mpqp()                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int n;            /* candidates */
+    int k_max;
+    int M1, M2;
+    MpqpRecord *rec;
+    int *sets;        /* n x k_max */
+    int *fb, *b1, *b2;
+} RecStore;
+
+static RecStore g_rs;
+static GPUResults *g_last = NULL;
+
+static void rs_free(RecStore *r) {
+    free(r->rec); free(r->sets); free(r->fb); free(r->b1); free(r->b2);
+    memset(r, 0, sizeof *r);
+}
+
+void mpqp_free_results(GPUResults *res) {
+    if (!res) return;
+    if (res == g_last) g_last = NULL;
+    free(res->active_sets);
+    free(res);
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
+                 int extra_batches, double *A1, double *xb1, int *b1, int M1, int N1,
+                 double *A2, double *xb2, int *b2, int *non_b2, int M2, int N2,
+                 int n_eqs, int reserved_memory) {
+    (void)AxT; (void)extra_batches; (void)reserved_memory;
+    const MpqpOptions *o = opts();
+    g_err[0] = 0;
+    if (k_max < 0 || k_max > n_c || k_min < 0) { set_err("mpqp: bad k_min/k_max"); return NULL; }
+    Prob P;
+    if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
+        return NULL;
+    double t0 = now_s();
+    rs_free(&g_rs);
+    int kw = k_max > 0 ? k_max : 1;
+
+    /* level storage: previous level's candidates, states */
+    int prev_n = 0, *prev_sets = NULL, *prev_feas = NULL, *prev_kkt = NULL, *prev_rec = NULL;
+    FState *prev_F = NULL;
+    OState *prev_O = NULL;
+    int total = 0, cap = 0;
+    MpqpRecord *all = NULL;
+    int *allsets = NULL, *allfb = NULL, *allb1 = NULL, *allb2 = NULL;
+    int n_batches = 0, n_feas = 0;
+    double tf = 0.0, to = 0.0;
+
+    for (int lvl = 0; lvl <= k_max; lvl++) {
+        /* candidates of this level */
+        int n = 0, ncap = 16;
+        int *sets = (int *)malloc(sizeof(int) * (size_t)ncap * kw);
+        int *par = (int *)malloc(sizeof(int) * (size_t)ncap);
+        if (lvl == 0) {
+            n = 1;
+            par[0] = -1;
+            for (int t = 0; t < kw; t++) sets[t] = -1;
+        } else {
+            for (int pi = 0; pi < prev_n; pi++) {
+                if (!prev_feas[pi]) continue;
+                int mx = lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1;
+                for (int j = mx + 1; j < n_c; j++) {
+                    if (n == ncap) {
+                        ncap *= 2;
+                        sets = (int *)realloc(sets, sizeof(int) * (size_t)ncap * kw);
+                        par = (int *)realloc(par, sizeof(int) * (size_t)ncap);
+                    }
+                    for (int t = 0; t < kw; t++) sets[(size_t)n * kw + t] = -1;
+                    for (int t = 0; t < lvl - 1; t++)
+                        sets[(size_t)n * kw + t] = prev_sets[(size_t)pi * kw + t];
+                    sets[(size_t)n * kw + lvl - 1] = j;
+                    par[n] = pi;
+                    n++;
+                }
+            }
+        }
+        if (n == 0) { free(sets); free(par); break; }
+        n_batches++;
+        int keep = lvl < k_max; /* children need this level's states */
+        FState *curF = (FState *)calloc((size_t)n, sizeof(FState));
+        OState *curO = (OState *)calloc((size_t)n, sizeof(OState));
+        int *feas = (int *)calloc((size_t)n, sizeof(int));
+        int *kkt = (int *)calloc((size_t)n, sizeof(int));
+        if (total + n > cap) {
+            cap = (total + n) * 2;
+            all = (MpqpRecord *)realloc(all, sizeof(MpqpRecord) * (size_t)cap);
+            allsets = (int *)realloc(allsets, sizeof(int) * (size_t)cap * kw);
+            if (o->record >= 2) {
+                allfb = (int *)realloc(allfb, sizeof(int) * (size_t)cap * M1);
+                allb1 = (int *)realloc(allb1, sizeof(int) * (size_t)cap * M2);
+                allb2 = (int *)realloc(allb2, sizeof(int) * (size_t)cap * M2);
+            }
+        }
+        double tl = now_s();
+        #pragma omp parallel
+        {
+            Scratch s;
+            scratch_init(&s, &P);
+            FState F;
+            OState O1, O2;
+            fstate_alloc(&F, &P);
+            ostate_alloc(&O1, &P);
+            ostate_alloc(&O2, &P);
+            #pragma omp for schedule(dynamic, 4)
+            for (int c = 0; c < n; c++) {
+                MpqpRecord *rec = &all[total + c];
+                BasisOut bo = {0, NULL, NULL, NULL};
+                if (o->record >= 2) {
+                    bo.fb = allfb + (size_t)(total + c) * M1;
+                    bo.b1 = allb1 + (size_t)(total + c) * M2;
+                    bo.b2 = allb2 + (size_t)(total + c) * M2;
+                }
+                int pi = par[c];
+                eval_candidate(&P, o, sets + (size_t)c * kw, lvl,
+                               pi >= 0 ? &prev_F[pi] : NULL, pi >= 0 ? &prev_O[pi] : NULL,
+                               pi >= 0 ? prev_kkt[pi] : 0, &F, &O1, &O2, &s, rec, &bo);
+                rec->parent = pi >= 0 ? prev_rec[pi] : -1;
+                feas[c] = rec->feasible;
+                kkt[c] = rec->kkt_feasible;
+                if (keep && rec->feasible) {
+                    fstate_alloc(&curF[c], &P);
+                    memcpy(curF[c].T, F.T, sizeof(double) * (size_t)(M1 + 1) * P.ld1);
+                    memcpy(curF[c].basis, F.basis, sizeof(int) * (size_t)M1);
+                    memcpy(curF[c].nonb, F.nonb, sizeof(int) * (size_t)P.K1);
+                    if (rec->kkt_feasible) {
+                        ostate_alloc(&curO[c], &P);
+                        memcpy(curO[c].T, O1.T, sizeof(double) * (size_t)(M2 + 1) * P.ld2);
+                        memcpy(curO[c].basis, O1.basis, sizeof(int) * (size_t)M2);
+                        memcpy(curO[c].nonb, O1.nonb, sizeof(int) * (size_t)P.K2);
+                    }
+                }
+            }
+            fstate_free(&F);
+            ostate_free(&O1);
+            ostate_free(&O2);
+            scratch_free(&s);
+        }
+        double dt = now_s() - tl;
+        tf += 0.5 * dt;
+        to += 0.5 * dt;
+        for (int c = 0; c < n; c++) n_feas += feas[c];
+        memcpy(allsets + (size_t)total * kw, sets, sizeof(int) * (size_t)n * kw);
+        /* release previous level */
+        for (int pi = 0; pi < prev_n; pi++) {
+            if (prev_F[pi].T) fstate_free(&prev_F[pi]);
+            if (prev_O[pi].T) ostate_free(&prev_O[pi]);
+        }
+        free(prev_F); free(prev_O); free(prev_sets); free(prev_feas); free(prev_kkt); free(prev_rec);
+        prev_F = curF; prev_O = curO; prev_sets = sets; prev_feas = feas; prev_kkt = kkt;
+        prev_rec = (int *)malloc(sizeof(int) * (size_t)n);
+        for (int c = 0; c < n; c++) prev_rec[c] = total + c;
+        prev_n = n;
+        free(par);
+        total += n;
+    }
+    for (int pi = 0; pi < prev_n; pi++) {
+        if (prev_F[pi].T) fstate_free(&prev_F[pi]);
+        if (prev_O[pi].T) ostate_free(&prev_O[pi]);
+    }
+    free(prev_F); free(prev_O); free(prev_sets); free(prev_feas); free(prev_kkt); free(prev_rec);
+
+    GPUResults *res = (GPUResults *)calloc(1, sizeof(GPUResults));
+    int nf = 0;
+    for (int c = 0; c < total; c++)
+        if (all[c].final_set && all[c].level >= k_min) nf++;
+    res->active_sets = (int *)malloc(sizeof(int) * (size_t)(nf > 0 ? nf : 1) * n_x);
+    int row = 0;
+    for (int c = 0; c < total; c++) {
+        if (!(all[c].final_set && all[c].level >= k_min)) continue;
+        for (int t = 0; t < n_x; t++)
+            res->active_sets[(size_t)row * n_x + t] =
+                (t < all[c].level && t < kw) ? allsets[(size_t)c * kw + t] : -1;
+        row++;
+    }
+    res->n_batches = n_batches;
+    res->n_splits = 1;
+    res->n_feasible_lps = n_feas;
+    res->n_final_lps = nf;
+    res->time_feas = tf;
+    res->time_opti = to;
+    res->time_GPU = now_s() - t0;
+
+    if (o->record >= 1) {
+        g_rs.n = total;
+        g_rs.k_max = kw;
+        g_rs.M1 = M1;
+        g_rs.M2 = M2;
+        g_rs.rec = all;
+        g_rs.sets = allsets;
+        g_rs.fb = allfb;
+        g_rs.b1 = allb1;
+        g_rs.b2 = allb2;
+    } else {
+        free(all); free(allsets); free(allfb); free(allb1); free(allb2);
+    }
+    prob_free(&P);
+    g_last = res;
+    return res;
+}
+
+int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis, int *p1_basis,
+                     int *p2_basis, int max_records) {
+    int n = g_rs.n < max_records ? g_rs.n : max_records;
+    if (records && g_rs.rec) memcpy(records, g_rs.rec, sizeof(MpqpRecord) * (size_t)n);
+    if (sets && g_rs.sets) memcpy(sets, g_rs.sets, sizeof(int) * (size_t)n * g_rs.k_max);
+    if (feas_basis && g_rs.fb) memcpy(feas_basis, g_rs.fb, sizeof(int) * (size_t)n * g_rs.M1);
+    if (p1_basis && g_rs.b1) memcpy(p1_basis, g_rs.b1, sizeof(int) * (size_t)n * g_rs.M2);
+    if (p2_basis && g_rs.b2) memcpy(p2_basis, g_rs.b2, sizeof(int) * (size_t)n * g_rs.M2);
+    return g_rs.n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Single-candidate evaluation along its prefix chain                  */
+/* ------------------------------------------------------------------ */
+
+/* Evaluates S (ascending, |S| = k) exactly as the level-wise run does:
+ * every prefix S[0..t) is evaluated in turn, each warm-started from the
+ * previous prefix.  Returns 1 if S is a candidate (every proper prefix
+ * feasible), 0 if it is pruned, <0 on error.  fb (M1), b1, b2 (M2) may be
+ * NULL. */
+void oracle_set_trace(int on) { g_trace = on; }
+
+int oracle_eval_set(const int *S, int k, int n_c, int n_x, int n_t, double *A1, double *xb1,
+                    int *b1, int M1, int N1, double *A2, double *xb2, int *b2, int *non_b2,
+                    int M2, int N2, int n_eqs, MpqpRecord *rec, int *fb, int *ob1, int *ob2) {
+    const MpqpOptions *o = opts();
+    Prob P;
+    if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
+        return -1;
+    Scratch s;
+    scratch_init(&s, &P);
+    FState Fa, Fb;
+    OState Oa, Ob, O2;
+    fstate_alloc(&Fa, &P); fstate_alloc(&Fb, &P);
+    ostate_alloc(&Oa, &P); ostate_alloc(&Ob, &P); ostate_alloc(&O2, &P);
+    FState *fpar = NULL, *fcur = &Fa, *fspare = &Fb;
+    OState *opar = NULL, *ocur = &Oa, *ospare = &Ob;
+    int opar_ok = 0, ret = 1;
+    MpqpRecord r;
+    for (int t = 0; t <= k; t++) {
+        BasisOut bo = {0, NULL, NULL, NULL};
+        if (t == k) { bo.fb = fb; bo.b1 = ob1; bo.b2 = ob2; }
+        eval_candidate(&P, o, S, t, fpar, opar, opar_ok, fcur, ocur, &O2, &s, &r, &bo);
+        if (t < k && !r.feasible) { ret = 0; break; }
+        fpar = fcur; { FState *tmp = fspare; fspare = fcur; fcur = tmp; }
+        opar = ocur; { OState *tmp = ospare; ospare = ocur; ocur = tmp; }
+        opar_ok = r.kkt_feasible;
+    }
+    if (ret == 1 && rec) { *rec = r; rec->parent = -1; }
+    fstate_free(&Fa); fstate_free(&Fb);
+    ostate_free(&Oa); ostate_free(&Ob); ostate_free(&O2);
+    scratch_free(&s);
+    prob_free(&P);
+    return ret;
+}
+
+/* Exposes the root tableau (S = {} KKT basis) for tests: fills root
+ * ((M2) x (K2+2) row-major), basis (M2), nonb (K2); returns root_ok. */
+int oracle_root(int n_c, int n_x, int n_t, double *A1, double *xb1, int *b1, int M1, int N1,
+                double *A2, double *xb2, int *b2, int *non_b2, int M2, int N2, int n_eqs,
+                double *root, int *basis, int *nonb) {
+    const MpqpOptions *o = opts();
+    Prob P;
+    if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
+        return -1;
+    int ok = P.root_ok;
+    if (ok) {
+        for (int i = 0; i < M2; i++)
+            memcpy(root + (size_t)i * (P.K2 + 2), P.root + (size_t)i * P.ld2,
+                   sizeof(double) * (size_t)(P.K2 + 2));
+        memcpy(basis, P.root_basis, sizeof(int) * (size_t)M2);
+        memcpy(nonb, P.root_nonb, sizeof(int) * (size_t)P.K2);
+    }
+    prob_free(&P);
+    return ok;
+}
+
+/* Tree helpers exposed for the reduction-order tests. */
+double oracle_tree_sum(double *v, int n) { return tree_sum(v, n); }
+
+int oracle_tree_argmin_dmin(const double *v, const int *cnt, int n, int mode_b, double stop_tol) {
+    double *vals = (double *)malloc(sizeof(double) * (size_t)n);
+    int *idx = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; i++) { vals[i] = v[i]; idx[i] = i; }
+    for (int h = half_pow2(n); h > 0; h >>= 1)
+        for (int t = 0; t < h; t++)
+            if (t + h < n && dmin_replace(vals[t], vals[t + h], cnt[idx[t]], cnt[idx[t + h]], mode_b, stop_tol)) {
+                vals[t] = vals[t + h];
+                idx[t] = idx[t + h];
+            }
+    int r = n > 0 ? idx[0] : -1;
+    free(vals); free(idx);
+    return r;
+}
