@@ -141,6 +141,30 @@ void mpqp_free_results(GPUResults *res);
 int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis,
                      int *p1_basis, int *p2_basis, int max_records);
 
+/* Work and time of the last mpqp() call.  Kernel times are HIP-event
+ * intervals on the library's stream, summed over launches.  Algorithmic
+ * bytes count one read and one write of every tableau entry a pivot updates
+ * plus the pivot row and column, 2(M+1)(K+1)s + (M+K+2)s per pivot with s
+ * the element size and (M, K) = (M1, K1) feasibility, (M2, K2) phase 1,
+ * (M2, N_-M2) phase 2; flops are 2(M+1)(K+1) per pivot. */
+typedef struct MpqpStats {
+    int levels;
+    int launches_feas, launches_opt;
+    int reserved;
+    long long candidates;     /* LPs checked: one feasibility LP each */
+    long long opt_lps;        /* optimality LPs run (feasible candidates) */
+    long long pivots_feas, pivots_p1, pivots_p2;
+    double ms_feas, ms_opt;   /* summed kernel time */
+    double bytes_feas, bytes_opt;
+    double flops_feas, flops_opt;
+} MpqpStats;
+
+int mpqp_get_stats(MpqpStats *stats);
+
+/* Measures the LDS read bandwidth of the device (bytes/s) with a
+ * ds_read_b128 streaming kernel over all CUs; <0 on error. */
+double mpqp_measure_lds_bandwidth(void);
+
 /* Copies the last error message; returns its length (0 if none). */
 int mpqp_last_error(char *buf, int len);
 

@@ -46,6 +46,7 @@ struct Ctx {
     int kw = 1, M1 = 0, M2 = 0;
     long long lds_limit = 0;  // bytes of LDS one workgroup may allocate
     int n_cu = 0;
+    MpqpStats stats{};
 };
 
 Ctx &ctx() {
@@ -199,6 +200,46 @@ std::string validate(int n_c, int n_x, int n_t, const int *b1, int M1, int N1, c
     return "";
 }
 
+// LDS read bandwidth: every lane streams 16-byte ds_read_b128 loads over a
+// 64 KiB LDS buffer; the result is folded so the loads stay live.
+__global__ __launch_bounds__(256) void lds_bw_kernel(float4 *out, int iters) {
+    __shared__ float4 buf[4096];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 4096; i += 256) buf[i] = make_float4((float)i, 1.f, 2.f, 3.f);
+    __syncthreads();
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int idx = tid;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const float4 v = buf[(idx + u * 256) & 4095];
+            acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+        }
+        idx = (idx + 1) & 4095;
+    }
+    if (acc.x == -1.f) out[blockIdx.x * 256 + tid] = acc;
+}
+
+double measure_lds(hipStream_t st, int n_cu) {
+    const int blocks = n_cu * 8, iters = 2048;
+    DBuf out(sizeof(float4) * (size_t)blocks * 256);
+    hipEvent_t a, b;
+    HIP_OK(hipEventCreate(&a));
+    HIP_OK(hipEventCreate(&b));
+    hipLaunchKernelGGL(lds_bw_kernel, dim3(blocks), dim3(256), 0, st, out.as<float4>(), 64);
+    HIP_OK(hipEventRecord(a, st));
+    hipLaunchKernelGGL(lds_bw_kernel, dim3(blocks), dim3(256), 0, st, out.as<float4>(), iters);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(b, st));
+    HIP_OK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double bytes = (double)blocks * 256 * iters * 16 * 16.0;
+    return bytes / (ms * 1e-3);
+}
+
 double now_s() {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -222,6 +263,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     hipStream_t st = C.stream;
     const int K1 = N1 - M1, K2 = N2 - M2;
     const int kw = std::max(k_max, 1);
+    C.stats = MpqpStats{};
 
     // problem data
     DBuf dA1(sizeof(double) * (size_t)M1 * N1), dxb1(sizeof(double) * M1);
@@ -365,6 +407,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, e0, e1));
         tf += ms * 1e-3;
+        C.stats.ms_feas += ms;
 
         std::vector<int> list, p1par(n, -1);
         for (int c = 0; c < n; ++c) {
@@ -409,6 +452,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             HIP_OK(hipStreamSynchronize(st));
             HIP_OK(hipEventElapsedTime(&ms, e1, e2));
             to += ms * 1e-3;
+            C.stats.ms_opt += ms;
             if (oa.trace) {
                 std::vector<int> ht(4 * 256);
                 HIP_OK(hipMemcpy(ht.data(), oa.trace, ht.size() * 4, hipMemcpyDeviceToHost));
@@ -419,6 +463,31 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             }
         }
 
+        {
+            MpqpStats &S = C.stats;
+            S.levels += 1;
+            S.launches_feas += 1;
+            S.launches_opt += nf > 0 ? 1 : 0;
+            S.candidates += n;
+            S.opt_lps += nf;
+            long long pf = 0, p1 = 0, p2 = 0;
+            for (int c = 0; c < n; ++c) {
+                pf += hrec[c].feas_iters;
+                p1 += hrec[c].p1_iters;
+                p2 += hrec[c].p2_iters;
+            }
+            S.pivots_feas += pf;
+            S.pivots_p1 += p1;
+            S.pivots_p2 += p2;
+            const double s8 = 8.0;
+            auto bytes = [&](double M, double K) { return 2.0 * (M + 1) * (K + 1) * s8 + (M + K + 2) * s8; };
+            auto flops = [&](double M, double K) { return 2.0 * (M + 1) * (K + 1); };
+            const double Kp2 = (double)(N2 - n_eqs - M2);
+            S.bytes_feas += pf * bytes(M1, K1);
+            S.flops_feas += pf * flops(M1, K1);
+            S.bytes_opt += p1 * bytes(M2, K2) + p2 * bytes(M2, Kp2);
+            S.flops_opt += p1 * flops(M2, K2) + p2 * flops(M2, Kp2);
+        }
         const int base = (int)all.size();
         std::vector<int> kkt(n);
         for (int c = 0; c < n; ++c) {
@@ -579,6 +648,24 @@ int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis, int *p1_ba
     if (p1_basis && n && !C.b1.empty()) std::memcpy(p1_basis, C.b1.data(), sizeof(int) * (size_t)n * C.M2);
     if (p2_basis && n && !C.b2.empty()) std::memcpy(p2_basis, C.b2.data(), sizeof(int) * (size_t)n * C.M2);
     return total;
+}
+
+int mpqp_get_stats(MpqpStats *stats) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!stats) return -1;
+    *stats = ctx().stats;
+    return 0;
+}
+
+double mpqp_measure_lds_bandwidth(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    try {
+        ensure_ready();
+        return measure_lds(ctx().stream, ctx().n_cu);
+    } catch (const HipError &e) {
+        ctx().err = e.msg;
+        return -1.0;
+    }
 }
 
 int mpqp_last_error(char *buf, int len) {

@@ -25,6 +25,7 @@ __all__ = [
     "GPUResults",
     "MpqpOptions",
     "MpqpRecord",
+    "MpqpStats",
     "MpqpLib",
     "MpqpRun",
     "library_path",
@@ -105,6 +106,29 @@ class MpqpRecord(ctypes.Structure):
         ("repivots", ctypes.c_int),
         ("reserved", ctypes.c_int),
     ]
+
+
+class MpqpStats(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int),
+        ("launches_feas", ctypes.c_int),
+        ("launches_opt", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("candidates", ctypes.c_longlong),
+        ("opt_lps", ctypes.c_longlong),
+        ("pivots_feas", ctypes.c_longlong),
+        ("pivots_p1", ctypes.c_longlong),
+        ("pivots_p2", ctypes.c_longlong),
+        ("ms_feas", ctypes.c_double),
+        ("ms_opt", ctypes.c_double),
+        ("bytes_feas", ctypes.c_double),
+        ("bytes_opt", ctypes.c_double),
+        ("flops_feas", ctypes.c_double),
+        ("flops_opt", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 RECORD_DTYPE = np.dtype([
@@ -211,6 +235,12 @@ class MpqpLib:
         L.mpqp_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.mpqp_last_error.restype = ctypes.c_int
         L.mpqp_backend.restype = ctypes.c_char_p
+        self.has_stats = hasattr(L, "mpqp_get_stats")
+        if self.has_stats:
+            L.mpqp_get_stats.argtypes = [ctypes.POINTER(MpqpStats)]
+            L.mpqp_get_stats.restype = ctypes.c_int
+        if hasattr(L, "mpqp_measure_lds_bandwidth"):
+            L.mpqp_measure_lds_bandwidth.restype = ctypes.c_double
 
     @property
     def backend(self) -> str:
@@ -218,6 +248,15 @@ class MpqpLib:
 
     def warm_up(self) -> int:
         return self.lib.warm_up()
+
+    def stats(self) -> dict:
+        st = MpqpStats()
+        if not self.has_stats or self.lib.mpqp_get_stats(ctypes.byref(st)) != 0:
+            raise RuntimeError("mpqp_get_stats unavailable")
+        return st.as_dict()
+
+    def measure_lds_bandwidth(self) -> float:
+        return float(self.lib.mpqp_measure_lds_bandwidth())
 
     def last_error(self) -> str:
         buf = ctypes.create_string_buffer(1024)
