@@ -43,7 +43,9 @@ struct OptArgs {
     int store;
     DevRecord *rec;       // by candidate
     int *b1, *b2;         // by candidate x M2, or null
+    const int *p1drop;    // by candidate: position in S of the constraint the warm start adds, or null
     int *trace;           // diagnostics: phase-1 decisions of list[0], or null
+    long long *prof;      // diagnostics: per-block phase cycle counts (8 each), or null
 };
 
 // Byte layout of one stored state: T (rows x cols doubles, contiguous) then
@@ -210,15 +212,29 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
 // Optimality LP
 // ---------------------------------------------------------------------
 __host__ __device__ inline long long opt_lds_bytes(int M, int K, int N2) {
-    const int ld = K + 3;
-    long long b = (long long)(M + 1) * ld * 8 + (long long)(M + 1) * 8 + (long long)(M + K) * 4 +
-                  (long long)N2 * 4 * 3;
+    const int ld = K + 2;
+    long long b = (long long)(M + 1) * ld * 8 + (long long)(M + 1) * 8 + (long long)M * 8 +
+                  (long long)(M + K) * 4 + (long long)N2 * 4 * 3 + (long long)(M + 1) * 4;
     return (b + 15) & ~15LL;
+}
+
+// slots per lane a reduction over n elements needs on one wave
+__host__ __device__ inline int red_slots(int n) {
+    const int s = (n + 63) / 64;
+    return s <= 4 ? (s < 1 ? 1 : s) : (s <= 6 ? 6 : 8);
 }
 
 __device__ __forceinline__ double lam_entry(const DevProb &P, int j, int row) {
     if (row == P.M2 - P.n_c + j) return 0.0;
     return P.A2[(long long)(P.n_x + P.n_t + j) * P.M2 + row];
+}
+
+__device__ __forceinline__ int rfl(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double rfl_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Combine predicates of the reference's reductions, branch-free (see
@@ -251,28 +267,45 @@ struct P2Pred {  // min_idx_kernel_optimal, batched_simplex.cu:343-344
     }
 };
 
-template <int NT>
+#define PROF_MARK(slot)                                                        \
+    do {                                                                       \
+        if (a.prof && tid == 0 && blockIdx.x < 4096) {                         \
+            const long long now_ = clock64();                                  \
+            a.prof[blockIdx.x * 8 + (slot)] += now_ - prof_t0;                 \
+            prof_t0 = now_;                                                    \
+        }                                                                      \
+    } while (0)
+
+// Optimality LP of one candidate per workgroup.  LDS: T (M+1) x (K+2)
+// [slots | rhs | B^-1 delta], colq (M+1), vcol (M, entering column of a
+// lambda swap), basis, nonb, where (var -> position >= 0 or -(slot+1)),
+// bcnt / nbcnt (use counters), rows (compact list of cost-carrying rows).
+template <int NT, int S>
 __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    __shared__ int sh_r, sh_q, sh_done, sh_feas, sh_stall;
+    __shared__ int sh_r, sh_q, sh_done, sh_feas, sh_stall, sh_nrows;
     __shared__ double sh_z;
     const DevProb &P = a.P;
     const DevOpts &O = a.O;
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = K + 3;
-    const int RHS = K, PERT = K + 1, TMP = K + 2;
+    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = K + 2;
+    const int RHS = K, PERT = K + 1;
     const int EPS = P.n_x + P.n_t + P.n_c;
+    const int r4 = M - P.n_c - P.n_x;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
     const int c = a.list[blockIdx.x];
     const int k = a.level;
     double *T = reinterpret_cast<double *>(smem);
     double *colq = T + (M + 1) * ld;
-    int *basis = reinterpret_cast<int *>(colq + (M + 1));
+    double *vcol = colq + (M + 1);
+    int *basis = reinterpret_cast<int *>(vcol + M);
     int *nonb = basis + M;
     int *where = nonb + K;
     int *bcnt = where + N2;
     int *nbcnt = bcnt + N2;
-    const int *S = a.sets + (long long)c * a.kw;
+    int *rows = nbcnt + N2;
+    const int *Sset = a.sets + (long long)c * a.kw;
     DevRecord &R = a.rec[c];
+    long long prof_t0 = clock64();
 
     if (*P.root_ok == 0) {
         if (tid == 0) R.p1_status = ST_SINGULAR;
@@ -281,19 +314,18 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
     int warm = (O.warm && k > 0 && a.p1parent[c] >= 0) ? 1 : 0;
     int started = 0;
     while (!started) {
-        const double *sT;
-        const int *sb;
-        if (warm) {
-            sT = reinterpret_cast<const double *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride);
-        } else {
-            sT = P.root;
-        }
-        sb = reinterpret_cast<const int *>(sT + (long long)M * (K + 2));
+        // ---- load the start state (flat copy: store layout == LDS layout)
+        const double *sT = warm ? reinterpret_cast<const double *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride)
+                                : P.root;
+        const int *sb = warm ? reinterpret_cast<const int *>(sT + (long long)M * ld) : P.root_basis;
         const int *sn = warm ? sb + M : P.root_nonb;
-        if (!warm) sb = P.root_basis;
-        for (int e = tid; e < M * (K + 2); e += NT) {
-            const int i = e / (K + 2), j = e - i * (K + 2);
-            T[i * ld + j] = sT[e];
+        const int ne = M * ld;
+        if ((ne & 1) == 0) {
+            const double2 *s2 = reinterpret_cast<const double2 *>(sT);
+            double2 *d2 = reinterpret_cast<double2 *>(T);
+            for (int e = tid; e < ne / 2; e += NT) d2[e] = s2[e];
+        } else {
+            for (int e = tid; e < ne; e += NT) T[e] = sT[e];
         }
         for (int i = tid; i < M; i += NT) basis[i] = sb[i];
         for (int kk = tid; kk < K; kk += NT) nonb[kk] = sn[kk];
@@ -302,18 +334,26 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
         for (int i = tid; i < M; i += NT) where[basis[i]] = i;
         for (int kk = tid; kk < K; kk += NT) where[nonb[kk]] = -(kk + 1);
         __syncthreads();
+        PROF_MARK(0);
 
+        // ---- lambda swaps: warm = the new constraint only, cold = all of S
         int ok = 1;
-        const int j0 = warm ? k - 1 : 0;
-        for (int jj = j0; jj < k; ++jj) {
-            const int j = S[jj];
-            for (int rho = tid; rho < M; rho += NT) colq[rho] = lam_entry(P, j, rho);
+        const int j0 = warm ? (a.p1drop ? a.p1drop[c] : k - 1) : 0;
+        const int j1 = warm ? j0 + 1 : k;
+        for (int jj = j0; jj < j1; ++jj) {
+            const int j = Sset[jj];
+            // nonzeros of lambda_j: row j (R1) then the R4 rows, ascending
+            if (tid <= P.n_x) {
+                const int rho = tid == 0 ? j : r4 + tid - 1;
+                colq[tid] = lam_entry(P, j, rho);
+            }
             __syncthreads();
             for (int i = tid; i < M; i += NT) {
                 double acc = 0.0;
-                for (int rho = 0; rho < M; ++rho) {
-                    const double av = colq[rho];
+                for (int t = 0; t <= P.n_x; ++t) {
+                    const double av = colq[t];
                     if (av == 0.0) continue;
+                    const int rho = t == 0 ? j : r4 + t - 1;
                     const int w = where[N2 - M + rho];
                     if (w >= 0) {
                         if (i == w) acc = fma(av, 1.0, acc);
@@ -321,20 +361,20 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
                         acc = fma(av, T[i * ld + (-w - 1)], acc);
                     }
                 }
-                T[i * ld + TMP] = acc;
+                vcol[i] = acc;
             }
             __syncthreads();
             const int p = where[P.n_x + P.n_t + j];
             if (p >= 0) {
-                const double pv = T[p * ld + TMP];
+                const double pv = vcol[p];
                 if (!(fabs(pv) > O.licq_tol)) {
                     ok = 0;
                     break;
                 }
-                pivot_team<NT, double>(T, ld, M, K + 3, p, TMP, colq);
+                pivot_ext<NT, double>(T, ld, M, K + 2, p, vcol);
             } else {
                 const int kk = -p - 1;
-                for (int i = tid; i < M; i += NT) T[i * ld + kk] = T[i * ld + TMP];
+                for (int i = tid; i < M; i += NT) T[i * ld + kk] = vcol[i];
                 __syncthreads();
             }
         }
@@ -351,13 +391,29 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
         if (tid == 0) R.p1_status = ST_SINGULAR;
         return;
     }
+    PROF_MARK(1);
 
     // ---------------- phase 1: dual simplex ----------------
-    cost_row_team<NT, double>(
-        T, ld, M, K + 2, [&](int i) { return basis[i] >= Nn ? 1.0 : 0.0; },
+    // rows whose basic variable carries cost (artificials), ascending
+    if (wave == 0) {
+        int base = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = lane + MPQP_WAVE * s;
+            const bool f = i < M && basis[i] >= Nn;
+            const unsigned long long m = __ballot(f);
+            if (f) rows[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+            base += __popcll(m);
+        }
+        if (lane == 0) sh_nrows = base;
+    }
+    __syncthreads();
+    cost_row_list<NT, double>(
+        T, ld, M, K + 2, rows, sh_nrows, [&](int) { return 1.0; },
         [&](int j) { return j < K ? (nonb[j] >= Nn ? 1.0 : 0.0) : 0.0; });
     for (int v = tid; v < N2; v += NT) { bcnt[v] = 0; nbcnt[v] = 0; }
     __syncthreads();
+    PROF_MARK(2);
 
     int n_same = 0;
     double best_p = -INFINITY;
@@ -366,44 +422,38 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
     for (int it = 0; it < O.max_iter_p1; ++it) {
         if (wave == 0) {
             const bool pert = n_same >= O.stall_perturb;
-            double xv[MPQP_MAXS];
-#pragma unroll
-            for (int s = 0; s < MPQP_MAXS; ++s) {
-                const int i = lane + MPQP_WAVE * s;
-                double x = 0.0;
-                if (i < M) {
-                    x = T[i * ld + RHS];
-                    if (pert) x = x + T[i * ld + PERT];
-                }
-                xv[s] = x;
-            }
-            double v[MPQP_MAXS];
-            int id[MPQP_MAXS], cn[MPQP_MAXS];
+            double xv[S], v[S];
+            int id[S], cn[S];
             bool neg = false, art = false;
 #pragma unroll
-            for (int s = 0; s < MPQP_MAXS; ++s) {
+            for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
-                v[s] = xv[s];
+                const bool valid = i < M;
+                double x = valid ? T[i * ld + RHS] : 0.0;
+                if (pert && valid) x = x + T[i * ld + PERT];
+                xv[s] = x;
+                v[s] = x;
                 id[s] = i;
-                cn[s] = (i < M) ? bcnt[basis[i]] : 0;
-                if (i < M && xv[s] < O.dual_stop_tol) neg = true;
-                if (i < M && basis[i] >= Nn) art = true;
+                const int bv = valid ? basis[i] : 0;
+                cn[s] = valid ? bcnt[bv] : 0;
+                neg |= valid && x < O.dual_stop_tol;
+                art |= valid && bv >= Nn;
             }
-            tree_argmin(v, id, cn, M, lane, DminPred{(n_same % 20) >= 10 ? 1 : 0, O.dual_stop_tol});
-            const int r = __shfl(id[0], 0, MPQP_WAVE);
+            tree_argmin_s<S>(v, id, cn, M, lane, DminPred{(n_same % 20) >= 10 ? 1 : 0, O.dual_stop_tol});
+            const int r = rfl(id[0]);
             const bool go_on = __any(neg);
             const bool any_art = __any(art);
             int done = 0, fe = 0, q = -1, stall = 0;
             double zz = 0.0;
             if (lane == 0) bcnt[basis[r]] += 1;
             if (!go_on) {
-                double zs[MPQP_MAXS];
+                double zs[S];
 #pragma unroll
-                for (int s = 0; s < MPQP_MAXS; ++s) {
+                for (int s = 0; s < S; ++s) {
                     const int i = lane + MPQP_WAVE * s;
                     zs[s] = (i < M) ? (basis[i] >= Nn ? 1.0 : 0.0) * xv[s] : 0.0;
                 }
-                zz = tree_sum(zs, M, lane);
+                zz = tree_sum_s<S>(zs, M, lane);
                 fe = (zz > -O.dual_feas_tol && zz < O.dual_feas_tol && !any_art) ? 1 : 0;
                 done = 1 + ST_OPTIMAL;
             } else {
@@ -411,7 +461,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
                 const double *tr = T + r * ld;
                 const double *tc = T + M * ld;
 #pragma unroll
-                for (int s = 0; s < MPQP_MAXS; ++s) {
+                for (int s = 0; s < S; ++s) {
                     const int kk = lane + MPQP_WAVE * s;
                     id[s] = -1;
                     v[s] = 100000.0;
@@ -419,25 +469,26 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
                     if (kk < K) {
                         const double al = tr[kk];
                         if (al < O.tol_minus) {
-                            id[s] = nonb[kk];
+                            const int var = nonb[kk];
+                            id[s] = var;
                             v[s] = -tc[kk] / al;
-                            cn[s] = nbcnt[id[s]];
+                            cn[s] = nbcnt[var];
                         }
                     }
                 }
-                tree_argmin(v, id, cn, K, lane, DratioPred{mode});
-                const int wv = __shfl(id[0], 0, MPQP_WAVE);
+                tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
+                const int wv = rfl(id[0]);
                 if (wv < 0) {
                     done = 1 + ST_UNBOUNDED;
                 } else {
                     q = -where[wv] - 1;
-                    double ps[MPQP_MAXS];
+                    double ps[S];
 #pragma unroll
-                    for (int s = 0; s < MPQP_MAXS; ++s) {
+                    for (int s = 0; s < S; ++s) {
                         const int i = lane + MPQP_WAVE * s;
                         ps[s] = (i < M && xv[s] < 0.0) ? xv[s] : 0.0;
                     }
-                    const double pinf = tree_sum(ps, M, lane);
+                    const double pinf = tree_sum_s<S>(ps, M, lane);
                     if (pinf > best_p) {
                         best_p = pinf;
                         n_same = 0;
@@ -464,6 +515,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
             }
         }
         __syncthreads();
+        PROF_MARK(3);
         if (sh_done) {
             status1 = sh_done - 1;
             z1 = sh_z;
@@ -471,7 +523,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
             break;
         }
         const int r = sh_r, q = sh_q, stall = sh_stall;
-        pivot_team<NT, double>(T, ld, M + 1, K + 2, r, q, colq);
+        pivot_rows<NT, double>(T, ld, M + 1, K + 2, r, q, colq);
         if (tid == 0) {
             const int t = basis[r];
             basis[r] = nonb[q];
@@ -480,6 +532,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
             where[t] = -(q + 1);
         }
         __syncthreads();
+        PROF_MARK(4);
         ++iters1;
         if (stall) {
             status1 = ST_STALL;
@@ -497,22 +550,33 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
     if (a.b1)
         for (int i = tid; i < M; i += NT) a.b1[(long long)c * M + i] = basis[i];
     if (!kkt) return;
+    PROF_MARK(7);
     if (a.store) {
-        unsigned char *dst = a.ostore_cur + (long long)c * a.ostride;
-        double *dT = reinterpret_cast<double *>(dst);
-        for (int e = tid; e < M * (K + 2); e += NT) {
-            const int i = e / (K + 2), j = e - i * (K + 2);
-            dT[e] = T[i * ld + j];
+        double *dT = reinterpret_cast<double *>(a.ostore_cur + (long long)c * a.ostride);
+        const int ne = M * ld;
+        if ((ne & 1) == 0) {
+            double2 *d2 = reinterpret_cast<double2 *>(dT);
+            const double2 *s2 = reinterpret_cast<const double2 *>(T);
+            for (int e = tid; e < ne / 2; e += NT) d2[e] = s2[e];
+        } else {
+            for (int e = tid; e < ne; e += NT) dT[e] = T[e];
         }
-        int *db = reinterpret_cast<int *>(dT + (long long)M * (K + 2));
+        int *db = reinterpret_cast<int *>(dT + (long long)M * ld);
         for (int i = tid; i < M; i += NT) db[i] = basis[i];
         for (int kk = tid; kk < K; kk += NT) db[M + kk] = nonb[kk];
     }
 
     // ---------------- phase 2: primal simplex, min -eps ----------------
     __syncthreads();
-    cost_row_team<NT, double>(
-        T, ld, M, K + 1, [&](int i) { return basis[i] == EPS ? -1.0 : 0.0; },
+    PROF_MARK(5);
+    if (tid == 0) {
+        const int pe = where[EPS];
+        sh_nrows = pe >= 0 ? 1 : 0;
+        rows[0] = pe;
+    }
+    __syncthreads();
+    cost_row_list<NT, double>(
+        T, ld, M, K + 1, rows, sh_nrows, [&](int) { return -1.0; },
         [&](int j) { return j < K ? (nonb[j] == EPS ? -1.0 : 0.0) : 0.0; });
     for (int v = tid; v < N2; v += NT) bcnt[v] = 0;
     __syncthreads();
@@ -520,10 +584,10 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
     double z2 = 0.0;
     for (int it = 0; it < O.max_iter_p2; ++it) {
         if (wave == 0) {
-            double v[MPQP_MAXS];
-            int id[MPQP_MAXS], cn[MPQP_MAXS];
+            double v[S];
+            int id[S], cn[S];
 #pragma unroll
-            for (int s = 0; s < MPQP_MAXS; ++s) {
+            for (int s = 0; s < S; ++s) {
                 const int var = lane + MPQP_WAVE * s;
                 v[s] = 0.0;
                 id[s] = var;
@@ -534,20 +598,20 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
                     cn[s] = bcnt[var];
                 }
             }
-            tree_argmin(v, id, cn, Nn, lane, P2Pred{});
-            const int wv = __shfl(id[0], 0, MPQP_WAVE);
-            const double dmin = __shfl(v[0], 0, MPQP_WAVE);
+            tree_argmin_s<S>(v, id, cn, Nn, lane, P2Pred{});
+            const int wv = rfl(id[0]);
+            const double dmin = rfl_d(v[0]);
             if (lane == 0) bcnt[wv] += 1;
             int done = 0, r = -1, q = -1;
             double zz = 0.0;
             if (dmin > O.tol_minus_opt) {
-                double zs[MPQP_MAXS];
+                double zs[S];
 #pragma unroll
-                for (int s = 0; s < MPQP_MAXS; ++s) {
+                for (int s = 0; s < S; ++s) {
                     const int i = lane + MPQP_WAVE * s;
                     zs[s] = (i < M) ? (basis[i] == EPS ? -1.0 : 0.0) * T[i * ld + RHS] : 0.0;
                 }
-                zz = tree_sum(zs, M, lane);
+                zz = tree_sum_s<S>(zs, M, lane);
                 done = 1 + ST_OPTIMAL;
             } else {
                 q = -where[wv] - 1;
@@ -578,7 +642,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
             break;
         }
         const int r = sh_r, q = sh_q;
-        pivot_team<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
+        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
         if (tid == 0) {
             const int t = basis[r];
             basis[r] = nonb[q];
@@ -589,6 +653,7 @@ __global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
         __syncthreads();
         ++iters2;
     }
+    PROF_MARK(6);
     if (tid == 0) {
         R.p2_status = status2;
         R.p2_iters = iters2;

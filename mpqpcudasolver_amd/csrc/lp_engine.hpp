@@ -70,6 +70,30 @@ __device__ __forceinline__ int shfl_down_i(int v, int d) {
     return __shfl_down(v, (unsigned)d, MPQP_WAVE);
 }
 
+// lane_down<I>(x): lane t receives lane t + I (valid for t < I <= 32, the
+// only lanes the reference tree reads), without an LDS round trip:
+// DPP row_shl for I <= 8 (inside a 16-lane row), ds_swizzle SWAP16 for 16,
+// v_permlane32_swap (gfx950) for 32.
+template <int I>
+__device__ __forceinline__ int lane_down_i(int x) {
+    static_assert(I == 1 || I == 2 || I == 4 || I == 8 || I == 16 || I == 32, "stride");
+    if constexpr (I <= 8) {
+        return __builtin_amdgcn_update_dpp(x, x, 0x100 + I, 0xf, 0xf, false);
+    } else if constexpr (I == 16) {
+        return __builtin_amdgcn_ds_swizzle(x, 0x401F);
+    } else {
+        return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
+    }
+}
+
+template <int I>
+__device__ __forceinline__ double lane_down_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = lane_down_i<I>((int)(unsigned)(b & 0xffffffffLL));
+    const int hi = lane_down_i<I>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // ---------------------------------------------------------------------
 // Reference-tree reductions on one wave.  Element t lives in lane t % 64,
 // slot t / 64.  Result in lane 0, slot 0.
@@ -110,6 +134,80 @@ __device__ __forceinline__ void tree_argmin(double (&v)[MPQP_MAXS], int (&id)[MP
         id[0] = take ? oi : id[0];
         cn[0] = take ? oc : cn[0];
     }
+}
+
+// Slot-count-templated versions (S = ceil(n / 64) rounded up, n <= 64 * S).
+template <int S, int I, typename Pred>
+__device__ __forceinline__ void tree_step(double (&v)[S], int (&id)[S], int (&cn)[S], int n, int lane,
+                                          Pred pred) {
+    const double ov = lane_down_d<I>(v[0]);
+    const int oi = lane_down_i<I>(id[0]);
+    const int oc = lane_down_i<I>(cn[0]);
+    const bool take = (lane < I) & (lane + I < n) & pred(v[0], id[0], cn[0], ov, oi, oc);
+    v[0] = take ? ov : v[0];
+    id[0] = take ? oi : id[0];
+    cn[0] = take ? oc : cn[0];
+}
+
+template <int S, typename Pred>
+__device__ __forceinline__ void tree_argmin_s(double (&v)[S], int (&id)[S], int (&cn)[S], int n, int lane,
+                                              Pred pred) {
+    const int half = half_pow2(n);
+#pragma unroll
+    for (int lvl = 2; lvl >= 0; --lvl) {
+        const int i = 64 << lvl;
+        const int ds = 1 << lvl;
+        if (i <= half) {
+#pragma unroll
+            for (int s = 0; s < ds; ++s) {
+                if (s + ds < S) {
+                    const int u = lane + 64 * (s + ds);
+                    const bool take = (u < n) & pred(v[s], id[s], cn[s], v[s + ds], id[s + ds], cn[s + ds]);
+                    v[s] = take ? v[s + ds] : v[s];
+                    id[s] = take ? id[s + ds] : id[s];
+                    cn[s] = take ? cn[s + ds] : cn[s];
+                }
+            }
+        }
+    }
+    if (half >= 32) tree_step<S, 32>(v, id, cn, n, lane, pred);
+    if (half >= 16) tree_step<S, 16>(v, id, cn, n, lane, pred);
+    if (half >= 8) tree_step<S, 8>(v, id, cn, n, lane, pred);
+    if (half >= 4) tree_step<S, 4>(v, id, cn, n, lane, pred);
+    if (half >= 2) tree_step<S, 2>(v, id, cn, n, lane, pred);
+    if (half >= 1) tree_step<S, 1>(v, id, cn, n, lane, pred);
+}
+
+template <int S, int I>
+__device__ __forceinline__ void sum_step(double (&v)[S], int n, int lane) {
+    const double ov = lane_down_d<I>(v[0]);
+    v[0] = ((lane < I) & (lane + I < n)) ? v[0] + ov : v[0];
+}
+
+template <int S>
+__device__ __forceinline__ double tree_sum_s(double (&v)[S], int n, int lane) {
+    const int half = half_pow2(n);
+#pragma unroll
+    for (int lvl = 2; lvl >= 0; --lvl) {
+        const int i = 64 << lvl;
+        const int ds = 1 << lvl;
+        if (i <= half) {
+#pragma unroll
+            for (int s = 0; s < ds; ++s) {
+                if (s + ds < S) {
+                    const int u = lane + 64 * (s + ds);
+                    v[s] = (u < n) ? v[s] + v[s + ds] : v[s];
+                }
+            }
+        }
+    }
+    if (half >= 32) sum_step<S, 32>(v, n, lane);
+    if (half >= 16) sum_step<S, 16>(v, n, lane);
+    if (half >= 8) sum_step<S, 8>(v, n, lane);
+    if (half >= 4) sum_step<S, 4>(v, n, lane);
+    if (half >= 2) sum_step<S, 2>(v, n, lane);
+    if (half >= 1) sum_step<S, 1>(v, n, lane);
+    return __shfl(v[0], 0, MPQP_WAVE);
 }
 
 __device__ __forceinline__ double tree_sum(double (&v)[MPQP_MAXS], int n, int lane) {
@@ -180,6 +278,103 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
                 ti[j] = fma(-colq[i], prj, ti[j]);
             }
         }
+    }
+    __syncthreads();
+}
+
+// Same pivot, rows updated four at a time per wave so that several LDS loads
+// are in flight (the tableau of a large LP leaves room for one workgroup per
+// CU only, so latency must be hidden inside the wave).  Per-entry arithmetic
+// is identical to pivot_team.
+template <int NT, typename Real>
+__device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols, int r, int q,
+                                           Real *colq) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nrows; i += NT) colq[i] = T[i * ld + q];
+    __syncthreads();
+    const Real p = colq[r];
+    Real *tr = T + r * ld;
+    for (int j = tid; j < ncols; j += NT)
+        if (j != q) tr[j] = tr[j] / p;
+    if (tid == 0) tr[q] = Real(1) / p;
+    for (int i = tid; i < nrows; i += NT)
+        if (i != r) T[i * ld + q] = Real(0);
+    __syncthreads();
+    constexpr int NW = NT / MPQP_WAVE;
+    constexpr int U = 4;
+    const int lane = tid & (MPQP_WAVE - 1);
+    const int wave = tid / MPQP_WAVE;
+    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
+        const int j = j0 + lane;
+        if (j < ncols) {
+            const Real prj = tr[j];
+            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
+                Real t[U], f[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
+                    f[u] = i < nrows ? colq[i] : Real(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, t[u]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Pivot on row r with an entering column v held outside the tableau (rows
+// [0, nrows) of v); the leaving column is discarded.  Retained entries get
+// exactly the arithmetic of pivot_team.
+template <int NT, typename Real>
+__device__ __forceinline__ void pivot_ext(Real *T, int ld, int nrows, int ncols, int r, const Real *v) {
+    const int tid = threadIdx.x;
+    const Real p = v[r];
+    Real *tr = T + r * ld;
+    for (int j = tid; j < ncols; j += NT) tr[j] = tr[j] / p;
+    __syncthreads();
+    constexpr int NW = NT / MPQP_WAVE;
+    constexpr int U = 4;
+    const int lane = tid & (MPQP_WAVE - 1);
+    const int wave = tid / MPQP_WAVE;
+    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
+        const int j = j0 + lane;
+        if (j < ncols) {
+            const Real prj = tr[j];
+            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
+                Real t[U], f[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
+                    f[u] = i < nrows ? v[i] : Real(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, t[u]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Reduced-cost row from a compact ascending list of the rows with cb != 0.
+template <int NT, typename Real, typename CbF, typename CnF>
+__device__ __forceinline__ void cost_row_list(Real *T, int ld, int M, int ncols, const int *rows, int nrows_cb,
+                                              CbF cb, CnF cN) {
+    for (int j = threadIdx.x; j < ncols; j += NT) {
+        Real acc = cN(j);
+        for (int t = 0; t < nrows_cb; ++t) {
+            const int i = rows[t];
+            acc = fma(-cb(i), T[i * ld + j], acc);
+        }
+        T[M * ld + j] = acc;
     }
     __syncthreads();
 }

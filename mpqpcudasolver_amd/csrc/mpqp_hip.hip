@@ -31,7 +31,24 @@ static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
 namespace {
 
 constexpr int FEAS_NT = 64;
-constexpr int OPT_NT = 256;
+constexpr int OPT_NT = 512;
+
+template <int NT>
+void launch_opt_nt(int S, int grid, size_t lds, hipStream_t st, const OptArgs &oa) {
+    switch (S) {
+    case 1: hipLaunchKernelGGL((opt_kernel<NT, 1>), dim3(grid), dim3(NT), lds, st, oa); break;
+    case 2: hipLaunchKernelGGL((opt_kernel<NT, 2>), dim3(grid), dim3(NT), lds, st, oa); break;
+    case 3: hipLaunchKernelGGL((opt_kernel<NT, 3>), dim3(grid), dim3(NT), lds, st, oa); break;
+    case 4: hipLaunchKernelGGL((opt_kernel<NT, 4>), dim3(grid), dim3(NT), lds, st, oa); break;
+    case 6: hipLaunchKernelGGL((opt_kernel<NT, 6>), dim3(grid), dim3(NT), lds, st, oa); break;
+    default: hipLaunchKernelGGL((opt_kernel<NT, 8>), dim3(grid), dim3(NT), lds, st, oa); break;
+    }
+}
+
+void launch_opt(int nt, int S, int grid, size_t lds, hipStream_t st, const OptArgs &oa) {
+    if (nt == 256) launch_opt_nt<256>(S, grid, lds, st, oa);
+    else launch_opt_nt<512>(S, grid, lds, st, oa);
+}
 
 struct Ctx {
     bool ready = false;
@@ -322,6 +339,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     const long long ostride = state_bytes(M2, K2 + 2, M2, K2);
     const long long flds = feas_lds_bytes(M1, K1, n_c);
     const long long olds = opt_lds_bytes(M2, K2, N2);
+    const int opt_nt = (o.opt_threads == 256) ? 256 : OPT_NT;
 
     hipEvent_t e0, e1, e2;
     HIP_OK(hipEventCreate(&e0));
@@ -437,7 +455,13 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             oa.rec = drec.as<DevRecord>();
             oa.b1 = keep_bases ? db1.as<int>() : nullptr;
             oa.b2 = keep_bases ? db2.as<int>() : nullptr;
-            DBuf dtrace;
+            DBuf dtrace, dprof;
+            const char *prof_path = std::getenv("MPQP_DEBUG_PROF");
+            if (prof_path) {
+                dprof.alloc(sizeof(long long) * 8 * 4096);
+                HIP_OK(hipMemsetAsync(dprof.p, 0, dprof.n, st));
+                oa.prof = dprof.as<long long>();
+            }
             const char *tr_path = std::getenv("MPQP_DEBUG_TRACE");
             if (tr_path && lvl == 0) {
                 dtrace.alloc(sizeof(int) * 4 * 256);
@@ -445,7 +469,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 oa.trace = dtrace.as<int>();
             }
             HIP_OK(hipEventRecord(e1, st));
-            hipLaunchKernelGGL(opt_kernel<OPT_NT>, dim3(nf), dim3(OPT_NT), (size_t)olds, st, oa);
+            launch_opt(opt_nt, red_slots(std::max(M2, N2 - n_eqs)), nf, (size_t)olds, st, oa);
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(e2, st));
             HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
@@ -453,6 +477,15 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             HIP_OK(hipEventElapsedTime(&ms, e1, e2));
             to += ms * 1e-3;
             C.stats.ms_opt += ms;
+            if (oa.prof) {
+                std::vector<long long> hp(8 * 4096);
+                HIP_OK(hipMemcpy(hp.data(), oa.prof, hp.size() * 8, hipMemcpyDeviceToHost));
+                std::string pth = std::string(prof_path) + "." + std::to_string(lvl);
+                if (FILE *f = std::fopen(pth.c_str(), "wb")) {
+                    std::fwrite(hp.data(), 8, hp.size(), f);
+                    std::fclose(f);
+                }
+            }
             if (oa.trace) {
                 std::vector<int> ht(4 * 256);
                 HIP_OK(hipMemcpy(ht.data(), oa.trace, ht.size() * 4, hipMemcpyDeviceToHost));

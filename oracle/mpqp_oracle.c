@@ -842,6 +842,7 @@ typedef struct {
 
 static RecStore g_rs;
 static GPUResults *g_last = NULL;
+static MpqpStats g_stats;
 
 static void rs_free(RecStore *r) {
     free(r->rec); free(r->sets); free(r->fb); free(r->b1); free(r->b2);
@@ -1012,6 +1013,18 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
                 (t < all[c].level && t < kw) ? allsets[(size_t)c * kw + t] : -1;
         row++;
     }
+    memset(&g_stats, 0, sizeof g_stats);
+    g_stats.levels = n_batches;
+    g_stats.launches_feas = n_batches;
+    g_stats.candidates = total;
+    for (int c = 0; c < total; c++) {
+        g_stats.opt_lps += all[c].feasible;
+        g_stats.pivots_feas += all[c].feas_iters;
+        g_stats.pivots_p1 += all[c].p1_iters;
+        g_stats.pivots_p2 += all[c].p2_iters;
+    }
+    g_stats.ms_feas = 1e3 * tf;
+    g_stats.ms_opt = 1e3 * to;
     res->n_batches = n_batches;
     res->n_splits = 1;
     res->n_feasible_lps = n_feas;
@@ -1037,6 +1050,14 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
     g_last = res;
     return res;
 }
+
+int mpqp_get_stats(MpqpStats *st) {
+    if (!st) return -1;
+    *st = g_stats;
+    return 0;
+}
+
+double mpqp_measure_lds_bandwidth(void) { return -1.0; }
 
 int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis, int *p1_basis,
                      int *p2_basis, int max_records) {

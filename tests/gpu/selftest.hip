@@ -7,32 +7,58 @@
 
 struct Out { int id; int pad; double v; };
 
-template <typename Pred>
+template <int S, typename Pred>
 __global__ void k_tree(Pred pred, const double *v, const int *id, const int *cn, int n, Out *out) {
     const int lane = threadIdx.x;
-    double vv[MPQP_MAXS];
-    int ii[MPQP_MAXS], cc[MPQP_MAXS];
+    double vv[S];
+    int ii[S], cc[S];
 #pragma unroll
-    for (int s = 0; s < MPQP_MAXS; ++s) {
+    for (int s = 0; s < S; ++s) {
         const int t = lane + MPQP_WAVE * s;
         vv[s] = t < n ? v[t] : 0.0;
         ii[s] = t < n ? id[t] : t;
         cc[s] = t < n ? cn[t] : 0;
     }
-    tree_argmin(vv, ii, cc, n, lane, pred);
+    tree_argmin_s<S>(vv, ii, cc, n, lane, pred);
     if (lane == 0) { out->id = ii[0]; out->v = vv[0]; }
 }
 
+template <int S>
 __global__ void k_sum(const double *v, int n, double *out) {
     const int lane = threadIdx.x;
-    double vv[MPQP_MAXS];
+    double vv[S];
 #pragma unroll
-    for (int s = 0; s < MPQP_MAXS; ++s) {
+    for (int s = 0; s < S; ++s) {
         const int t = lane + MPQP_WAVE * s;
         vv[s] = t < n ? v[t] : 0.0;
     }
-    double r = tree_sum(vv, n, lane);
+    double r = tree_sum_s<S>(vv, n, lane);
     if (lane == 0) *out = r;
+}
+
+// lane_down<I> for every stride: out[I-index][lane] = value received
+__global__ void k_lanes(const double *v, double *out) {
+    const int lane = threadIdx.x;
+    const double x = v[lane];
+    out[0 * 64 + lane] = lane_down_d<1>(x);
+    out[1 * 64 + lane] = lane_down_d<2>(x);
+    out[2 * 64 + lane] = lane_down_d<4>(x);
+    out[3 * 64 + lane] = lane_down_d<8>(x);
+    out[4 * 64 + lane] = lane_down_d<16>(x);
+    out[5 * 64 + lane] = lane_down_d<32>(x);
+}
+
+template <typename Pred>
+void launch_tree(Pred pred, const double *dv, const int *di, const int *dc, int n, Out *dout) {
+    const int S = red_slots(n);
+    switch (S) {
+    case 1: hipLaunchKernelGGL((k_tree<1, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    case 2: hipLaunchKernelGGL((k_tree<2, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    case 3: hipLaunchKernelGGL((k_tree<3, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    case 4: hipLaunchKernelGGL((k_tree<4, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    case 6: hipLaunchKernelGGL((k_tree<6, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    default: hipLaunchKernelGGL((k_tree<8, Pred>), dim3(1), dim3(64), 0, 0, pred, dv, di, dc, n, dout); break;
+    }
 }
 
 __global__ void k_div(const double *a, const double *b, double *q, int n) {
@@ -56,9 +82,9 @@ int selftest_tree(int kind, int mode, double stop_tol, const double *v, const in
     hipMemcpy(dv, v, 8 * n, hipMemcpyHostToDevice);
     hipMemcpy(di, id, 4 * n, hipMemcpyHostToDevice);
     hipMemcpy(dc, cn, 4 * n, hipMemcpyHostToDevice);
-    if (kind == 0) hipLaunchKernelGGL(k_tree<DminPred>, dim3(1), dim3(64), 0, 0, DminPred{mode, stop_tol}, dv, di, dc, n, dout);
-    else if (kind == 1) hipLaunchKernelGGL(k_tree<DratioPred>, dim3(1), dim3(64), 0, 0, DratioPred{mode}, dv, di, dc, n, dout);
-    else hipLaunchKernelGGL(k_tree<P2Pred>, dim3(1), dim3(64), 0, 0, P2Pred{}, dv, di, dc, n, dout);
+    if (kind == 0) launch_tree(DminPred{mode, stop_tol}, dv, di, dc, n, dout);
+    else if (kind == 1) launch_tree(DratioPred{mode}, dv, di, dc, n, dout);
+    else launch_tree(P2Pred{}, dv, di, dc, n, dout);
     Out h;
     hipError_t e = hipMemcpy(&h, dout, sizeof(Out), hipMemcpyDeviceToHost);
     hipFree(dv); hipFree(di); hipFree(dc); hipFree(dout);
@@ -72,8 +98,25 @@ int selftest_sum(const double *v, int n, double *out) {
     double *dv, *dout;
     if (hipMalloc(&dv, 8 * n) || hipMalloc(&dout, 8)) return -2;
     hipMemcpy(dv, v, 8 * n, hipMemcpyHostToDevice);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, 0, dv, n, dout);
+    switch (red_slots(n)) {
+    case 1: hipLaunchKernelGGL(k_sum<1>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    case 2: hipLaunchKernelGGL(k_sum<2>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    case 3: hipLaunchKernelGGL(k_sum<3>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    case 4: hipLaunchKernelGGL(k_sum<4>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    case 6: hipLaunchKernelGGL(k_sum<6>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    default: hipLaunchKernelGGL(k_sum<8>, dim3(1), dim3(64), 0, 0, dv, n, dout); break;
+    }
     hipError_t e = hipMemcpy(out, dout, 8, hipMemcpyDeviceToHost);
+    hipFree(dv); hipFree(dout);
+    return e == hipSuccess ? 0 : -3;
+}
+
+int selftest_lanes(const double *v, double *out) {
+    double *dv, *dout;
+    if (hipMalloc(&dv, 8 * 64) || hipMalloc(&dout, 8 * 64 * 6)) return -2;
+    hipMemcpy(dv, v, 8 * 64, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_lanes, dim3(1), dim3(64), 0, 0, dv, dout);
+    hipError_t e = hipMemcpy(out, dout, 8 * 64 * 6, hipMemcpyDeviceToHost);
     hipFree(dv); hipFree(dout);
     return e == hipSuccess ? 0 : -3;
 }

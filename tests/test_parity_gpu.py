@@ -1,0 +1,184 @@
+"""GPU parity: libmpqp_hip.so against the CPU oracle, through the C ABI.
+
+Verdicts, statuses, iteration counts, candidate sets and final bases must be
+identical; objectives (z_feas, z1, z2) within 1e-9 relative.  Large runs are
+checked on a random sample of candidates (oracle_eval_set walks each
+sample's prefix chain) plus size-independent properties."""
+import ctypes
+import os
+import sys
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from mpqpcudasolver_amd import (assemble_lps, config_problem, load_library, precision_stress_mpqp,
+                                synthetic_mpqp)
+from oracle_lib import OracleLib
+from parity import FLOAT_FIELDS, INT_FIELDS, REL_TOL, compare_runs
+from reftree import pred_dmin, pred_dratio, pred_p2, tree_argmin, tree_sum
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SELFTEST = os.path.join(ROOT, "tests", "gpu", "_build", "libmpqp_selftest.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    lib = load_library()
+    assert lib.warm_up() == 0, lib.last_error()
+    return lib
+
+
+@pytest.fixture(scope="module")
+def orc(oracle_path):
+    return OracleLib(oracle_path)
+
+
+@pytest.fixture(scope="module")
+def st():
+    L = ctypes.cdll.LoadLibrary(SELFTEST)
+    L.selftest_tree.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _ip, ctypes.c_int, _ip, _dp]
+    L.selftest_sum.argtypes = [_dp, ctypes.c_int, _dp]
+    L.selftest_lanes.argtypes = [_dp, _dp]
+    L.selftest_arith.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+    return L
+
+
+def _run_pair(hip, orc, lp, k, bases=True, **opts):
+    hip.set_options(record=2 if bases else 1, **opts)
+    orc.set_options(record=2 if bases else 1, **opts)
+    g = hip.run(lp, k_max=k, records=True, bases=bases)
+    o = orc.run(lp, k_max=k, records=True, bases=bases)
+    return g, o
+
+
+# ---------------------------------------------------------------- primitives
+
+def test_lane_shifts(st):
+    _log("case: lane_down strides 1..32")
+    v = np.arange(64, dtype=np.float64) * 1.5 + 0.25
+    out = np.zeros(6 * 64)
+    assert st.selftest_lanes(v.ctypes.data_as(_dp), out.ctypes.data_as(_dp)) == 0
+    out = out.reshape(6, 64)
+    for k, stride in enumerate((1, 2, 4, 8, 16, 32)):
+        assert np.array_equal(out[k, :stride], v[stride:2 * stride])
+
+
+def test_reduction_trees_match_reference(st):
+    _log("case: device reduction trees vs reference tree, 150 random inputs")
+    rng = np.random.default_rng(0)
+    for _ in range(150):
+        n = int(rng.integers(1, 513))
+        v = np.round(rng.standard_normal(n), 1)
+        cn = rng.integers(0, 20, n).astype(np.intc)
+        for kind, modes in ((0, (0, 1)), (1, (0, 1, 2)), (2, (0,))):
+            for mode in modes:
+                ids = np.arange(n, dtype=np.intc) if kind != 1 else np.where(rng.random(n) < 0.5, np.arange(n), -1).astype(np.intc)
+                pr = pred_dmin(mode, -1e-9) if kind == 0 else (pred_dratio(mode) if kind == 1 else pred_p2())
+                oi, ov = ctypes.c_int(), ctypes.c_double()
+                vv = np.ascontiguousarray(v)
+                assert st.selftest_tree(kind, mode, -1e-9, vv.ctypes.data_as(_dp), ids.ctypes.data_as(_ip),
+                                        cn.ctypes.data_as(_ip), n, ctypes.byref(oi), ctypes.byref(ov)) == 0
+                assert oi.value == tree_argmin(pr, v, ids, cn)[0]
+        x = rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, n)
+        out = ctypes.c_double()
+        assert st.selftest_sum(np.ascontiguousarray(x).ctypes.data_as(_dp), n, ctypes.byref(out)) == 0
+        assert out.value == tree_sum(x)
+
+
+def test_device_division_and_fma_are_ieee(st):
+    _log("case: fp64 division and fma rounding")
+    rng = np.random.default_rng(1)
+    n = 4096
+    a = rng.standard_normal(n) * 10.0 ** rng.integers(-20, 20, n)
+    b = rng.standard_normal(n) * 10.0 ** rng.integers(-20, 20, n)
+    c = rng.standard_normal(n) * 10.0 ** rng.integers(-20, 20, n)
+    q = np.zeros(n)
+    r = np.zeros(n)
+    assert st.selftest_arith(a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), c.ctypes.data_as(_dp),
+                             q.ctypes.data_as(_dp), r.ctypes.data_as(_dp), n) == 0
+    assert np.array_equal(q, a / b)  # numpy division is IEEE round-to-nearest
+    for i in range(0, n, 7):
+        exact = -Fraction(a[i]) * Fraction(b[i]) + Fraction(c[i])
+        assert r[i] == float(exact)
+
+
+# ---------------------------------------------------------------- full parity
+
+@pytest.mark.parametrize("name", ["config0", "small8x24", "small5x16k4", "stress10x24"])
+def test_parity_small(hip, orc, name):
+    _log(f"case: parity {name}")
+    prog, k = {
+        "config0": lambda: config_problem(0),
+        "small8x24": lambda: (synthetic_mpqp(8, 24, 3, 11), 3),
+        "small5x16k4": lambda: (synthetic_mpqp(5, 16, 2, 13), 4),
+        "stress10x24": lambda: (precision_stress_mpqp(10, 24, 3, 13, n_dup=4), 3),
+    }[name]()
+    g, o = _run_pair(hip, orc, assemble_lps(prog), k)
+    assert compare_runs(g, o) == []
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_parity_config_first_levels(hip, orc, cfg):
+    _log(f"case: parity config {cfg}, |S| <= 2")
+    prog, _ = config_problem(cfg)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 2)
+    assert compare_runs(g, o) == []
+
+
+def test_parity_cold_start(hip, orc):
+    _log("case: parity without warm starts")
+    prog = synthetic_mpqp(8, 24, 3, 11)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 3, warm_start=0)
+    assert compare_runs(g, o) == []
+    hip.set_options()
+    orc.set_options()
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_parity_full_config_sampled(hip, orc, cfg):
+    """Full |S| <= 4 run; 150 random candidates re-evaluated by the oracle
+    along their prefix chains, plus enumeration properties of all."""
+    _log(f"case: config {cfg} full run, sampled parity")
+    prog, k = config_problem(cfg)
+    lp = assemble_lps(prog)
+    hip.set_options(record=1)
+    g = hip.run(lp, k_max=k, records=True)
+    r = g.records
+    sets = [[int(v) for v in row if v >= 0] for row in g.sets]
+    # enumeration properties
+    assert r["feasible"].sum() == g.n_feasible_lps
+    assert (r["final_set"] == 1).sum() == g.n_final_lps
+    par = r["parent"]
+    assert np.all(r["feasible"][par[par >= 0]] == 1)
+    assert np.all(r["level"] == [len(s) for s in sets])
+    assert np.all((r["kkt_feasible"] == 0) | (r["feasible"] == 1))
+    assert np.all((r["final_set"] == 0) | (r["kkt_feasible"] == 1))
+    rng = np.random.default_rng(cfg)
+    sample = rng.choice(len(sets), 150, replace=False)
+    for c in sample:
+        rec = orc.eval_set(lp, sets[c])[0]
+        for f in INT_FIELDS:
+            if f == "parent":
+                continue
+            assert rec[f] == r[f][c], (sets[c], f, rec[f], r[f][c])
+        for f in FLOAT_FIELDS:
+            assert abs(rec[f] - r[f][c]) <= REL_TOL * max(1.0, abs(rec[f])), (sets[c], f)
+
+
+def test_errors_are_reported(hip):
+    _log("case: invalid input")
+    prog, k = config_problem(0)
+    lp = assemble_lps(prog)
+    lp.b2 = lp.b2[::-1].copy()
+    assert not hip.mpqp_raw(lp, 0, k)
+    assert "b2" in hip.last_error()
