@@ -106,7 +106,10 @@ typedef struct MpqpOptions {
     double final_tol;          /* final active set if margin eps* > final_tol */
     double perturb_step;       /* rhs perturbation (i+1)*perturb_step (1e-6) */
     double certify_tol;        /* mixed precision: FP64 certification tolerance */
-    double reserved1;
+    double pivot_rel;          /* ratio tests also require |entry| > pivot_rel * max|entry| (0: off) */
+    double tol32;              /* mixed precision: pivot / stop tolerance of the FP32 pass (1e-6) */
+    double licq32;             /* mixed precision: FP32 LICQ screen; smaller pivots go to FP64 (1e-6) */
+    double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass */
 } MpqpOptions;
 
 /* One record per candidate active set of the last mpqp() call, in
@@ -121,7 +124,7 @@ typedef struct MpqpRecord {
     double z1;          /* phase-1 objective (sum of artificials) */
     double z2;          /* phase-2 objective = -eps* */
     int repivots;       /* mixed precision: FP64 re-pivots after certification */
-    int reserved;
+    int p1_clean;       /* phase-1 basis holds no artificial (after drive-out) */
 } MpqpRecord;
 
 void mpqp_default_options(MpqpOptions *opt);
@@ -150,13 +153,14 @@ int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis,
 typedef struct MpqpStats {
     int levels;
     int launches_feas, launches_opt;
-    int reserved;
+    int precision_used;       /* MPQP_FP64, or MPQP_MIXED if the FP32 probe passed */
     long long candidates;     /* LPs checked: one feasibility LP each */
     long long opt_lps;        /* optimality LPs run (feasible candidates) */
     long long pivots_feas, pivots_p1, pivots_p2;
     double ms_feas, ms_opt;   /* summed kernel time */
     double bytes_feas, bytes_opt;
     double flops_feas, flops_opt;
+    long long fallbacks;      /* mixed precision: optimality LPs re-solved in FP64 */
 } MpqpStats;
 
 int mpqp_get_stats(MpqpStats *stats);

@@ -105,6 +105,10 @@ void default_opts(MpqpOptions *o) {
     o->final_tol = 1e-8;
     o->perturb_step = 1e-6;
     o->certify_tol = 1e-9;
+    o->pivot_rel = 1e-9;
+    o->tol32 = 1e-5;
+    o->licq32 = 1e-6;
+    o->pivot_rel32 = 1e-3;
 }
 
 const MpqpOptions &opts() {

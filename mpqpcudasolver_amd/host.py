@@ -82,7 +82,10 @@ class MpqpOptions(ctypes.Structure):
         ("final_tol", ctypes.c_double),
         ("perturb_step", ctypes.c_double),
         ("certify_tol", ctypes.c_double),
-        ("reserved1", ctypes.c_double),
+        ("pivot_rel", ctypes.c_double),
+        ("tol32", ctypes.c_double),
+        ("licq32", ctypes.c_double),
+        ("pivot_rel32", ctypes.c_double),
     ]
 
 
@@ -104,7 +107,7 @@ class MpqpRecord(ctypes.Structure):
         ("z1", ctypes.c_double),
         ("z2", ctypes.c_double),
         ("repivots", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("p1_clean", ctypes.c_int),
     ]
 
 
@@ -113,7 +116,7 @@ class MpqpStats(ctypes.Structure):
         ("levels", ctypes.c_int),
         ("launches_feas", ctypes.c_int),
         ("launches_opt", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("precision_used", ctypes.c_int),
         ("candidates", ctypes.c_longlong),
         ("opt_lps", ctypes.c_longlong),
         ("pivots_feas", ctypes.c_longlong),
@@ -125,6 +128,7 @@ class MpqpStats(ctypes.Structure):
         ("bytes_opt", ctypes.c_double),
         ("flops_feas", ctypes.c_double),
         ("flops_opt", ctypes.c_double),
+        ("fallbacks", ctypes.c_longlong),
     ]
 
     def as_dict(self) -> dict:
@@ -137,7 +141,7 @@ RECORD_DTYPE = np.dtype([
     ("p1_status", np.int32), ("p1_iters", np.int32), ("kkt_feasible", np.int32), ("p1_warm", np.int32),
     ("p2_status", np.int32), ("p2_iters", np.int32), ("final_set", np.int32),
     ("z_feas", np.float64), ("z1", np.float64), ("z2", np.float64),
-    ("repivots", np.int32), ("reserved", np.int32),
+    ("repivots", np.int32), ("p1_clean", np.int32),
 ])
 assert RECORD_DTYPE.itemsize == ctypes.sizeof(MpqpRecord)
 

@@ -52,7 +52,10 @@
  *     finished; here each LP counts its own iterations since its phase-1
  *     primal infeasibility sum(min(x_B,0)) last reached a new best.
  *   - the artificial-in-basis test of :238 compares row positions with N_
- *     and never fires; here an artificial left basic makes phase 1 fail.
+ *     and never fires; here phase 1 is KKT-feasible iff its optimum
+ *     |z| < dual_feasible_tol (a property of the LP, not of the pivot path),
+ *     and artificials left basic at zero are pivoted out (drive_out) before
+ *     phase 2, which prices only the first N_ columns.
  *   - the mode-1/2 counter lookups of :852,861 index non_b by a variable id;
  *     here the counter of that variable is used.
  *   - an empty ratio test (stale th_idxs :871-876, idx -1 write :941)
@@ -110,6 +113,10 @@ void mpqp_default_options(MpqpOptions *o) {
     o->final_tol = 1e-8;
     o->perturb_step = 1e-6;
     o->certify_tol = 1e-9;
+    o->pivot_rel = 1e-9;
+    o->tol32 = 1e-5;
+    o->licq32 = 1e-6;
+    o->pivot_rel32 = 1e-3;
 }
 
 static const MpqpOptions *opts(void) {
@@ -165,15 +172,6 @@ static int half_pow2(int n) {
     return (int)((m + 1u) >> 1);
 }
 
-/* Tree sum in the reference's order (:231-236, :301-306); destroys v. */
-static double tree_sum(double *v, int n) {
-    if (n <= 0) return 0.0;
-    for (int i = half_pow2(n); i > 0; i >>= 1)
-        for (int t = 0; t < i; t++)
-            if (t + i < n) v[t] = v[t] + v[t + i];
-    return v[0];
-}
-
 /* dual_min_idx_kernel combine predicate (:177-198): replace slot t by t+i. */
 static int dmin_replace(double vt, double vu, int ct, int cu, int mode_b, double stop_tol) {
     if (!mode_b) return (vt > vu) || (vt > vu - 1e-10 && ct > cu);
@@ -194,42 +192,6 @@ static int dratio_replace(double vt, int it, double vu, int iu, int ct, int cu, 
 /* min_idx_kernel_optimal combine predicate (:343-344). */
 static int p2price_replace(double vt, double vu, int ct, int cu) {
     return (vt > vu) || (ct > cu && vu < -1e-10);
-}
-
-/* ------------------------------------------------------------------ */
-/* Pivot                                                               */
-/* ------------------------------------------------------------------ */
-
-/* Condensed-tableau pivot on (r, q) over rows [0, nrows) and columns
- * [0, ncols) of a row-major tableau with leading dimension ld.  Row r is
- * divided by p; column q becomes -T[i][q]/p (as fma(-f, 1/p, 0)); every
- * other entry becomes fma(-T[i][q], T'[r][j], T[i][j]). */
-static void pivot(double *T, int ld, int nrows, int ncols, int r, int q, double *colq) {
-    double p = T[(size_t)r * ld + q];
-    for (int i = 0; i < nrows; i++) colq[i] = T[(size_t)i * ld + q];
-    double *tr = T + (size_t)r * ld;
-    for (int j = 0; j < ncols; j++)
-        if (j != q) tr[j] = tr[j] / p;
-    tr[q] = 1.0 / p;
-    for (int i = 0; i < nrows; i++) {
-        if (i == r) continue;
-        double *ti = T + (size_t)i * ld;
-        double f = colq[i];
-        ti[q] = 0.0;
-        for (int j = 0; j < ncols; j++) ti[j] = fma(-f, tr[j], ti[j]);
-    }
-}
-
-/* Reduced-cost row: T[M][j] = cN[j] - sum_i cb[i] T[i][j] (ascending i,
- * one fma per nonzero cb[i]). */
-static void cost_row(double *T, int ld, int M, int ncols, const double *cb, const double *cN) {
-    double *tc = T + (size_t)M * ld;
-    for (int j = 0; j < ncols; j++) {
-        double acc = cN[j];
-        for (int i = 0; i < M; i++)
-            if (cb[i] != 0.0) acc = fma(-cb[i], T[(size_t)i * ld + j], acc);
-        tc[j] = acc;
-    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -269,6 +231,117 @@ static double slack_entry(const Prob *P, int j, int row) {
     return a2(P, row, W_VAR(P, j));
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Feasibility LP                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double *T;   /* (M1+1) x ld1 */
+    int *basis;  /* M1 */
+    int *nonb;   /* K1 */
+} FState;
+
+
+typedef struct {
+    double *colq, *cb, *cN, *red;
+    int *ired, *where, *bcnt, *nbcnt;
+    char *inS;
+    double *cx, *cr, *cd, *cy, *ccb;  /* certification vectors (M2 each) */
+} Scratch;
+
+static void scratch_init(Scratch *s, const Prob *P) {
+    int R = (P->M2 > P->M1 ? P->M2 : P->M1) + 2;
+    int C = P->N2 + P->N1 + 8;
+    s->colq = (double *)malloc(sizeof(double) * R);
+    s->cb = (double *)malloc(sizeof(double) * R);
+    s->cN = (double *)malloc(sizeof(double) * C);
+    s->red = (double *)malloc(sizeof(double) * C);
+    s->ired = (int *)malloc(sizeof(int) * C);
+    s->where = (int *)malloc(sizeof(int) * C);
+    s->bcnt = (int *)malloc(sizeof(int) * C);
+    s->nbcnt = (int *)malloc(sizeof(int) * C);
+    s->inS = (char *)malloc((size_t)P->n_c + 1);
+    s->cx = (double *)malloc(sizeof(double) * R);
+    s->cr = (double *)malloc(sizeof(double) * R);
+    s->cd = (double *)malloc(sizeof(double) * R);
+    s->cy = (double *)malloc(sizeof(double) * R);
+    s->ccb = (double *)malloc(sizeof(double) * R);
+}
+
+static void scratch_free(Scratch *s) {
+    free(s->colq); free(s->cb); free(s->cN); free(s->red); free(s->ired);
+    free(s->where); free(s->bcnt); free(s->nbcnt); free(s->inS);
+    free(s->cx); free(s->cr); free(s->cd); free(s->cy); free(s->ccb);
+}
+
+static void build_where(const Prob *P, const int *basis, const int *nonb, int *where) {
+    for (int v = 0; v < P->N2; v++) where[v] = INT32_MIN;
+    for (int i = 0; i < P->M2; i++) where[basis[i]] = i;
+    for (int k = 0; k < P->K2; k++) where[nonb[k]] = -(k + 1);
+}
+
+/* Tolerances and caps of one simplex pass (FP64 path, or the FP32 pivots
+ * of the mixed-precision path). */
+typedef struct {
+    double tol_plus, tol_minus, tol_minus_opt, dual_stop_tol, dual_feas_tol;
+    double licq, pivot_rel, final_tol;
+    int max_iter_p1, max_iter_p2, stall_perturb, stall_exit;
+} LpTol;
+
+static LpTol lptol64(const MpqpOptions *o) {
+    LpTol t;
+    t.tol_plus = o->tol_plus;
+    t.tol_minus = o->tol_minus;
+    t.tol_minus_opt = o->tol_minus_optimal;
+    t.dual_stop_tol = o->dual_stop_tol;
+    t.dual_feas_tol = o->dual_feasible_tol;
+    t.licq = o->licq_tol;
+    t.pivot_rel = o->pivot_rel;
+    t.final_tol = o->final_tol;
+    t.max_iter_p1 = o->max_iter_phase1;
+    t.max_iter_p2 = o->max_iter_phase2;
+    t.stall_perturb = o->stall_perturb;
+    t.stall_exit = o->stall_exit;
+    return t;
+}
+
+/* FP32 pass: pivot and stop tolerances at tol32, LICQ screen at licq32;
+ * its verdicts are not used (the FP64 certification decides). */
+static LpTol lptol32(const MpqpOptions *o) {
+    LpTol t = lptol64(o);
+    t.tol_plus = o->tol32;
+    t.tol_minus = -o->tol32;
+    t.tol_minus_opt = -o->tol32;
+    t.dual_stop_tol = -o->tol32;
+    t.dual_feas_tol = o->tol32;
+    t.licq = o->licq32;
+    t.pivot_rel = o->pivot_rel32;
+    return t;
+}
+
+typedef struct {
+    int p1_status, p1_iters, kkt_feasible;
+    int p2_status, p2_iters, final_set;
+    double z1, z2;
+    int p1_cleanup;  /* primal pivots that restored dual feasibility */
+} OptOut;
+
+#define REAL double
+#define RS(x) x##_d
+#define FMA fma
+#include "opt_lp.inc"
+#undef REAL
+#undef RS
+#undef FMA
+#define REAL float
+#define RS(x) x##_f
+#define FMA fmaf
+#include "opt_lp.inc"
+#undef REAL
+#undef RS
+#undef FMA
+
 /* Gauss-Jordan build of the S = {} KKT basis b2 from the identity basis of
  * the e columns: w_j at rows R5_j first (pivots +-1), then x'_i at the
  * unused R4 row of largest |entry| (lowest row on ties). */
@@ -295,7 +368,7 @@ static int build_root(Prob *P, const MpqpOptions *o) {
     for (int j = 0; j < n_c && ok; j++) {
         int r = M - n_c + j, q = n_x + P->n_t + j;
         if (!(fabs(G[(size_t)r * ld + q]) > o->licq_tol)) { ok = 0; break; }
-        pivot(G, ld, M, K + 2, r, q, colq);
+        pivot_d(G, ld, M, K + 2, r, q, colq);
         int t = rowvar[r]; rowvar[r] = slotvar[q]; slotvar[q] = t;
     }
     int r4 = M - n_c - n_x;
@@ -308,7 +381,7 @@ static int build_root(Prob *P, const MpqpOptions *o) {
             if (a > best) { best = a; r = rr; }
         }
         if (r < 0 || !(best > o->licq_tol)) { ok = 0; break; }
-        pivot(G, ld, M, K + 2, r, i, colq);
+        pivot_d(G, ld, M, K + 2, r, i, colq);
         used[r] = 1;
         int t = rowvar[r]; rowvar[r] = slotvar[i]; slotvar[i] = t;
     }
@@ -347,47 +420,6 @@ static int build_root(Prob *P, const MpqpOptions *o) {
     return P->root_ok;
 }
 
-/* ------------------------------------------------------------------ */
-/* Feasibility LP                                                      */
-/* ------------------------------------------------------------------ */
-
-typedef struct {
-    double *T;   /* (M1+1) x ld1 */
-    int *basis;  /* M1 */
-    int *nonb;   /* K1 */
-} FState;
-
-typedef struct {
-    double *T;   /* (M2+1) x ld2 */
-    int *basis;  /* M2 */
-    int *nonb;   /* K2 */
-} OState;
-
-typedef struct {
-    double *colq, *cb, *cN, *red;
-    int *ired, *where, *bcnt, *nbcnt;
-    char *inS;
-} Scratch;
-
-static void scratch_init(Scratch *s, const Prob *P) {
-    int R = (P->M2 > P->M1 ? P->M2 : P->M1) + 2;
-    int C = P->N2 + P->N1 + 8;
-    s->colq = (double *)malloc(sizeof(double) * R);
-    s->cb = (double *)malloc(sizeof(double) * R);
-    s->cN = (double *)malloc(sizeof(double) * C);
-    s->red = (double *)malloc(sizeof(double) * C);
-    s->ired = (int *)malloc(sizeof(int) * C);
-    s->where = (int *)malloc(sizeof(int) * C);
-    s->bcnt = (int *)malloc(sizeof(int) * C);
-    s->nbcnt = (int *)malloc(sizeof(int) * C);
-    s->inS = (char *)malloc((size_t)P->n_c + 1);
-}
-
-static void scratch_free(Scratch *s) {
-    free(s->colq); free(s->cb); free(s->cN); free(s->red); free(s->ired);
-    free(s->where); free(s->bcnt); free(s->nbcnt); free(s->inS);
-}
-
 static void feas_cold(const Prob *P, FState *F) {
     int M = P->M1, K = P->K1, ld = P->ld1;
     for (int i = 0; i < M; i++) {
@@ -411,7 +443,7 @@ static void feas_solve(const Prob *P, const MpqpOptions *o, FState *F, Scratch *
     for (int i = 0; i < M; i++) s->cb[i] = c_feas(P, s->inS, F->basis[i]);
     for (int k = 0; k < K; k++) s->cN[k] = c_feas(P, s->inS, F->nonb[k]);
     s->cN[K] = 0.0;
-    cost_row(T, ld, M, K + 1, s->cb, s->cN);
+    cost_row_d(T, ld, M, K + 1, s->cb, s->cN);
     *status = MPQP_ST_ITER_CAP;
     *iters = 0;
     *z = 0.0;
@@ -425,22 +457,15 @@ static void feas_solve(const Prob *P, const MpqpOptions *o, FState *F, Scratch *
         }
         if (tc[q] > o->tol_minus_1) {
             for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * T[(size_t)i * ld + K];
-            *z = tree_sum(s->red, M);
+            *z = tree_sum_d(s->red, M);
             *feasible = (*z >= -o->feas_z_tol && *z <= o->feas_z_tol);
             *status = MPQP_ST_OPTIMAL;
             return;
         }
-        int r = -1;
-        double best = 0.0;
-        for (int i = 0; i < M; i++) {
-            double a = T[(size_t)i * ld + q];
-            if (a > o->tol_plus) {
-                double rt = T[(size_t)i * ld + K] / a;
-                if (r < 0 || rt < best) { r = i; best = rt; }
-            }
-        }
+        LpTol tl = lptol64(o);
+        int r = primal_ratio_d(T, ld, M, q, K, &tl);
         if (r < 0) { *status = MPQP_ST_UNBOUNDED; return; }
-        pivot(T, ld, M + 1, K + 1, r, q, s->colq);
+        pivot_d(T, ld, M + 1, K + 1, r, q, s->colq);
         int t = F->basis[r]; F->basis[r] = F->nonb[q]; F->nonb[q] = t;
         s->cb[r] = c_feas(P, s->inS, F->basis[r]);
         (*iters)++;
@@ -451,223 +476,7 @@ static void feas_solve(const Prob *P, const MpqpOptions *o, FState *F, Scratch *
 /* Optimality LP                                                       */
 /* ------------------------------------------------------------------ */
 
-static void build_where(const Prob *P, const int *basis, const int *nonb, int *where) {
-    for (int v = 0; v < P->N2; v++) where[v] = INT32_MIN;
-    for (int i = 0; i < P->M2; i++) where[basis[i]] = i;
-    for (int k = 0; k < P->K2; k++) where[nonb[k]] = -(k + 1);
-}
 
-/* temp column (K2+2) := B^-1 lambda_j, assembled from the columns of B^-1,
- * i.e. the e-variable columns of the tableau (unit vectors when basic). */
-static void lambda_column(const Prob *P, OState *O, int j, const int *where) {
-    int M = P->M2, ld = P->ld2, tmp = P->K2 + 2;
-    double *T = O->T;
-    for (int i = 0; i < M; i++) T[(size_t)i * ld + tmp] = 0.0;
-    for (int rho = 0; rho < M; rho++) {
-        double a = lam_entry(P, j, rho);
-        if (a == 0.0) continue;
-        int w = where[P->N2 - M + rho];
-        if (w >= 0) {
-            T[(size_t)w * ld + tmp] = fma(a, 1.0, T[(size_t)w * ld + tmp]);
-        } else {
-            int k = -w - 1;
-            for (int i = 0; i < M; i++)
-                T[(size_t)i * ld + tmp] = fma(a, T[(size_t)i * ld + k], T[(size_t)i * ld + tmp]);
-        }
-    }
-}
-
-/* Bring lambda_j into the basis in place of s_j (same variable id w_j).
- * Returns 0 if the pivot magnitude is <= licq_tol (singular basis). */
-static int lambda_swap(const Prob *P, const MpqpOptions *o, OState *O, int j, Scratch *s) {
-    int M = P->M2, ld = P->ld2, tmp = P->K2 + 2;
-    build_where(P, O->basis, O->nonb, s->where);
-    lambda_column(P, O, j, s->where);
-    int w = s->where[W_VAR(P, j)];
-    if (w >= 0) {
-        double pv = O->T[(size_t)w * ld + tmp];
-        if (!(fabs(pv) > o->licq_tol)) return 0;
-        pivot(O->T, ld, M, P->K2 + 3, w, tmp, s->colq);
-    } else {
-        int k = -w - 1;
-        for (int i = 0; i < M; i++) O->T[(size_t)i * ld + k] = O->T[(size_t)i * ld + tmp];
-    }
-    return 1;
-}
-
-static void opt_cold(const Prob *P, OState *O) {
-    int M = P->M2, ld = P->ld2;
-    for (int i = 0; i < M; i++) {
-        memcpy(O->T + (size_t)i * ld, P->root + (size_t)i * ld, sizeof(double) * (size_t)(P->K2 + 2));
-        O->T[(size_t)i * ld + P->K2 + 2] = 0.0;
-    }
-    memcpy(O->basis, P->root_basis, sizeof(int) * (size_t)M);
-    memcpy(O->nonb, P->root_nonb, sizeof(int) * (size_t)P->K2);
-}
-
-static double c_ph1(const Prob *P, int var) { return var >= P->N_ ? 1.0 : 0.0; }
-static double c_ph2(const Prob *P, int var) { return var == EPS_VAR(P) ? -1.0 : 0.0; }
-
-typedef struct {
-    int p1_status, p1_iters, kkt_feasible;
-    int p2_status, p2_iters, final_set;
-    double z1, z2;
-} OptOut;
-
-/* Phase 1 (dual simplex) from the basis held in O. */
-static void phase1(const Prob *P, const MpqpOptions *o, OState *O, Scratch *s, OptOut *out) {
-    int M = P->M2, K = P->K2, ld = P->ld2, N2 = P->N2;
-    int rhs = K, pert = K + 1;
-    double *T = O->T;
-    for (int i = 0; i < M; i++) s->cb[i] = c_ph1(P, O->basis[i]);
-    for (int k = 0; k < K; k++) s->cN[k] = c_ph1(P, O->nonb[k]);
-    s->cN[K] = 0.0;
-    s->cN[K + 1] = 0.0;
-    cost_row(T, ld, M, K + 2, s->cb, s->cN);
-    for (int v = 0; v < N2; v++) { s->bcnt[v] = 0; s->nbcnt[v] = 0; }
-    int n_same = 0;
-    double best_p = -INFINITY;
-    double *xb = (double *)malloc(sizeof(double) * (size_t)M);
-    double *vals = (double *)malloc(sizeof(double) * (size_t)(M > K ? M : K));
-    int *idx = (int *)malloc(sizeof(int) * (size_t)(M > K ? M : K));
-    out->p1_status = MPQP_ST_ITER_CAP;
-    out->p1_iters = 0;
-    out->kkt_feasible = 0;
-    out->z1 = 0.0;
-    for (int it = 0; it < o->max_iter_phase1; it++) {
-        int perturbed = n_same >= o->stall_perturb;
-        for (int i = 0; i < M; i++) {
-            double v = T[(size_t)i * ld + rhs];
-            xb[i] = perturbed ? v + T[(size_t)i * ld + pert] : v;
-        }
-        /* leaving row */
-        int mode_b = (n_same % 20) >= 10;
-        for (int i = 0; i < M; i++) { vals[i] = xb[i]; idx[i] = i; }
-        for (int h = half_pow2(M); h > 0; h >>= 1)
-            for (int t = 0; t < h; t++)
-                if (t + h < M &&
-                    dmin_replace(vals[t], vals[t + h], s->bcnt[O->basis[idx[t]]],
-                                 s->bcnt[O->basis[idx[t + h]]], mode_b, o->dual_stop_tol)) {
-                    vals[t] = vals[t + h];
-                    idx[t] = idx[t + h];
-                }
-        int r = idx[0];
-        s->bcnt[O->basis[r]] += 1;
-        int go_on = 0;
-        for (int i = 0; i < M; i++)
-            if (xb[i] < o->dual_stop_tol) go_on = 1;
-        if (!go_on) {
-            for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * xb[i];
-            out->z1 = tree_sum(s->red, M);
-            int f = (out->z1 > -o->dual_feasible_tol && out->z1 < o->dual_feasible_tol);
-            for (int i = 0; i < M; i++)
-                if (O->basis[i] >= P->N_) f = 0;
-            out->kkt_feasible = f;
-            out->p1_status = MPQP_ST_OPTIMAL;
-            break;
-        }
-        /* entering column */
-        int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
-        const double *tr = T + (size_t)r * ld;
-        const double *tc = T + (size_t)M * ld;
-        for (int k = 0; k < K; k++) {
-            double a = tr[k];
-            if (a < o->tol_minus) {
-                idx[k] = O->nonb[k];
-                vals[k] = -tc[k] / a;
-            } else {
-                idx[k] = -1;
-                vals[k] = 100000.0;
-            }
-        }
-        for (int h = half_pow2(K); h > 0; h >>= 1)
-            for (int t = 0; t < h; t++)
-                if (t + h < K) {
-                    int it_ = idx[t], iu = idx[t + h];
-                    int ct = it_ >= 0 ? s->nbcnt[it_] : 0;
-                    int cu = iu >= 0 ? s->nbcnt[iu] : 0;
-                    if (dratio_replace(vals[t], it_, vals[t + h], iu, ct, cu, mode)) {
-                        vals[t] = vals[t + h];
-                        idx[t] = idx[t + h];
-                    }
-                }
-        int wv = idx[0];
-        int q = -1;
-        for (int k = 0; k < K; k++)
-            if (O->nonb[k] == wv) { q = k; break; }
-        if (g_trace && it < 256) fprintf(stderr, "oracle it %d r %d q %d n_same %d\n", it, r, q, n_same);
-        if (wv < 0) { out->p1_status = MPQP_ST_UNBOUNDED; break; }
-        s->nbcnt[wv] += 1;
-        /* infeasibility progress of this iteration (before the pivot) */
-        for (int i = 0; i < M; i++) s->red[i] = xb[i] < 0.0 ? xb[i] : 0.0;
-        double pinf = tree_sum(s->red, M);
-        pivot(T, ld, M + 1, K + 2, r, q, s->colq);
-        int t = O->basis[r]; O->basis[r] = O->nonb[q]; O->nonb[q] = t;
-        s->cb[r] = c_ph1(P, O->basis[r]);
-        out->p1_iters++;
-        if (pinf > best_p) { best_p = pinf; n_same = 0; } else { n_same++; }
-        if (n_same > o->stall_exit) { out->p1_status = MPQP_ST_STALL; break; }
-    }
-    free(xb); free(vals); free(idx);
-}
-
-/* Phase 2 (primal simplex, min -eps over the first N_ columns). */
-static void phase2(const Prob *P, const MpqpOptions *o, OState *O, Scratch *s, OptOut *out) {
-    int M = P->M2, K = P->K2, ld = P->ld2, Nn = P->N_;
-    int rhs = K;
-    double *T = O->T;
-    for (int i = 0; i < M; i++) s->cb[i] = c_ph2(P, O->basis[i]);
-    for (int k = 0; k < K; k++) s->cN[k] = c_ph2(P, O->nonb[k]);
-    s->cN[K] = 0.0;
-    cost_row(T, ld, M, K + 1, s->cb, s->cN);
-    for (int v = 0; v < P->N2; v++) s->bcnt[v] = 0;
-    double *vals = (double *)malloc(sizeof(double) * (size_t)Nn);
-    int *idx = (int *)malloc(sizeof(int) * (size_t)Nn);
-    out->p2_status = MPQP_ST_ITER_CAP;
-    out->p2_iters = 0;
-    out->z2 = 0.0;
-    for (int it = 0; it < o->max_iter_phase2; it++) {
-        build_where(P, O->basis, O->nonb, s->where);
-        const double *tc = T + (size_t)M * ld;
-        for (int v = 0; v < Nn; v++) {
-            int w = s->where[v];
-            vals[v] = w >= 0 ? 100000.0 : tc[-w - 1];
-            idx[v] = v;
-        }
-        for (int h = half_pow2(Nn); h > 0; h >>= 1)
-            for (int t = 0; t < h; t++)
-                if (t + h < Nn &&
-                    p2price_replace(vals[t], vals[t + h], s->bcnt[idx[t]], s->bcnt[idx[t + h]])) {
-                    vals[t] = vals[t + h];
-                    idx[t] = idx[t + h];
-                }
-        int wv = idx[0];
-        s->bcnt[wv] += 1;
-        if (vals[0] > o->tol_minus_optimal) {
-            for (int i = 0; i < M; i++) s->red[i] = s->cb[i] * T[(size_t)i * ld + rhs];
-            out->z2 = tree_sum(s->red, M);
-            out->p2_status = MPQP_ST_OPTIMAL;
-            break;
-        }
-        int q = -s->where[wv] - 1;
-        int r = -1;
-        double best = 0.0;
-        for (int i = 0; i < M; i++) {
-            double a = T[(size_t)i * ld + q];
-            if (a > o->tol_plus) {
-                double rt = T[(size_t)i * ld + rhs] / a;
-                if (r < 0 || rt < best) { r = i; best = rt; }
-            }
-        }
-        if (r < 0) { out->p2_status = MPQP_ST_UNBOUNDED; break; }
-        pivot(T, ld, M + 1, K + 1, r, q, s->colq);
-        int t = O->basis[r]; O->basis[r] = O->nonb[q]; O->nonb[q] = t;
-        s->cb[r] = c_ph2(P, O->basis[r]);
-        out->p2_iters++;
-    }
-    out->final_set = (out->p2_status == MPQP_ST_OPTIMAL && -out->z2 > o->final_tol);
-    free(vals); free(idx);
-}
 
 /* ------------------------------------------------------------------ */
 /* Candidate evaluation                                                */
@@ -682,9 +491,312 @@ typedef struct {
  * states (fpar / opar may be NULL; opar_ok: parent phase 1 feasible).
  * The candidate's final feasibility state is left in F and its phase-1
  * final state in O1 (when kkt feasible). */
+/* ------------------------------------------------------------------ */
+/* FP64 certification of an FP32 basis (mixed precision)               */
+/* ------------------------------------------------------------------ */
+
+#define CERT_STEPS 3
+
+/* diagnostics: why certifications failed (test infrastructure) */
+static long long g_cert_fail[16];
+#define CFAIL(k) do { __atomic_fetch_add(&g_cert_fail[k], 1, __ATOMIC_RELAXED); return 0; } while (0)
+void oracle_cert_stats(long long *out, int reset) {
+    for (int i = 0; i < 16; i++) { out[i] = g_cert_fail[i]; if (reset) g_cert_fail[i] = 0; }
+}
+
+/* A2 entry of variable v at row rho under the S-mask of the w columns
+ * (initialise_A_kernel_1 :635-650). */
+static double a2m(const Prob *P, const char *inS, int rho, int v) {
+    int w0 = P->n_x + P->n_t;
+    if (v >= w0 && v < w0 + P->n_c) {
+        int j = v - w0;
+        if (inS[j]) {
+            if (rho == P->M2 - P->n_c + j) return 0.0;
+        } else {
+            int r4 = P->M2 - P->n_c - P->n_x;
+            if (rho >= r4 && rho < P->M2 - P->n_c) return 0.0;
+        }
+    }
+    return P->A2[(size_t)v * P->M2 + rho];
+}
+
+/* dx = B32^-1 r: columns of B^-1 are the e-variable tableau columns (unit
+ * vectors for basic e variables), FP32 values accumulated in FP64. */
+static void binv_apply(const Prob *P, const OSt_f *O, const int *where, const double *r, double *dx) {
+    int M = P->M2, ld = P->ld2;
+    for (int i = 0; i < M; i++) {
+        double acc = 0.0;
+        for (int rho = 0; rho < M; rho++) {
+            if (r[rho] == 0.0) continue;
+            int w = where[P->N2 - M + rho];
+            if (w >= 0) {
+                if (i == w) acc = fma(r[rho], 1.0, acc);
+            } else {
+                acc = fma(r[rho], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
+            }
+        }
+        dx[i] = acc;
+    }
+}
+
+/* out = B32^-T v (v indexed by basis position, out by row). */
+static void binvT_apply(const Prob *P, const OSt_f *O, const int *where, const double *v, double *out) {
+    int M = P->M2, ld = P->ld2;
+    for (int rho = 0; rho < M; rho++) {
+        int w = where[P->N2 - M + rho];
+        double acc = 0.0;
+        if (w >= 0) {
+            if (v[w] != 0.0) acc = fma(v[w], 1.0, acc);
+        } else {
+            int k = -w - 1;
+            for (int i = 0; i < M; i++)
+                if (v[i] != 0.0) acc = fma(v[i], (double)O->T[(size_t)i * ld + k], acc);
+        }
+        out[rho] = acc;
+    }
+}
+
+/* x := FP64 solution of B x = xb2 by iterative refinement from the FP32
+ * rhs column; returns 1 once max|xb2 - B x| <= 1e-12 (1 + max|xb2|). */
+static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const char *inS, double *x,
+                       double *r, double *dx) {
+    int M = P->M2, ld = P->ld2, RHS = P->K2;
+    double bmax = 0.0;
+    for (int rho = 0; rho < M; rho++)
+        if (fabs(P->xb2[rho]) > bmax) bmax = fabs(P->xb2[rho]);
+    double tol = 1e-12 * (1.0 + bmax);
+    for (int i = 0; i < M; i++) x[i] = (double)O->T[(size_t)i * ld + RHS];
+    for (int it = 0;; it++) {
+        double rn = 0.0;
+        for (int rho = 0; rho < M; rho++) {
+            double acc = P->xb2[rho];
+            for (int i = 0; i < M; i++) {
+                double a = a2m(P, inS, rho, O->basis[i]);
+                if (a != 0.0) acc = fma(-x[i], a, acc);
+            }
+            r[rho] = acc;
+            if (fabs(acc) > rn) rn = fabs(acc);
+        }
+        if (g_trace) fprintf(stderr, "cert_primal it %d rn %.3e tol %.3e\n", it, rn, tol);
+        if (rn <= tol) return 1;
+        if (it == CERT_STEPS) return 0;
+        binv_apply(P, O, where, r, dx);
+        for (int i = 0; i < M; i++) x[i] = x[i] + dx[i];
+    }
+}
+
+/* y := FP64 solution of B^T y = cb by iterative refinement; returns 1 once
+ * max|cb - B^T y| <= 2e-12. */
+static int cert_dual(const Prob *P, const OSt_f *O, const int *where, const char *inS, const double *cb,
+                     double *y, double *r, double *dy) {
+    int M = P->M2;
+    binvT_apply(P, O, where, cb, y);
+    for (int it = 0;; it++) {
+        double rn = 0.0;
+        for (int i = 0; i < M; i++) {
+            double acc = cb[i];
+            for (int rho = 0; rho < M; rho++) {
+                double a = a2m(P, inS, rho, O->basis[i]);
+                if (a != 0.0) acc = fma(-a, y[rho], acc);
+            }
+            r[i] = acc;
+            if (fabs(acc) > rn) rn = fabs(acc);
+        }
+        if (rn <= 2e-12) return 1;
+        if (it == CERT_STEPS) return 0;
+        binvT_apply(P, O, where, r, dy);
+        for (int rho = 0; rho < P->M2; rho++) y[rho] = y[rho] + dy[rho];
+    }
+}
+
+/* reduced cost c_v - a_v^T y of variable v */
+static double red_cost(const Prob *P, const char *inS, int v, double cv, const double *y) {
+    double acc = cv;
+    for (int rho = 0; rho < P->M2; rho++) {
+        double a = a2m(P, inS, rho, v);
+        if (a != 0.0) acc = fma(-a, y[rho], acc);
+    }
+    return acc;
+}
+
+/* Certifies the FP32 phase-1 result in FP64 and sets out->kkt_feasible,
+ * out->z1.  Returns 0 if the FP32 basis cannot be certified. */
+static int certify_phase1(const Prob *P, const MpqpOptions *o, const OSt_f *O, Scratch *s, OptOut *out) {
+    int M = P->M2;
+    if (out->p1_status != MPQP_ST_OPTIMAL) CFAIL(0);
+    build_where(P, O->basis, O->nonb, s->where);
+    double *x = s->cx, *r = s->cr, *d = s->cd, *y = s->cy, *cb = s->ccb;
+    if (!cert_primal(P, O, s->where, s->inS, x, r, d)) CFAIL(1);
+    for (int i = 0; i < M; i++)
+        if (x[i] < o->dual_stop_tol) CFAIL(2);
+    int any_art = 0;
+    for (int i = 0; i < M; i++) {
+        cb[i] = O->basis[i] >= P->N_ ? 1.0 : 0.0;
+        any_art |= cb[i] != 0.0;
+    }
+    if (any_art) {
+        if (!cert_dual(P, O, s->where, s->inS, cb, y, r, d)) CFAIL(3);
+        for (int k = 0; k < P->K2; k++) {
+            int v = O->nonb[k];
+            if (red_cost(P, s->inS, v, v >= P->N_ ? 1.0 : 0.0, y) < -o->certify_tol) CFAIL(4);
+        }
+    }
+    for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
+    out->z1 = tree_sum_d(r, M);
+    out->kkt_feasible = (out->z1 > -o->dual_feasible_tol && out->z1 < o->dual_feasible_tol);
+    return 1;
+}
+
+/* Certifies the FP32 phase-2 result in FP64 and sets out->z2,
+ * out->final_set.  Returns 0 if it cannot be certified. */
+static int certify_phase2(const Prob *P, const MpqpOptions *o, const OSt_f *O, Scratch *s, OptOut *out) {
+    int M = P->M2, eps = EPS_VAR(P);
+    if (out->p2_status != MPQP_ST_OPTIMAL) CFAIL(5);
+    build_where(P, O->basis, O->nonb, s->where);
+    double *x = s->cx, *r = s->cr, *d = s->cd, *y = s->cy, *cb = s->ccb;
+    if (!cert_primal(P, O, s->where, s->inS, x, r, d)) CFAIL(6);
+    for (int i = 0; i < M; i++)
+        if (x[i] < -o->certify_tol) CFAIL(7);
+    for (int i = 0; i < M; i++) cb[i] = O->basis[i] == eps ? -1.0 : 0.0;
+    if (!cert_dual(P, O, s->where, s->inS, cb, y, r, d)) CFAIL(8);
+    for (int k = 0; k < P->K2; k++) {
+        int v = O->nonb[k];
+        if (v >= P->N_) continue;
+        if (red_cost(P, s->inS, v, v == eps ? -1.0 : 0.0, y) < -o->certify_tol) CFAIL(9);
+    }
+    for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
+    out->z2 = tree_sum_d(r, M);
+    out->final_set = -out->z2 > o->final_tol;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* Candidate evaluation                                                */
+/* ------------------------------------------------------------------ */
+
+static void ocopy_d(const Prob *P, OSt_d *dst, const OSt_d *src) {
+    memcpy(dst->T, src->T, sizeof(double) * (size_t)(P->M2 + 1) * P->ld2);
+    memcpy(dst->basis, src->basis, sizeof(int) * (size_t)P->M2);
+    memcpy(dst->nonb, src->nonb, sizeof(int) * (size_t)P->K2);
+}
+static void ocopy_f(const Prob *P, OSt_f *dst, const OSt_f *src) {
+    memcpy(dst->T, src->T, sizeof(float) * (size_t)(P->M2 + 1) * P->ld2);
+    memcpy(dst->basis, src->basis, sizeof(int) * (size_t)P->M2);
+    memcpy(dst->nonb, src->nonb, sizeof(int) * (size_t)P->K2);
+}
+
+/* The FP64 optimality LP (phase 1, drive-out, phase 2).  O1 keeps the
+ * phase-1 state for children. */
+static void opt_fp64(const Prob *P, const MpqpOptions *o, const int *S, int k, const OSt_d *opar,
+                     int opar_ok, OSt_d *O1, OSt_d *O2, Scratch *s, MpqpRecord *rec, BasisOut *bo) {
+    LpTol t64 = lptol64(o);
+    OptOut out;
+    memset(&out, 0, sizeof out);
+    int started = 0;
+    if (opar && opar_ok && o->warm_start && k > 0) {
+        ocopy_d(P, O1, opar);
+        if (lambda_swap_d(P, &t64, O1, S[k - 1], s)) {
+            started = 1;
+            rec->p1_warm = 1;
+        }
+    }
+    if (!started) {
+        opt_cold_d(P, O1);
+        started = 1;
+        for (int t = 0; t < k; t++) {
+            if (!lambda_swap_d(P, &t64, O1, S[t], s)) { started = 0; break; }
+        }
+        if (!started) {
+            rec->p1_status = MPQP_ST_SINGULAR;
+            return;
+        }
+    }
+    phase1_d(P, &t64, O1, s, &out);
+    rec->p1_status = out.p1_status;
+    rec->p1_iters = out.p1_iters;
+    rec->kkt_feasible = out.kkt_feasible;
+    rec->z1 = out.z1;
+    if (out.kkt_feasible) rec->p1_clean = drive_out_d(P, &t64, O1, s);
+    if (bo && bo->b1) memcpy(bo->b1, O1->basis, sizeof(int) * (size_t)P->M2);
+    if (!out.kkt_feasible) return;
+    ocopy_d(P, O2, O1);
+    phase2_d(P, &t64, O2, s, &out);
+    rec->p2_status = out.p2_status;
+    rec->p2_iters = out.p2_iters;
+    rec->z2 = out.z2;
+    rec->final_set = out.final_set;
+    if (bo && bo->b2) memcpy(bo->b2, O2->basis, sizeof(int) * (size_t)P->M2);
+}
+
+/* Mixed precision: FP32 pivots from the parent's FP32 state, FP64
+ * certification; an LP whose FP32 result cannot be certified is solved
+ * again in FP64 from its cold start.  O1f keeps the phase-1 state (FP32)
+ * for children. */
+static void opt_mixed(const Prob *P, const MpqpOptions *o, const int *S, int k, const OSt_f *opar,
+                      int opar_ok, OSt_f *O1f, OSt_f *O2f, OSt_d *O1d, OSt_d *O2d, Scratch *s,
+                      MpqpRecord *rec, BasisOut *bo) {
+    LpTol t32 = lptol32(o);
+    OptOut out;
+    memset(&out, 0, sizeof out);
+    int started = 0, warm = 0, ok = 0;
+    if (opar && opar_ok && o->warm_start && k > 0) {
+        ocopy_f(P, O1f, opar);
+        if (lambda_swap_f(P, &t32, O1f, S[k - 1], s)) started = warm = 1;
+    }
+    if (!started) {
+        opt_cold_f(P, O1f);
+        started = 1;
+        for (int t = 0; t < k; t++)
+            if (!lambda_swap_f(P, &t32, O1f, S[t], s)) { started = 0; break; }
+    }
+    if (!started) __atomic_fetch_add(&g_cert_fail[10], 1, __ATOMIC_RELAXED);
+    if (started) {
+        phase1_f(P, &t32, O1f, s, &out);
+        ok = certify_phase1(P, o, O1f, s, &out);
+        if (ok && out.kkt_feasible) {
+            int clean = drive_out_f(P, &t32, O1f, s);
+            ocopy_f(P, O2f, O1f);
+            phase2_f(P, &t32, O2f, s, &out);
+            ok = certify_phase2(P, o, O2f, s, &out);
+            if (ok) {
+                rec->p1_clean = clean;
+                rec->p2_status = out.p2_status;
+                rec->p2_iters = out.p2_iters;
+                rec->z2 = out.z2;
+                rec->final_set = out.final_set;
+                if (bo && bo->b2) memcpy(bo->b2, O2f->basis, sizeof(int) * (size_t)P->M2);
+            }
+        }
+        if (ok) {
+            rec->p1_status = out.p1_status;
+            rec->p1_iters = out.p1_iters;
+            rec->kkt_feasible = out.kkt_feasible;
+            rec->z1 = out.z1;
+            rec->p1_warm = warm;
+            if (bo && bo->b1) memcpy(bo->b1, O1f->basis, sizeof(int) * (size_t)P->M2);
+        }
+    }
+    if (ok) return;
+    /* FP64 re-solve from the cold start */
+    rec->repivots = 1;
+    rec->p1_clean = 0;
+    rec->p2_status = rec->p2_iters = rec->final_set = 0;
+    rec->z2 = 0.0;
+    opt_fp64(P, o, S, k, NULL, 0, O1d, O2d, s, rec, bo);
+    if (rec->kkt_feasible) {
+        for (size_t e = 0; e < (size_t)(P->M2 + 1) * P->ld2; e++) O1f->T[e] = (float)O1d->T[e];
+        memcpy(O1f->basis, O1d->basis, sizeof(int) * (size_t)P->M2);
+        memcpy(O1f->nonb, O1d->nonb, sizeof(int) * (size_t)P->K2);
+    }
+}
+
+/* Evaluate one candidate S (|S| = k, ascending) given its parent's final
+ * states (fpar / opar may be NULL; opar_ok: parent phase-1 state clean).
+ * opar is an OSt_d (FP64) or an OSt_f (mixed precision).  The candidate's
+ * feasibility state is left in F and its phase-1 state in O1d / O1f. */
 static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, int k,
-                           const FState *fpar, const OState *opar, int opar_ok,
-                           FState *F, OState *O1, OState *O2, Scratch *s,
+                           const FState *fpar, const void *opar, int opar_ok, FState *F,
+                           OSt_d *O1d, OSt_d *O2d, OSt_f *O1f, OSt_f *O2f, Scratch *s,
                            MpqpRecord *rec, BasisOut *bo) {
     int M1 = P->M1, M2 = P->M2;
     memset(s->inS, 0, (size_t)P->n_c);
@@ -696,11 +808,12 @@ static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, in
     rec->p1_warm = 0;
     rec->z_feas = rec->z1 = rec->z2 = 0.0;
     rec->repivots = 0;
+    rec->p1_clean = 0;
     if (bo && bo->fb) for (int i = 0; i < M1; i++) bo->fb[i] = -1;
     if (bo && bo->b1) for (int i = 0; i < M2; i++) bo->b1[i] = -1;
     if (bo && bo->b2) for (int i = 0; i < M2; i++) bo->b2[i] = -1;
 
-    /* feasibility LP */
+    /* feasibility LP (FP64 in both precisions) */
     if (fpar && o->warm_start) {
         memcpy(F->T, fpar->T, sizeof(double) * (size_t)(M1 + 1) * P->ld1);
         memcpy(F->basis, fpar->basis, sizeof(int) * (size_t)M1);
@@ -717,53 +830,14 @@ static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, in
     rec->feasible = fe;
     if (bo && bo->fb) memcpy(bo->fb, F->basis, sizeof(int) * (size_t)M1);
     if (!fe) return;
-
-    /* optimality LP, phase 1 */
-    OptOut out;
-    memset(&out, 0, sizeof out);
-    int started = 0;
     if (!P->root_ok) {
         rec->p1_status = MPQP_ST_SINGULAR;
         return;
     }
-    if (opar && opar_ok && o->warm_start && k > 0) {
-        memcpy(O1->T, opar->T, sizeof(double) * (size_t)(M2 + 1) * P->ld2);
-        memcpy(O1->basis, opar->basis, sizeof(int) * (size_t)M2);
-        memcpy(O1->nonb, opar->nonb, sizeof(int) * (size_t)P->K2);
-        if (lambda_swap(P, o, O1, S[k - 1], s)) {
-            started = 1;
-            rec->p1_warm = 1;
-        }
-    }
-    if (!started) {
-        opt_cold(P, O1);
-        started = 1;
-        for (int t = 0; t < k; t++) {
-            if (!lambda_swap(P, o, O1, S[t], s)) { started = 0; break; }
-        }
-        if (!started) {
-            rec->p1_status = MPQP_ST_SINGULAR;
-            return;
-        }
-    }
-    phase1(P, o, O1, s, &out);
-    rec->p1_status = out.p1_status;
-    rec->p1_iters = out.p1_iters;
-    rec->kkt_feasible = out.kkt_feasible;
-    rec->z1 = out.z1;
-    if (bo && bo->b1) memcpy(bo->b1, O1->basis, sizeof(int) * (size_t)M2);
-    if (!out.kkt_feasible) return;
-
-    /* phase 2 on a copy (O1 keeps the phase-1 state for children) */
-    memcpy(O2->T, O1->T, sizeof(double) * (size_t)(M2 + 1) * P->ld2);
-    memcpy(O2->basis, O1->basis, sizeof(int) * (size_t)M2);
-    memcpy(O2->nonb, O1->nonb, sizeof(int) * (size_t)P->K2);
-    phase2(P, o, O2, s, &out);
-    rec->p2_status = out.p2_status;
-    rec->p2_iters = out.p2_iters;
-    rec->z2 = out.z2;
-    rec->final_set = out.final_set;
-    if (bo && bo->b2) memcpy(bo->b2, O2->basis, sizeof(int) * (size_t)M2);
+    if (o->precision == MPQP_MIXED)
+        opt_mixed(P, o, S, k, (const OSt_f *)opar, opar_ok, O1f, O2f, O1d, O2d, s, rec, bo);
+    else
+        opt_fp64(P, o, S, k, (const OSt_d *)opar, opar_ok, O1d, O2d, s, rec, bo);
 }
 
 /* ------------------------------------------------------------------ */
@@ -820,12 +894,40 @@ static void fstate_alloc(FState *F, const Prob *P) {
     F->nonb = (int *)malloc(sizeof(int) * (size_t)P->K1);
 }
 static void fstate_free(FState *F) { free(F->T); free(F->basis); free(F->nonb); }
-static void ostate_alloc(OState *O, const Prob *P) {
-    O->T = (double *)calloc((size_t)(P->M2 + 1) * P->ld2, sizeof(double));
-    O->basis = (int *)malloc(sizeof(int) * (size_t)P->M2);
-    O->nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
+/* A phase-1 state in the precision of the run (FP64 or FP32). */
+typedef struct {
+    OSt_d d;
+    OSt_f f;
+} OPack;
+
+static void opack_alloc(OPack *O, const Prob *P, int which /* 1: d, 2: f, 3: both */) {
+    memset(O, 0, sizeof *O);
+    if (which & 1) {
+        O->d.T = (double *)calloc((size_t)(P->M2 + 1) * P->ld2, sizeof(double));
+        O->d.basis = (int *)malloc(sizeof(int) * (size_t)P->M2);
+        O->d.nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
+    }
+    if (which & 2) {
+        O->f.T = (float *)calloc((size_t)(P->M2 + 1) * P->ld2, sizeof(float));
+        O->f.basis = (int *)malloc(sizeof(int) * (size_t)P->M2);
+        O->f.nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
+    }
 }
-static void ostate_free(OState *O) { free(O->T); free(O->basis); free(O->nonb); }
+static void opack_free(OPack *O) {
+    free(O->d.T); free(O->d.basis); free(O->d.nonb);
+    free(O->f.T); free(O->f.basis); free(O->f.nonb);
+    memset(O, 0, sizeof *O);
+}
+static int opack_used(const OPack *O) { return O->d.T != NULL || O->f.T != NULL; }
+/* keeps the candidate's phase-1 state for its children */
+static void opack_keep(OPack *dst, const Prob *P, int mixed, const OSt_d *Od, const OSt_f *Of) {
+    opack_alloc(dst, P, mixed ? 2 : 1);
+    if (mixed) ocopy_f(P, &dst->f, Of);
+    else ocopy_d(P, &dst->d, Od);
+}
+static const void *opack_ptr(const OPack *O, int mixed) {
+    return mixed ? (const void *)&O->f : (const void *)&O->d;
+}
 
 /* ------------------------------------------------------------------ */
 /* Level-wise enumeration: mpqp()                                      */
@@ -862,6 +964,39 @@ static double now_s(void) {
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Mixed-precision policy: the empty set and every singleton (levels 0
+ * and 1, n_c + 1 LPs) are solved with FP32 pivots; if more than a quarter
+ * of their optimality LPs fail FP64 certification, the problem is too
+ * ill-conditioned for FP32 pivots and is solved in FP64 throughout.
+ * Returns 1 to keep mixed precision. */
+#define MIXED_PROBE_MAX_FALLBACK 0.25
+static int mixed_probe(const Prob *P, const MpqpOptions *o) {
+    Scratch s;
+    scratch_init(&s, P);
+    FState F0, F1;
+    fstate_alloc(&F0, P);
+    fstate_alloc(&F1, P);
+    OPack W0, W1, W2;
+    opack_alloc(&W0, P, 3);
+    opack_alloc(&W1, P, 3);
+    opack_alloc(&W2, P, 3);
+    MpqpRecord r0, r1;
+    int n_opt = 0, n_fb = 0;
+    eval_candidate(P, o, NULL, 0, NULL, NULL, 0, &F0, &W0.d, &W2.d, &W0.f, &W2.f, &s, &r0, NULL);
+    if (r0.feasible) { n_opt++; n_fb += r0.repivots; }
+    if (r0.feasible) {
+        for (int j = 0; j < P->n_c; j++) {
+            int S[1] = {j};
+            eval_candidate(P, o, S, 1, &F0, &W0.f, r0.p1_clean, &F1, &W1.d, &W2.d, &W1.f, &W2.f, &s, &r1, NULL);
+            if (r1.feasible) { n_opt++; n_fb += r1.repivots; }
+        }
+    }
+    fstate_free(&F0); fstate_free(&F1);
+    opack_free(&W0); opack_free(&W1); opack_free(&W2);
+    scratch_free(&s);
+    return !(n_opt > 0 && (double)n_fb > MIXED_PROBE_MAX_FALLBACK * (double)n_opt);
+}
+
 GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
                  int extra_batches, double *A1, double *xb1, int *b1, int M1, int N1,
                  double *A2, double *xb2, int *b2, int *non_b2, int M2, int N2,
@@ -874,13 +1009,21 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
     if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
         return NULL;
     double t0 = now_s();
+    MpqpOptions o_run = *o;
+    int mixed_used = 0;
+    if (o->precision == MPQP_MIXED && P.root_ok) {
+        mixed_used = mixed_probe(&P, o);
+        if (!mixed_used) o_run.precision = MPQP_FP64;
+    }
+    o = &o_run;
     rs_free(&g_rs);
     int kw = k_max > 0 ? k_max : 1;
 
     /* level storage: previous level's candidates, states */
     int prev_n = 0, *prev_sets = NULL, *prev_feas = NULL, *prev_kkt = NULL, *prev_rec = NULL;
     FState *prev_F = NULL;
-    OState *prev_O = NULL;
+    OPack *prev_O = NULL;
+    const int mixed = o->precision == MPQP_MIXED;
     int total = 0, cap = 0;
     MpqpRecord *all = NULL;
     int *allsets = NULL, *allfb = NULL, *allb1 = NULL, *allb2 = NULL;
@@ -919,7 +1062,7 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
         n_batches++;
         int keep = lvl < k_max; /* children need this level's states */
         FState *curF = (FState *)calloc((size_t)n, sizeof(FState));
-        OState *curO = (OState *)calloc((size_t)n, sizeof(OState));
+        OPack *curO = (OPack *)calloc((size_t)n, sizeof(OPack));
         int *feas = (int *)calloc((size_t)n, sizeof(int));
         int *kkt = (int *)calloc((size_t)n, sizeof(int));
         if (total + n > cap) {
@@ -938,10 +1081,10 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
             Scratch s;
             scratch_init(&s, &P);
             FState F;
-            OState O1, O2;
+            OPack W1, W2;
             fstate_alloc(&F, &P);
-            ostate_alloc(&O1, &P);
-            ostate_alloc(&O2, &P);
+            opack_alloc(&W1, &P, 3);
+            opack_alloc(&W2, &P, 3);
             #pragma omp for schedule(dynamic, 4)
             for (int c = 0; c < n; c++) {
                 MpqpRecord *rec = &all[total + c];
@@ -953,27 +1096,23 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
                 }
                 int pi = par[c];
                 eval_candidate(&P, o, sets + (size_t)c * kw, lvl,
-                               pi >= 0 ? &prev_F[pi] : NULL, pi >= 0 ? &prev_O[pi] : NULL,
-                               pi >= 0 ? prev_kkt[pi] : 0, &F, &O1, &O2, &s, rec, &bo);
+                               pi >= 0 ? &prev_F[pi] : NULL,
+                               pi >= 0 ? opack_ptr(&prev_O[pi], mixed) : NULL,
+                               pi >= 0 ? prev_kkt[pi] : 0, &F, &W1.d, &W2.d, &W1.f, &W2.f, &s, rec, &bo);
                 rec->parent = pi >= 0 ? prev_rec[pi] : -1;
                 feas[c] = rec->feasible;
-                kkt[c] = rec->kkt_feasible;
+                kkt[c] = rec->p1_clean;
                 if (keep && rec->feasible) {
                     fstate_alloc(&curF[c], &P);
                     memcpy(curF[c].T, F.T, sizeof(double) * (size_t)(M1 + 1) * P.ld1);
                     memcpy(curF[c].basis, F.basis, sizeof(int) * (size_t)M1);
                     memcpy(curF[c].nonb, F.nonb, sizeof(int) * (size_t)P.K1);
-                    if (rec->kkt_feasible) {
-                        ostate_alloc(&curO[c], &P);
-                        memcpy(curO[c].T, O1.T, sizeof(double) * (size_t)(M2 + 1) * P.ld2);
-                        memcpy(curO[c].basis, O1.basis, sizeof(int) * (size_t)M2);
-                        memcpy(curO[c].nonb, O1.nonb, sizeof(int) * (size_t)P.K2);
-                    }
+                    if (rec->p1_clean) opack_keep(&curO[c], &P, mixed, &W1.d, &W1.f);
                 }
             }
             fstate_free(&F);
-            ostate_free(&O1);
-            ostate_free(&O2);
+            opack_free(&W1);
+            opack_free(&W2);
             scratch_free(&s);
         }
         double dt = now_s() - tl;
@@ -984,7 +1123,7 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
         /* release previous level */
         for (int pi = 0; pi < prev_n; pi++) {
             if (prev_F[pi].T) fstate_free(&prev_F[pi]);
-            if (prev_O[pi].T) ostate_free(&prev_O[pi]);
+            if (opack_used(&prev_O[pi])) opack_free(&prev_O[pi]);
         }
         free(prev_F); free(prev_O); free(prev_sets); free(prev_feas); free(prev_kkt); free(prev_rec);
         prev_F = curF; prev_O = curO; prev_sets = sets; prev_feas = feas; prev_kkt = kkt;
@@ -996,7 +1135,7 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
     }
     for (int pi = 0; pi < prev_n; pi++) {
         if (prev_F[pi].T) fstate_free(&prev_F[pi]);
-        if (prev_O[pi].T) ostate_free(&prev_O[pi]);
+        if (opack_used(&prev_O[pi])) opack_free(&prev_O[pi]);
     }
     free(prev_F); free(prev_O); free(prev_sets); free(prev_feas); free(prev_kkt); free(prev_rec);
 
@@ -1022,9 +1161,11 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
         g_stats.pivots_feas += all[c].feas_iters;
         g_stats.pivots_p1 += all[c].p1_iters;
         g_stats.pivots_p2 += all[c].p2_iters;
+        g_stats.fallbacks += all[c].repivots;
     }
     g_stats.ms_feas = 1e3 * tf;
     g_stats.ms_opt = 1e3 * to;
+    g_stats.precision_used = o->precision;
     res->n_batches = n_batches;
     res->n_splits = 1;
     res->n_feasible_lps = n_feas;
@@ -1088,28 +1229,33 @@ int oracle_eval_set(const int *S, int k, int n_c, int n_x, int n_t, double *A1, 
     Prob P;
     if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
         return -1;
+    MpqpOptions o_run = *o;
+    if (o->precision == MPQP_MIXED && P.root_ok && !mixed_probe(&P, o)) o_run.precision = MPQP_FP64;
+    o = &o_run;
     Scratch s;
     scratch_init(&s, &P);
     FState Fa, Fb;
-    OState Oa, Ob, O2;
+    OPack Wa, Wb, W2;
+    const int mixed = o->precision == MPQP_MIXED;
     fstate_alloc(&Fa, &P); fstate_alloc(&Fb, &P);
-    ostate_alloc(&Oa, &P); ostate_alloc(&Ob, &P); ostate_alloc(&O2, &P);
+    opack_alloc(&Wa, &P, 3); opack_alloc(&Wb, &P, 3); opack_alloc(&W2, &P, 3);
     FState *fpar = NULL, *fcur = &Fa, *fspare = &Fb;
-    OState *opar = NULL, *ocur = &Oa, *ospare = &Ob;
+    OPack *opar = NULL, *ocur = &Wa, *ospare = &Wb;
     int opar_ok = 0, ret = 1;
     MpqpRecord r;
     for (int t = 0; t <= k; t++) {
         BasisOut bo = {0, NULL, NULL, NULL};
         if (t == k) { bo.fb = fb; bo.b1 = ob1; bo.b2 = ob2; }
-        eval_candidate(&P, o, S, t, fpar, opar, opar_ok, fcur, ocur, &O2, &s, &r, &bo);
+        eval_candidate(&P, o, S, t, fpar, opar ? opack_ptr(opar, mixed) : NULL, opar_ok, fcur, &ocur->d,
+                       &W2.d, &ocur->f, &W2.f, &s, &r, &bo);
         if (t < k && !r.feasible) { ret = 0; break; }
         fpar = fcur; { FState *tmp = fspare; fspare = fcur; fcur = tmp; }
-        opar = ocur; { OState *tmp = ospare; ospare = ocur; ocur = tmp; }
-        opar_ok = r.kkt_feasible;
+        opar = ocur; { OPack *tmp = ospare; ospare = ocur; ocur = tmp; }
+        opar_ok = r.p1_clean;
     }
     if (ret == 1 && rec) { *rec = r; rec->parent = -1; }
     fstate_free(&Fa); fstate_free(&Fb);
-    ostate_free(&Oa); ostate_free(&Ob); ostate_free(&O2);
+    opack_free(&Wa); opack_free(&Wb); opack_free(&W2);
     scratch_free(&s);
     prob_free(&P);
     return ret;
@@ -1137,7 +1283,7 @@ int oracle_root(int n_c, int n_x, int n_t, double *A1, double *xb1, int *b1, int
 }
 
 /* Tree helpers exposed for the reduction-order tests. */
-double oracle_tree_sum(double *v, int n) { return tree_sum(v, n); }
+double oracle_tree_sum(double *v, int n) { return tree_sum_d(v, n); }
 
 int oracle_tree_argmin_dmin(const double *v, const int *cnt, int n, int mode_b, double stop_tol) {
     double *vals = (double *)malloc(sizeof(double) * (size_t)n);

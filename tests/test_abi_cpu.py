@@ -49,8 +49,8 @@ def test_gpuresults_layout_matches_reference():
 
 def test_struct_sizes():
     assert ctypes.sizeof(MpqpRecord) == 80
-    assert ctypes.sizeof(MpqpOptions) == 48 + 12 * 8
-    assert ctypes.sizeof(MpqpStats) == 16 + 5 * 8 + 6 * 8
+    assert ctypes.sizeof(MpqpOptions) == 48 + 15 * 8
+    assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8
 
 
 def test_backend_and_options(hip):
