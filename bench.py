@@ -4,15 +4,22 @@
 A step is one mpqp() call over the whole synthetic workload: level-wise
 enumeration of every candidate active set with |S| <= k_max, a feasibility LP
 per candidate and an optimality LP (phase 1 + phase 2) per feasible candidate,
-children warm-started from their parent's final basis.  One check is one
+children warm-started from their parent's final states.  One check is one
 candidate that received an LP verdict.
 
-Prints one JSON line (see README of the driver contract): value is checks/s
-of the whole job with the problem data already on the host side of the C ABI
-(the ABI takes host arrays, as the reference's does), roofline is the dominant
-kernel's algorithmic LDS traffic rate against the measured LDS peak, and
-cpu_baseline is the CPU restatement of the same algorithm timed on this
-host's cores on a bounded sample.
+The headline variant is mixed precision (FP32 pivots, FP64 certification,
+FP64 re-solve of what fails it) with persistent launches.  The plain FP64
+variant (one workgroup per LP, FP64 pivots) runs in the same process on the
+same input; its verdicts (feasible, KKT-feasible, final) are compared with
+the headline's candidate by candidate and its throughput gives the speedup.
+
+Prints one JSON line.  value is checks/s of the whole job; the problem data
+crosses the C ABI as host arrays (as in the reference) and is uploaded once
+per step (a few hundred KB, inside the timed region).  roofline is the
+dominant kernel's algorithmic LDS traffic rate (the tableau lives in LDS;
+HBM is touched once per LP to load and store a state) against the LDS read
+bandwidth measured on the device; cpu_baseline is the CPU restatement of the
+same algorithm on a bounded sample, timed on this host's cores.
 """
 from __future__ import annotations
 
@@ -22,6 +29,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -30,29 +39,21 @@ from mpqpcudasolver_amd.problem import CONFIGS  # noqa: E402
 
 METRIC = "active-set checks/sec on 1 MI355X: mixed+batched vs FP64 plain, 0 verdict mismatch"
 LDS_NOMINAL = 256 * 128 * 2.4e9  # 256 CUs x 128 B/clk x 2.4 GHz
-
-
-def _sync():
-    try:
-        import torch  # noqa: F401
-
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:  # torch is optional plumbing here; mpqp() is synchronous
-        pass
+VERDICTS = ("feasible", "kkt_feasible", "final_set")
 
 
 def _workload(cfg: int) -> str:
     c = CONFIGS[cfg]
     if c["kind"] == "uneven":
-        return f"config{cfg}: {c['n_problems']} uneven mpQPs, |S|<={c['k_max']}, seed {c['seed']}"
-    return (f"config{cfg}: n={c['n']} m={c['m']} p={c['p']} ({c['kind']}), level-wise |S|<={c['k_max']}, "
-            f"parent-feasible children, basis sharing, seed {c['seed']}")
+        return f"{c['n_problems']} uneven mpQPs, |S|<={c['k_max']}, seed {c['seed']}"
+    return (f"mpQP n={c['n']} m={c['m']} p={c['p']} ({c['kind']}), level-wise |S|<={c['k_max']}, "
+            f"parent-feasible children, warm starts, seed {c['seed']}")
 
 
 def cpu_baseline(lp, k_sample: int) -> dict:
     """The CPU restatement timed on a bounded sample: every level up to
-    k_sample of the same problem (OpenMP over the candidates of a level)."""
+    k_sample of the same problem, FP64, OpenMP over the candidates of a
+    level."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleLib  # test infrastructure: the CPU baseline leg
 
@@ -65,7 +66,49 @@ def cpu_baseline(lp, k_sample: int) -> dict:
     dt = time.perf_counter() - t0
     n = len(run.records)
     return {"value": n / dt, "unit": "checks/s", "cores": threads, "kind": "port",
-            "sample": f"levels |S|<={k_sample} of the same problem: {n} checks in {dt:.2f} s"}
+            "sample": f"levels |S|<={k_sample} of the same problem, FP64: {n} checks in {dt:.2f} s"}
+
+
+def timed(lib, lp, k_max, steps, warmup, **opts):
+    lib.set_options(**opts)
+    for _ in range(warmup):
+        lib.run(lp, k_max=k_max)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.run(lp, k_max=k_max)
+    dt = time.perf_counter() - t0
+    st = lib.stats()
+    return st, dt
+
+
+def roofline(st: dict, lds_peak: float, fp32: bool) -> dict:
+    """Dominant kernel: the optimality kernel (FP32 pivots in a mixed run,
+    fallback launches excluded) or the feasibility kernel, whichever took
+    longer in the last step.  Times are HIP events on the library's stream."""
+    ms_opt = st["ms_opt"] - st["ms_fallback"]
+    if ms_opt >= st["ms_feas"]:
+        name = "opt_kernel_t<float> (FP32 pivots + FP64 certification)" if fp32 else "opt_kernel_t<double>"
+        launches = st["launches_opt"] - st["launches_fallback"]
+        ms, byts, flops = ms_opt, st["bytes_opt"] - st["bytes_fallback"], st["flops_opt"] - st["flops_fallback"]
+    else:
+        name = "feas_kernel"
+        launches, ms, byts, flops = st["launches_feas"], st["ms_feas"], st["bytes_feas"], st["flops_feas"]
+    launches = max(launches, 1)
+    avg_s = ms / launches * 1e-3
+    achieved = byts / launches / avg_s / 1e9 if avg_s > 0 else 0.0
+    return {
+        "bound": "lds",
+        "kernel": name,
+        "achieved": achieved,
+        "peak": lds_peak / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / (lds_peak / 1e9),
+        "traffic": None,
+        "avg_launch_ms": ms / launches,
+        "launches": launches,
+        "tflops": flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0,
+        "peak_note": "LDS read bandwidth of this device, measured by mpqp_measure_lds_bandwidth",
+    }
 
 
 def main() -> int:
@@ -74,8 +117,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--batching", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--batching", choices=["plain", "persistent"], default="persistent")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64-plain comparison run")
     ap.add_argument("--cpu-k", type=int, default=3, help="levels of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -95,7 +139,7 @@ def main() -> int:
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the level frontier would be sharded across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(the candidates of a level would be split across GPUs)", file=sys.stderr)
         return 2
 
     lib = load_library()
@@ -103,33 +147,17 @@ def main() -> int:
         raise RuntimeError(lib.last_error())
     prog, k_max = config_problem(args.config)
     if isinstance(prog, list):
-        raise SystemExit("config 2 (many problems) is benchmarked through bench_multi (not yet built)")
+        raise SystemExit("the many-problem workload is measured by its parity tests, not here")
     lp = assemble_lps(prog)
-    lib.set_options(precision=0 if args.precision == "fp64" else 1,
-                    batching=0 if args.batching == "plain" else 1)
+    mixed = args.precision == "mixed"
+    opts = dict(precision=1 if mixed else 0, batching=1 if args.batching == "persistent" else 0)
 
-    for _ in range(args.warmup):
-        lib.run(lp, k_max=k_max)
-    _sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run = lib.run(lp, k_max=k_max)
-    _sync()
-    dt = time.perf_counter() - t0
-    st = lib.stats()
+    st, dt = timed(lib, lp, k_max, args.steps, args.warmup, **opts)
     checks = st["candidates"]
-    ms_per_step = dt / args.steps * 1e3
     value = checks * args.steps / dt
-
     lds_peak = lib.measure_lds_bandwidth()
     if not lds_peak > 0:
         lds_peak = LDS_NOMINAL
-    dominant = "opt" if st["ms_opt"] >= st["ms_feas"] else "feas"
-    launches = max(st[f"launches_{dominant}"], 1)
-    avg_s = st[f"ms_{dominant}"] / launches * 1e-3
-    bytes_per_launch = st[f"bytes_{dominant}"] / launches
-    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    flops_rate = st[f"flops_{dominant}"] / launches / avg_s / 1e12 if avg_s > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": value,
@@ -137,28 +165,33 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-pivot+f64-certify",
+        "dtype": "f32-pivot+f64-certify" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": _workload(args.config), "variant": f"{args.precision}-{args.batching}",
-                   "checks_per_step": checks, "feasible_per_step": run.n_feasible_lps,
-                   "final_sets": run.n_final_lps},
-        "roofline": {
-            "bound": "lds",
-            "kernel": f"{dominant}_kernel",
-            "achieved": achieved,
-            "peak": lds_peak / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / (lds_peak / 1e9),
-            "traffic": None,
-            "fp64_tflops": flops_rate,
-            "peak_note": "LDS read bandwidth measured on this device by mpqp_measure_lds_bandwidth",
-        },
+                   "checks_per_step": checks, "precision_used": "mixed" if st["precision_used"] else "fp64",
+                   "fallbacks_fp64": st["fallbacks"], "optimality_lps": st["opt_lps"]},
+        "roofline": roofline(st, lds_peak, mixed and st["precision_used"] == 1),
         "stats": st,
     }
+    if not args.no_compare and (mixed or args.batching != "plain"):
+        # the FP64 plain variant: throughput and candidate-by-candidate verdicts
+        st64, dt64 = timed(lib, lp, k_max, args.steps, args.warmup, precision=0, batching=0)
+        v64 = st64["candidates"] * args.steps / dt64
+        lib.set_options(record=1, **opts)
+        a = lib.run(lp, k_max=k_max, records=True)
+        lib.set_options(record=1, precision=0, batching=0)
+        b = lib.run(lp, k_max=k_max, records=True)
+        lib.set_options()
+        same_sets = a.sets.shape == b.sets.shape and bool(np.array_equal(a.sets, b.sets))
+        mism = {f: int(np.count_nonzero(a.records[f] != b.records[f])) if same_sets else -1 for f in VERDICTS}
+        out["fp64_plain"] = {"value": v64, "ms_per_step": dt64 / args.steps * 1e3,
+                             "speedup": value / v64 if v64 > 0 else None,
+                             "verdict_mismatches": mism, "same_candidates": same_sets,
+                             "same_active_sets": a.active_sets == b.active_sets}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lp, min(args.cpu_k, k_max))
     print(json.dumps(out), flush=True)

@@ -90,8 +90,13 @@ typedef struct MpqpOptions {
     int max_iter_feas;      /* 500   (batched_simplex.cu:1162) */
     int max_iter_phase1;    /* 10000 (batched_simplex.cu:1474) */
     int max_iter_phase2;    /* 800   (batched_simplex.cu:1633) */
-    int stall_perturb;      /* 20    (batched_simplex.cu:1550) */
-    int stall_exit;         /* 300   (batched_simplex.cu:1561) */
+    int stall_perturb;      /* 20: perturbed rhs after 20 pivots without progress
+                               (batch-level in batched_simplex.cu:1550, per LP here) */
+    int stall_exit;         /* 10000: STALL after this many pivots without progress.
+                               batched_simplex.cu:1561 stops the whole batch after 300
+                               batch iterations in which no LP finished; per LP, 300
+                               drops LPs that converge later, so the default leaves
+                               termination to max_iter_phase1 */
     int opt_threads;        /* threads per optimality workgroup; 0 = auto */
     int warm_start;         /* 1: children start from the parent's final basis */
     int reserved0;
@@ -107,9 +112,9 @@ typedef struct MpqpOptions {
     double perturb_step;       /* rhs perturbation (i+1)*perturb_step (1e-6) */
     double certify_tol;        /* mixed precision: FP64 certification tolerance */
     double pivot_rel;          /* ratio tests also require |entry| > pivot_rel * max|entry| (0: off) */
-    double tol32;              /* mixed precision: pivot / stop tolerance of the FP32 pass (1e-6) */
+    double tol32;              /* mixed precision: pivot / stop tolerance of the FP32 pass (1e-5) */
     double licq32;             /* mixed precision: FP32 LICQ screen; smaller pivots go to FP64 (1e-6) */
-    double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass */
+    double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass (1e-3) */
 } MpqpOptions;
 
 /* One record per candidate active set of the last mpqp() call, in
@@ -161,6 +166,10 @@ typedef struct MpqpStats {
     double bytes_feas, bytes_opt;
     double flops_feas, flops_opt;
     long long fallbacks;      /* mixed precision: optimality LPs re-solved in FP64 */
+    /* the FP64 re-solve launches of a mixed run, included in the _opt totals */
+    int launches_fallback;
+    int reserved1;
+    double ms_fallback, bytes_fallback, flops_fallback;
 } MpqpStats;
 
 int mpqp_get_stats(MpqpStats *stats);

@@ -7,8 +7,10 @@
 //   feas_kernel  one 64-lane workgroup per candidate: the feasibility LP
 //                (replaces simplex_gpu_feasibility, batched_simplex.cu:
 //                990-1258, and its kernels a3/a4/a10-a15/a22/a23/a27).
-//   opt_kernel   one workgroup per feasible candidate: phase-1 dual simplex,
-//                then phase-2 primal simplex (replaces
+//   opt_kernel_t one workgroup per feasible candidate (or a persistent grid
+//                drawing candidates from a work counter): phase-1 dual
+//                simplex, drive-out, phase-2 primal simplex, in FP64 or in
+//                FP32 with FP64 certification (opt_kernel.hpp; replaces
 //                dual_simplex_gpu_optimality, batched_simplex.cu:1260-1721).
 #pragma once
 #include "lp_engine.hpp"
@@ -34,18 +36,24 @@ struct OptArgs {
     DevProb P;
     DevOpts O;
     int n, level, kw;
-    const int *list;      // n: candidate ids to solve
+    const int *list;      // candidate ids to solve: n of them, or *n_dev
+    const int *n_dev;     // device-side length of list, or null
+    int *counter;         // work counter of a persistent launch (starts at 0), or
+                          // null: workgroup b solves list[b]
     const int *sets;      // by candidate
     const int *p1parent;  // by candidate: parent row in ostore_prev, -1 = cold
     const unsigned char *ostore_prev;
     unsigned char *ostore_cur;
     long long ostride;
-    int store;
+    int store;            // keep clean phase-1 states for the next level
+    int store32;          // FP64 kernel: keep them rounded to FP32 (mixed runs)
+    int force_cold;       // ignore p1parent (FP64 re-solve of the mixed path)
     DevRecord *rec;       // by candidate
     int *b1, *b2;         // by candidate x M2, or null
-    const int *p1drop;    // by candidate: position in S of the constraint the warm start adds, or null
-    int *trace;           // diagnostics: phase-1 decisions of list[0], or null
-    long long *prof;      // diagnostics: per-block phase cycle counts (8 each), or null
+    int *fb_list;         // mixed: candidates whose FP32 result failed
+    int *fb_count;        //        certification (appended), and their count
+    double *cert;         // mixed: 6 x (M2 + 8) doubles of scratch per workgroup
+    long long *prof;      // phase cycle counts, 8 per workgroup (MPQP_PROFILE builds), or null
 };
 
 // Byte layout of one stored state: T (rows x cols doubles, contiguous) then
@@ -81,6 +89,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     int *nonb = basis + M;
     unsigned *inS = reinterpret_cast<unsigned *>(nonb + K);
     const int nw = (P.n_c + 31) / 32;
+    const DevTol tl = dev_tol64(O);
 
     for (int w = tid; w < nw; w += NT) inS[w] = 0u;
     __syncthreads();
@@ -143,18 +152,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
                 done = 1 + ST_OPTIMAL;
             } else {
                 // ratio test, lowest row on ties (pivot_kernel_1_2 :901-942)
-                const int q = aux;
-                double rv = 0.0;
-                int rk = -1, dummy = 0;
-                for (int i = lane; i < M; i += MPQP_WAVE) {
-                    const double al = T[i * ld + q];
-                    if (al > O.tol_plus) {
-                        const double rt = T[i * ld + K] / al;
-                        if (rk < 0 || rt < rv) { rv = rt; rk = i; }
-                    }
-                }
-                lex_min(rv, rk, dummy);
-                r = rk;
+                r = primal_ratio_wave<double>(T, ld, M, aux, K, tl);
                 if (r < 0) done = 1 + ST_UNBOUNDED;
             }
             if (lane == 0) {
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
         R.p2_status = R.p2_iters = R.final_set = 0;
         R.z1 = R.z2 = 0.0;
         R.repivots = 0;
-        R.reserved = 0;
+        R.p1_clean = 0;
     }
     if (a.fbasis)
         for (int i = tid; i < M; i += NT) a.fbasis[(long long)c * M + i] = basis[i];
@@ -211,13 +209,6 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
 // ---------------------------------------------------------------------
 // Optimality LP
 // ---------------------------------------------------------------------
-__host__ __device__ inline long long opt_lds_bytes(int M, int K, int N2) {
-    const int ld = K + 2;
-    long long b = (long long)(M + 1) * ld * 8 + (long long)(M + 1) * 8 + (long long)M * 8 +
-                  (long long)(M + K) * 4 + (long long)N2 * 4 * 3 + (long long)(M + 1) * 4;
-    return (b + 15) & ~15LL;
-}
-
 // slots per lane a reduction over n elements needs on one wave
 __host__ __device__ inline int red_slots(int n) {
     const int s = (n + 63) / 64;
@@ -267,402 +258,7 @@ struct P2Pred {  // min_idx_kernel_optimal, batched_simplex.cu:343-344
     }
 };
 
-#define PROF_MARK(slot)                                                        \
-    do {                                                                       \
-        if (a.prof && tid == 0 && blockIdx.x < 4096) {                         \
-            const long long now_ = clock64();                                  \
-            a.prof[blockIdx.x * 8 + (slot)] += now_ - prof_t0;                 \
-            prof_t0 = now_;                                                    \
-        }                                                                      \
-    } while (0)
-
-// Optimality LP of one candidate per workgroup.  LDS: T (M+1) x (K+2)
-// [slots | rhs | B^-1 delta], colq (M+1), vcol (M, entering column of a
-// lambda swap), basis, nonb, where (var -> position >= 0 or -(slot+1)),
-// bcnt / nbcnt (use counters), rows (compact list of cost-carrying rows).
-template <int NT, int S>
-__global__ __launch_bounds__(NT) void opt_kernel(OptArgs a) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    __shared__ int sh_r, sh_q, sh_done, sh_feas, sh_stall, sh_nrows;
-    __shared__ double sh_z;
-    const DevProb &P = a.P;
-    const DevOpts &O = a.O;
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = K + 2;
-    const int RHS = K, PERT = K + 1;
-    const int EPS = P.n_x + P.n_t + P.n_c;
-    const int r4 = M - P.n_c - P.n_x;
-    const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
-    const int c = a.list[blockIdx.x];
-    const int k = a.level;
-    double *T = reinterpret_cast<double *>(smem);
-    double *colq = T + (M + 1) * ld;
-    double *vcol = colq + (M + 1);
-    int *basis = reinterpret_cast<int *>(vcol + M);
-    int *nonb = basis + M;
-    int *where = nonb + K;
-    int *bcnt = where + N2;
-    int *nbcnt = bcnt + N2;
-    int *rows = nbcnt + N2;
-    const int *Sset = a.sets + (long long)c * a.kw;
-    DevRecord &R = a.rec[c];
-    long long prof_t0 = clock64();
-
-    if (*P.root_ok == 0) {
-        if (tid == 0) R.p1_status = ST_SINGULAR;
-        return;
-    }
-    int warm = (O.warm && k > 0 && a.p1parent[c] >= 0) ? 1 : 0;
-    int started = 0;
-    while (!started) {
-        // ---- load the start state (flat copy: store layout == LDS layout)
-        const double *sT = warm ? reinterpret_cast<const double *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride)
-                                : P.root;
-        const int *sb = warm ? reinterpret_cast<const int *>(sT + (long long)M * ld) : P.root_basis;
-        const int *sn = warm ? sb + M : P.root_nonb;
-        const int ne = M * ld;
-        if ((ne & 1) == 0) {
-            const double2 *s2 = reinterpret_cast<const double2 *>(sT);
-            double2 *d2 = reinterpret_cast<double2 *>(T);
-            for (int e = tid; e < ne / 2; e += NT) d2[e] = s2[e];
-        } else {
-            for (int e = tid; e < ne; e += NT) T[e] = sT[e];
-        }
-        for (int i = tid; i < M; i += NT) basis[i] = sb[i];
-        for (int kk = tid; kk < K; kk += NT) nonb[kk] = sn[kk];
-        for (int v = tid; v < N2; v += NT) where[v] = INT32_MIN;
-        __syncthreads();
-        for (int i = tid; i < M; i += NT) where[basis[i]] = i;
-        for (int kk = tid; kk < K; kk += NT) where[nonb[kk]] = -(kk + 1);
-        __syncthreads();
-        PROF_MARK(0);
-
-        // ---- lambda swaps: warm = the new constraint only, cold = all of S
-        int ok = 1;
-        const int j0 = warm ? (a.p1drop ? a.p1drop[c] : k - 1) : 0;
-        const int j1 = warm ? j0 + 1 : k;
-        for (int jj = j0; jj < j1; ++jj) {
-            const int j = Sset[jj];
-            // nonzeros of lambda_j: row j (R1) then the R4 rows, ascending
-            if (tid <= P.n_x) {
-                const int rho = tid == 0 ? j : r4 + tid - 1;
-                colq[tid] = lam_entry(P, j, rho);
-            }
-            __syncthreads();
-            for (int i = tid; i < M; i += NT) {
-                double acc = 0.0;
-                for (int t = 0; t <= P.n_x; ++t) {
-                    const double av = colq[t];
-                    if (av == 0.0) continue;
-                    const int rho = t == 0 ? j : r4 + t - 1;
-                    const int w = where[N2 - M + rho];
-                    if (w >= 0) {
-                        if (i == w) acc = fma(av, 1.0, acc);
-                    } else {
-                        acc = fma(av, T[i * ld + (-w - 1)], acc);
-                    }
-                }
-                vcol[i] = acc;
-            }
-            __syncthreads();
-            const int p = where[P.n_x + P.n_t + j];
-            if (p >= 0) {
-                const double pv = vcol[p];
-                if (!(fabs(pv) > O.licq_tol)) {
-                    ok = 0;
-                    break;
-                }
-                pivot_ext<NT, double>(T, ld, M, K + 2, p, vcol);
-            } else {
-                const int kk = -p - 1;
-                for (int i = tid; i < M; i += NT) T[i * ld + kk] = vcol[i];
-                __syncthreads();
-            }
-        }
-        if (ok) {
-            started = 1;
-        } else if (warm) {
-            warm = 0;
-            __syncthreads();
-        } else {
-            break;
-        }
-    }
-    if (!started) {
-        if (tid == 0) R.p1_status = ST_SINGULAR;
-        return;
-    }
-    PROF_MARK(1);
-
-    // ---------------- phase 1: dual simplex ----------------
-    // rows whose basic variable carries cost (artificials), ascending
-    if (wave == 0) {
-        int base = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int i = lane + MPQP_WAVE * s;
-            const bool f = i < M && basis[i] >= Nn;
-            const unsigned long long m = __ballot(f);
-            if (f) rows[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
-            base += __popcll(m);
-        }
-        if (lane == 0) sh_nrows = base;
-    }
-    __syncthreads();
-    cost_row_list<NT, double>(
-        T, ld, M, K + 2, rows, sh_nrows, [&](int) { return 1.0; },
-        [&](int j) { return j < K ? (nonb[j] >= Nn ? 1.0 : 0.0) : 0.0; });
-    for (int v = tid; v < N2; v += NT) { bcnt[v] = 0; nbcnt[v] = 0; }
-    __syncthreads();
-    PROF_MARK(2);
-
-    int n_same = 0;
-    double best_p = -INFINITY;
-    int status1 = ST_ITER_CAP, iters1 = 0, kkt = 0;
-    double z1 = 0.0;
-    for (int it = 0; it < O.max_iter_p1; ++it) {
-        if (wave == 0) {
-            const bool pert = n_same >= O.stall_perturb;
-            double xv[S], v[S];
-            int id[S], cn[S];
-            bool neg = false, art = false;
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int i = lane + MPQP_WAVE * s;
-                const bool valid = i < M;
-                double x = valid ? T[i * ld + RHS] : 0.0;
-                if (pert && valid) x = x + T[i * ld + PERT];
-                xv[s] = x;
-                v[s] = x;
-                id[s] = i;
-                const int bv = valid ? basis[i] : 0;
-                cn[s] = valid ? bcnt[bv] : 0;
-                neg |= valid && x < O.dual_stop_tol;
-                art |= valid && bv >= Nn;
-            }
-            tree_argmin_s<S>(v, id, cn, M, lane, DminPred{(n_same % 20) >= 10 ? 1 : 0, O.dual_stop_tol});
-            const int r = rfl(id[0]);
-            const bool go_on = __any(neg);
-            const bool any_art = __any(art);
-            int done = 0, fe = 0, q = -1, stall = 0;
-            double zz = 0.0;
-            if (lane == 0) bcnt[basis[r]] += 1;
-            if (!go_on) {
-                double zs[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int i = lane + MPQP_WAVE * s;
-                    zs[s] = (i < M) ? (basis[i] >= Nn ? 1.0 : 0.0) * xv[s] : 0.0;
-                }
-                zz = tree_sum_s<S>(zs, M, lane);
-                fe = (zz > -O.dual_feas_tol && zz < O.dual_feas_tol && !any_art) ? 1 : 0;
-                done = 1 + ST_OPTIMAL;
-            } else {
-                const int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
-                const double *tr = T + r * ld;
-                const double *tc = T + M * ld;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int kk = lane + MPQP_WAVE * s;
-                    id[s] = -1;
-                    v[s] = 100000.0;
-                    cn[s] = 0;
-                    if (kk < K) {
-                        const double al = tr[kk];
-                        if (al < O.tol_minus) {
-                            const int var = nonb[kk];
-                            id[s] = var;
-                            v[s] = -tc[kk] / al;
-                            cn[s] = nbcnt[var];
-                        }
-                    }
-                }
-                tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
-                const int wv = rfl(id[0]);
-                if (wv < 0) {
-                    done = 1 + ST_UNBOUNDED;
-                } else {
-                    q = -where[wv] - 1;
-                    double ps[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const int i = lane + MPQP_WAVE * s;
-                        ps[s] = (i < M && xv[s] < 0.0) ? xv[s] : 0.0;
-                    }
-                    const double pinf = tree_sum_s<S>(ps, M, lane);
-                    if (pinf > best_p) {
-                        best_p = pinf;
-                        n_same = 0;
-                    } else {
-                        ++n_same;
-                    }
-                    stall = n_same > O.stall_exit ? 1 : 0;
-                    if (lane == 0) nbcnt[wv] += 1;
-                }
-            }
-            if (lane == 0) {
-                sh_r = r;
-                sh_q = q;
-                sh_done = done;
-                sh_z = zz;
-                sh_feas = fe;
-                sh_stall = stall;
-                if (a.trace && blockIdx.x == 0 && it < 256) {
-                    a.trace[it * 4 + 0] = r;
-                    a.trace[it * 4 + 1] = q;
-                    a.trace[it * 4 + 2] = done;
-                    a.trace[it * 4 + 3] = n_same;
-                }
-            }
-        }
-        __syncthreads();
-        PROF_MARK(3);
-        if (sh_done) {
-            status1 = sh_done - 1;
-            z1 = sh_z;
-            kkt = sh_feas;
-            break;
-        }
-        const int r = sh_r, q = sh_q, stall = sh_stall;
-        pivot_rows<NT, double>(T, ld, M + 1, K + 2, r, q, colq);
-        if (tid == 0) {
-            const int t = basis[r];
-            basis[r] = nonb[q];
-            nonb[q] = t;
-            where[basis[r]] = r;
-            where[t] = -(q + 1);
-        }
-        __syncthreads();
-        PROF_MARK(4);
-        ++iters1;
-        if (stall) {
-            status1 = ST_STALL;
-            break;
-        }
-    }
-
-    if (tid == 0) {
-        R.p1_status = status1;
-        R.p1_iters = iters1;
-        R.kkt_feasible = kkt;
-        R.p1_warm = warm;
-        R.z1 = z1;
-    }
-    if (a.b1)
-        for (int i = tid; i < M; i += NT) a.b1[(long long)c * M + i] = basis[i];
-    if (!kkt) return;
-    PROF_MARK(7);
-    if (a.store) {
-        double *dT = reinterpret_cast<double *>(a.ostore_cur + (long long)c * a.ostride);
-        const int ne = M * ld;
-        if ((ne & 1) == 0) {
-            double2 *d2 = reinterpret_cast<double2 *>(dT);
-            const double2 *s2 = reinterpret_cast<const double2 *>(T);
-            for (int e = tid; e < ne / 2; e += NT) d2[e] = s2[e];
-        } else {
-            for (int e = tid; e < ne; e += NT) dT[e] = T[e];
-        }
-        int *db = reinterpret_cast<int *>(dT + (long long)M * ld);
-        for (int i = tid; i < M; i += NT) db[i] = basis[i];
-        for (int kk = tid; kk < K; kk += NT) db[M + kk] = nonb[kk];
-    }
-
-    // ---------------- phase 2: primal simplex, min -eps ----------------
-    __syncthreads();
-    PROF_MARK(5);
-    if (tid == 0) {
-        const int pe = where[EPS];
-        sh_nrows = pe >= 0 ? 1 : 0;
-        rows[0] = pe;
-    }
-    __syncthreads();
-    cost_row_list<NT, double>(
-        T, ld, M, K + 1, rows, sh_nrows, [&](int) { return -1.0; },
-        [&](int j) { return j < K ? (nonb[j] == EPS ? -1.0 : 0.0) : 0.0; });
-    for (int v = tid; v < N2; v += NT) bcnt[v] = 0;
-    __syncthreads();
-    int status2 = ST_ITER_CAP, iters2 = 0;
-    double z2 = 0.0;
-    for (int it = 0; it < O.max_iter_p2; ++it) {
-        if (wave == 0) {
-            double v[S];
-            int id[S], cn[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int var = lane + MPQP_WAVE * s;
-                v[s] = 0.0;
-                id[s] = var;
-                cn[s] = 0;
-                if (var < Nn) {
-                    const int w = where[var];
-                    v[s] = w >= 0 ? 100000.0 : T[M * ld + (-w - 1)];
-                    cn[s] = bcnt[var];
-                }
-            }
-            tree_argmin_s<S>(v, id, cn, Nn, lane, P2Pred{});
-            const int wv = rfl(id[0]);
-            const double dmin = rfl_d(v[0]);
-            if (lane == 0) bcnt[wv] += 1;
-            int done = 0, r = -1, q = -1;
-            double zz = 0.0;
-            if (dmin > O.tol_minus_opt) {
-                double zs[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int i = lane + MPQP_WAVE * s;
-                    zs[s] = (i < M) ? (basis[i] == EPS ? -1.0 : 0.0) * T[i * ld + RHS] : 0.0;
-                }
-                zz = tree_sum_s<S>(zs, M, lane);
-                done = 1 + ST_OPTIMAL;
-            } else {
-                q = -where[wv] - 1;
-                double rv = 0.0;
-                int rk = -1, dummy = 0;
-                for (int i = lane; i < M; i += MPQP_WAVE) {
-                    const double al = T[i * ld + q];
-                    if (al > O.tol_plus) {
-                        const double rt = T[i * ld + RHS] / al;
-                        if (rk < 0 || rt < rv) { rv = rt; rk = i; }
-                    }
-                }
-                lex_min(rv, rk, dummy);
-                r = rk;
-                if (r < 0) done = 1 + ST_UNBOUNDED;
-            }
-            if (lane == 0) {
-                sh_r = r;
-                sh_q = q;
-                sh_done = done;
-                sh_z = zz;
-            }
-        }
-        __syncthreads();
-        if (sh_done) {
-            status2 = sh_done - 1;
-            z2 = sh_z;
-            break;
-        }
-        const int r = sh_r, q = sh_q;
-        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
-        if (tid == 0) {
-            const int t = basis[r];
-            basis[r] = nonb[q];
-            nonb[q] = t;
-            where[basis[r]] = r;
-            where[t] = -(q + 1);
-        }
-        __syncthreads();
-        ++iters2;
-    }
-    PROF_MARK(6);
-    if (tid == 0) {
-        R.p2_status = status2;
-        R.p2_iters = iters2;
-        R.z2 = z2;
-        R.final_set = (status2 == ST_OPTIMAL && -z2 > O.final_tol) ? 1 : 0;
-    }
-    if (a.b2)
-        for (int i = tid; i < M; i += NT) a.b2[(long long)c * M + i] = basis[i];
-}
+#include "opt_kernel.hpp"
 
 // ---------------------------------------------------------------------
 // Root: S = {} KKT basis tableau (one workgroup, tableau in global memory)

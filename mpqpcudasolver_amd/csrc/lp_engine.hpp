@@ -27,7 +27,24 @@ struct DevOpts {
     int max_iter_feas, max_iter_p1, max_iter_p2, stall_perturb, stall_exit, warm;
     double tol_plus, tol_minus, tol_minus_1, tol_minus_opt;
     double dual_feas_tol, dual_stop_tol, feas_z_tol, licq_tol, final_tol, perturb_step;
+    double pivot_rel, certify_tol, tol32, licq32, pivot_rel32;
 };
+
+// Tolerances of one simplex pass: the FP64 path, or the FP32 pivots of the
+// mixed-precision path (pivot / stop tolerances tol32, LICQ screen licq32,
+// relative pivot threshold pivot_rel32).
+struct DevTol {
+    double tol_plus, tol_minus, tol_minus_opt, dual_stop_tol, dual_feas_tol, licq, pivot_rel, final_tol;
+};
+
+__host__ __device__ inline DevTol dev_tol64(const DevOpts &o) {
+    return DevTol{o.tol_plus, o.tol_minus, o.tol_minus_opt, o.dual_stop_tol, o.dual_feas_tol,
+                  o.licq_tol, o.pivot_rel, o.final_tol};
+}
+
+__host__ __device__ inline DevTol dev_tol32(const DevOpts &o) {
+    return DevTol{o.tol32, -o.tol32, -o.tol32, -o.tol32, o.tol32, o.licq32, o.pivot_rel32, o.final_tol};
+}
 
 struct DevProb {
     int n_c, n_x, n_t, n_cp;
@@ -47,7 +64,7 @@ struct DevRecord {
     int p1_status, p1_iters, kkt_feasible, p1_warm;
     int p2_status, p2_iters, final_set;
     double z_feas, z1, z2;
-    int repivots, reserved;
+    int repivots, p1_clean;
 };
 
 enum { ST_NOT_RUN = 0, ST_OPTIMAL = 1, ST_UNBOUNDED = 2, ST_ITER_CAP = 3, ST_STALL = 4, ST_SINGULAR = 5 };
@@ -92,6 +109,24 @@ __device__ __forceinline__ double lane_down_d(double x) {
     const int lo = lane_down_i<I>((int)(unsigned)(b & 0xffffffffLL));
     const int hi = lane_down_i<I>((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int I>
+__device__ __forceinline__ float lane_down_f(float x) {
+    return __int_as_float(lane_down_i<I>(__float_as_int(x)));
+}
+template <int I>
+__device__ __forceinline__ double lane_down_v(double x) { return lane_down_d<I>(x); }
+template <int I>
+__device__ __forceinline__ float lane_down_v(float x) { return lane_down_f<I>(x); }
+
+// Wave maximum (order-free), broadcast to all lanes.
+__device__ __forceinline__ double wave_max(double v) {
+    for (int d = 32; d > 0; d >>= 1) {
+        const double o = __shfl_xor(v, d, MPQP_WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
 }
 
 // ---------------------------------------------------------------------
@@ -178,14 +213,14 @@ __device__ __forceinline__ void tree_argmin_s(double (&v)[S], int (&id)[S], int 
     if (half >= 1) tree_step<S, 1>(v, id, cn, n, lane, pred);
 }
 
-template <int S, int I>
-__device__ __forceinline__ void sum_step(double (&v)[S], int n, int lane) {
-    const double ov = lane_down_d<I>(v[0]);
+template <int S, int I, typename V>
+__device__ __forceinline__ void sum_step(V (&v)[S], int n, int lane) {
+    const V ov = lane_down_v<I>(v[0]);
     v[0] = ((lane < I) & (lane + I < n)) ? v[0] + ov : v[0];
 }
 
-template <int S>
-__device__ __forceinline__ double tree_sum_s(double (&v)[S], int n, int lane) {
+template <int S, typename V>
+__device__ __forceinline__ V tree_sum_s(V (&v)[S], int n, int lane) {
     const int half = half_pow2(n);
 #pragma unroll
     for (int lvl = 2; lvl >= 0; --lvl) {
@@ -201,12 +236,12 @@ __device__ __forceinline__ double tree_sum_s(double (&v)[S], int n, int lane) {
             }
         }
     }
-    if (half >= 32) sum_step<S, 32>(v, n, lane);
-    if (half >= 16) sum_step<S, 16>(v, n, lane);
-    if (half >= 8) sum_step<S, 8>(v, n, lane);
-    if (half >= 4) sum_step<S, 4>(v, n, lane);
-    if (half >= 2) sum_step<S, 2>(v, n, lane);
-    if (half >= 1) sum_step<S, 1>(v, n, lane);
+    if (half >= 32) sum_step<S, 32, V>(v, n, lane);
+    if (half >= 16) sum_step<S, 16, V>(v, n, lane);
+    if (half >= 8) sum_step<S, 8, V>(v, n, lane);
+    if (half >= 4) sum_step<S, 4, V>(v, n, lane);
+    if (half >= 2) sum_step<S, 2, V>(v, n, lane);
+    if (half >= 1) sum_step<S, 1, V>(v, n, lane);
     return __shfl(v[0], 0, MPQP_WAVE);
 }
 
@@ -245,6 +280,36 @@ __device__ __forceinline__ void lex_min(double &v, int &key, int &aux) {
         key = take ? ok : key;
         aux = take ? oa : aux;
     }
+}
+
+// primal ratio test on column q of the tableau (wave 0 only): rows with
+// T[i][q] > max(tol_plus, pivot_rel * max_i |T[i][q]|), minimum
+// T[i][RHS] / T[i][q], lowest row on ties.  Result broadcast to the wave.
+template <typename Real>
+__device__ __forceinline__ int primal_ratio_wave(const Real *T, int ld, int M, int q, int rhs, const DevTol &tl) {
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    Real lim = (Real)tl.tol_plus;
+    if (tl.pivot_rel > 0.0) {
+        double cm = 0.0;
+        for (int i = lane; i < M; i += MPQP_WAVE) {
+            const double a = fabs((double)T[i * ld + q]);
+            cm = a > cm ? a : cm;
+        }
+        const Real cmax = (Real)wave_max(cm);
+        const Real rl = (Real)tl.pivot_rel * cmax;
+        lim = rl > lim ? rl : lim;
+    }
+    double rv = 0.0;
+    int rk = -1, dummy = 0;
+    for (int i = lane; i < M; i += MPQP_WAVE) {
+        const Real al = T[i * ld + q];
+        if (al > lim) {
+            const Real rt = T[i * ld + rhs] / al;
+            if (rk < 0 || (double)rt < rv) { rv = (double)rt; rk = i; }
+        }
+    }
+    lex_min(rv, rk, dummy);
+    return rk;
 }
 
 // ---------------------------------------------------------------------

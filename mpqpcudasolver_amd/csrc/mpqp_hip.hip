@@ -33,21 +33,28 @@ namespace {
 constexpr int FEAS_NT = 64;
 constexpr int OPT_NT = 512;
 
-template <int NT>
-void launch_opt_nt(int S, int grid, size_t lds, hipStream_t st, const OptArgs &oa) {
+template <typename Real, int NT, bool MIXED>
+const void *opt_fn_nt(int S) {
     switch (S) {
-    case 1: hipLaunchKernelGGL((opt_kernel<NT, 1>), dim3(grid), dim3(NT), lds, st, oa); break;
-    case 2: hipLaunchKernelGGL((opt_kernel<NT, 2>), dim3(grid), dim3(NT), lds, st, oa); break;
-    case 3: hipLaunchKernelGGL((opt_kernel<NT, 3>), dim3(grid), dim3(NT), lds, st, oa); break;
-    case 4: hipLaunchKernelGGL((opt_kernel<NT, 4>), dim3(grid), dim3(NT), lds, st, oa); break;
-    case 6: hipLaunchKernelGGL((opt_kernel<NT, 6>), dim3(grid), dim3(NT), lds, st, oa); break;
-    default: hipLaunchKernelGGL((opt_kernel<NT, 8>), dim3(grid), dim3(NT), lds, st, oa); break;
+    case 1: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 1, MIXED>);
+    case 2: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 2, MIXED>);
+    case 3: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 3, MIXED>);
+    case 4: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 4, MIXED>);
+    case 6: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 6, MIXED>);
+    default: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 8, MIXED>);
     }
 }
 
-void launch_opt(int nt, int S, int grid, size_t lds, hipStream_t st, const OptArgs &oa) {
-    if (nt == 256) launch_opt_nt<256>(S, grid, lds, st, oa);
-    else launch_opt_nt<512>(S, grid, lds, st, oa);
+// The optimality kernel for a precision (FP32 pivots with certification, or
+// FP64), workgroup size and reduction slot count.
+const void *opt_fn(bool fp32, int nt, int S) {
+    if (fp32) return nt == 256 ? opt_fn_nt<float, 256, true>(S) : opt_fn_nt<float, 512, true>(S);
+    return nt == 256 ? opt_fn_nt<double, 256, false>(S) : opt_fn_nt<double, 512, false>(S);
+}
+
+__global__ void fill_i32(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 struct Ctx {
@@ -92,7 +99,7 @@ void default_opts(MpqpOptions *o) {
     o->max_iter_phase1 = 10000;
     o->max_iter_phase2 = 800;
     o->stall_perturb = 20;
-    o->stall_exit = 300;
+    o->stall_exit = 10000;
     o->warm_start = 1;
     o->tol_plus = 1e-9;
     o->tol_minus = -1e-9;
@@ -168,6 +175,15 @@ struct DBuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// HIP event with RAII (released on every exit path, errors included).
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { HIP_OK(hipEventCreate(&e)); }
+    ~Event() { (void)hipEventDestroy(e); }
+    Event(const Event &) = delete;
+    Event &operator=(const Event &) = delete;
+};
+
 DevOpts dev_opts(const MpqpOptions &o) {
     DevOpts d;
     d.max_iter_feas = o.max_iter_feas;
@@ -186,6 +202,11 @@ DevOpts dev_opts(const MpqpOptions &o) {
     d.licq_tol = o.licq_tol;
     d.final_tol = o.final_tol;
     d.perturb_step = o.perturb_step;
+    d.pivot_rel = o.pivot_rel;
+    d.certify_tol = o.certify_tol;
+    d.tol32 = o.tol32;
+    d.licq32 = o.licq32;
+    d.pivot_rel32 = o.pivot_rel32;
     return d;
 }
 
@@ -244,9 +265,8 @@ __global__ __launch_bounds__(256) void lds_bw_kernel(float4 *out, int iters) {
 double measure_lds(hipStream_t st, int n_cu) {
     const int blocks = n_cu * 8, iters = 2048;
     DBuf out(sizeof(float4) * (size_t)blocks * 256);
-    hipEvent_t a, b;
-    HIP_OK(hipEventCreate(&a));
-    HIP_OK(hipEventCreate(&b));
+    Event ea, eb;
+    hipEvent_t a = ea.e, b = eb.e;
     hipLaunchKernelGGL(lds_bw_kernel, dim3(blocks), dim3(256), 0, st, out.as<float4>(), 64);
     HIP_OK(hipEventRecord(a, st));
     hipLaunchKernelGGL(lds_bw_kernel, dim3(blocks), dim3(256), 0, st, out.as<float4>(), iters);
@@ -255,8 +275,6 @@ double measure_lds(hipStream_t st, int n_cu) {
     HIP_OK(hipEventSynchronize(b));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     const double bytes = (double)blocks * 256 * iters * 16 * 16.0;
     return bytes / (ms * 1e-3);
 }
@@ -278,7 +296,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     if (feas_lds_bytes(M1, N1 - M1, n_c) > C.lds_limit)
         throw HipError{"mpqp: feasibility tableau exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
-    if (opt_lds_bytes(M2, N2 - M2, N2) > C.lds_limit)
+    if (opt_lds_bytes_t<double>(M2, N2 - M2, N2) > C.lds_limit)
         throw HipError{"mpqp: optimality tableau exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
     hipStream_t st = C.stream;
@@ -340,236 +358,310 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     P.root_ok = ra.root_ok;
 
     const long long fstride = state_bytes(M1, K1 + 1, M1, K1);
-    const long long ostride = state_bytes(M2, K2 + 2, M2, K2);
     const long long flds = feas_lds_bytes(M1, K1, n_c);
-    const long long olds = opt_lds_bytes(M2, K2, N2);
+    const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2);
+    const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2);
     const int opt_nt = (o.opt_threads == 256) ? 256 : OPT_NT;
+    const int S = red_slots(std::max(M2, N2 - n_eqs));
+    const void *fn64 = opt_fn(false, opt_nt, S);
+    const void *fn32 = opt_fn(true, opt_nt, S);
+    auto capacity = [&](const void *fn, long long lds) {
+        int nb = 0;
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, opt_nt, (size_t)lds));
+        return std::max(nb, 1) * std::max(C.n_cu, 1);
+    };
+    const int cap64 = capacity(fn64, lds64);
+    const int cap32 = o.precision == MPQP_MIXED ? capacity(fn32, lds32) : 0;
+    // counters: [0] FP32 work, [1] fallback count, [2] FP64 work
+    DBuf dctr(sizeof(int) * 4);
+    DBuf dcert;
+    if (o.precision == MPQP_MIXED) dcert.alloc(sizeof(double) * 6 * (size_t)(M2 + 8) * cap32);
+    auto launch = [&](const void *fn, int grid, long long lds, OptArgs &oa) {
+#ifdef MPQP_PROFILE
+        // diagnostic builds: per-phase cycle totals of this launch on stderr
+        DBuf dprof(sizeof(long long) * 8 * (size_t)grid);
+        HIP_OK(hipMemsetAsync(dprof.p, 0, dprof.n, st));
+        oa.prof = dprof.as<long long>();
+#endif
+        void *args[] = {&oa};
+        HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(opt_nt), args, (size_t)lds, st));
+#ifdef MPQP_PROFILE
+        std::vector<long long> hp(8 * (size_t)grid);
+        HIP_OK(hipMemcpyAsync(hp.data(), dprof.p, dprof.n, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        long long tot[8] = {0};
+        for (size_t b = 0; b < (size_t)grid; ++b)
+            for (int k = 0; k < 8; ++k) tot[k] += hp[b * 8 + k];
+        std::fprintf(stderr, "prof grid=%d fp32=%d:", grid, fn == fn32 ? 1 : 0);
+        for (int k = 0; k < 8; ++k) std::fprintf(stderr, " %lld", tot[k]);
+        std::fprintf(stderr, "\n");
+        oa.prof = nullptr;
+#endif
+    };
 
-    hipEvent_t e0, e1, e2;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    HIP_OK(hipEventCreate(&e2));
+    Event ev0, ev1, ev2, ev3;
+    hipEvent_t e0 = ev0.e, e1 = ev1.e, e2 = ev2.e, e3 = ev3.e;
 
     std::vector<MpqpRecord> all;
     std::vector<int> allsets, allfb, allb1, allb2;
     const bool keep_rec = o.record >= 1, keep_bases = o.record >= 2;
-
-    // previous level
-    std::vector<int> prev_sets, prev_feas, prev_kkt;
-    std::vector<int> prev_recidx;
-    DBuf fstore_prev, ostore_prev;
-    int prev_n = 0;
     int n_batches = 0, n_feas_total = 0;
     double tf = 0.0, to = 0.0;
-    std::vector<int> cur_sets, cur_par;
-    std::vector<DevRecord> hrec;
 
-    for (int lvl = 0; lvl <= k_max; ++lvl) {
-        cur_sets.clear();
-        cur_par.clear();
-        if (lvl == 0) {
-            cur_sets.assign(kw, -1);
-            cur_par.push_back(-1);
-        } else {
-            for (int pi = 0; pi < prev_n; ++pi) {
-                if (!prev_feas[pi]) continue;
-                const int mx = lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1;
-                for (int j = mx + 1; j < n_c; ++j) {
-                    for (int t = 0; t < kw; ++t)
-                        cur_sets.push_back(t < lvl - 1 ? prev_sets[(size_t)pi * kw + t]
-                                                       : (t == lvl - 1 ? j : -1));
-                    cur_par.push_back(pi);
+    // Mixed precision runs levels 0 and 1 with FP32 pivots first; if more
+    // than a quarter of their optimality LPs fail certification, the
+    // problem is too ill-conditioned for FP32 pivots and the whole run
+    // restarts in FP64.  (With k_max = 0, level 1 is solved only for this
+    // decision.)
+    bool mixed = o.precision == MPQP_MIXED;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        all.clear();
+        allsets.clear();
+        allfb.clear();
+        allb1.clear();
+        allb2.clear();
+        n_batches = n_feas_total = 0;
+        tf = to = 0.0;
+        C.stats = MpqpStats{};
+        bool restart = false;
+        long long probe_opt = 0, probe_fb = 0;
+        const int last_lvl = (mixed && k_max == 0) ? 1 : k_max;
+        const long long ostride = mixed ? opt_state_bytes<float>(M2, K2) : opt_state_bytes<double>(M2, K2);
+
+        std::vector<int> prev_sets, prev_feas, prev_clean;
+        std::vector<int> prev_recidx;
+        DBuf fstore_prev, ostore_prev;
+        int prev_n = 0;
+        std::vector<int> cur_sets, cur_par;
+        std::vector<DevRecord> hrec;
+
+        for (int lvl = 0; lvl <= last_lvl; ++lvl) {
+            cur_sets.clear();
+            cur_par.clear();
+            if (lvl == 0) {
+                cur_sets.assign(kw, -1);
+                cur_par.push_back(-1);
+            } else {
+                for (int pi = 0; pi < prev_n; ++pi) {
+                    if (!prev_feas[pi]) continue;
+                    const int mx = lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1;
+                    for (int j = mx + 1; j < n_c; ++j) {
+                        for (int t = 0; t < kw; ++t)
+                            cur_sets.push_back(t < lvl - 1 ? prev_sets[(size_t)pi * kw + t]
+                                                           : (t == lvl - 1 ? j : -1));
+                        cur_par.push_back(pi);
+                    }
                 }
             }
-        }
-        const int n = (int)cur_par.size();
-        if (n == 0) break;
-        ++n_batches;
-        const bool store = lvl < k_max;
+            const int n = (int)cur_par.size();
+            if (n == 0) break;
+            const bool probe_only = lvl > k_max;
+            const bool store = lvl < last_lvl;
+            const bool bases = keep_bases && !probe_only;
 
-        DBuf dsets(sizeof(int) * (size_t)n * kw), dpar(sizeof(int) * (size_t)n);
-        DBuf drec(sizeof(DevRecord) * (size_t)n);
-        HIP_OK(hipMemcpyAsync(dsets.p, cur_sets.data(), dsets.n, hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync(dpar.p, cur_par.data(), dpar.n, hipMemcpyHostToDevice, st));
-        DBuf fstore_cur, ostore_cur;
-        if (store) {
-            fstore_cur.alloc((size_t)fstride * n);
-            ostore_cur.alloc((size_t)ostride * n);
-        }
-        DBuf dfb, db1, db2;
-        if (keep_bases) {
-            dfb.alloc(sizeof(int) * (size_t)n * M1);
-            db1.alloc(sizeof(int) * (size_t)n * M2);
-            db2.alloc(sizeof(int) * (size_t)n * M2);
-            HIP_OK(hipMemsetAsync(db1.p, 0xff, db1.n, st));
-            HIP_OK(hipMemsetAsync(db2.p, 0xff, db2.n, st));
-        }
-
-        FeasArgs fa{};
-        fa.P = P;
-        fa.O = dop;
-        fa.n = n;
-        fa.level = lvl;
-        fa.kw = kw;
-        fa.sets = dsets.as<int>();
-        fa.parent = dpar.as<int>();
-        fa.fstore_prev = fstore_prev.as<unsigned char>();
-        fa.fstore_cur = fstore_cur.as<unsigned char>();
-        fa.fstride = fstride;
-        fa.store = store ? 1 : 0;
-        fa.rec = drec.as<DevRecord>();
-        fa.fbasis = keep_bases ? dfb.as<int>() : nullptr;
-        HIP_OK(hipEventRecord(e0, st));
-        hipLaunchKernelGGL(feas_kernel<FEAS_NT>, dim3(n), dim3(FEAS_NT), (size_t)flds, st, fa);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipEventRecord(e1, st));
-        hrec.resize(n);
-        HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        tf += ms * 1e-3;
-        C.stats.ms_feas += ms;
-
-        std::vector<int> list, p1par(n, -1);
-        for (int c = 0; c < n; ++c) {
-            if (hrec[c].feasible) list.push_back(c);
-            const int pi = cur_par[c];
-            if (pi >= 0 && prev_kkt[pi]) p1par[c] = pi;
-        }
-        const int nf = (int)list.size();
-        n_feas_total += nf;
-        if (nf > 0) {
-            DBuf dlist(sizeof(int) * (size_t)nf), dp1(sizeof(int) * (size_t)n);
-            HIP_OK(hipMemcpyAsync(dlist.p, list.data(), dlist.n, hipMemcpyHostToDevice, st));
-            HIP_OK(hipMemcpyAsync(dp1.p, p1par.data(), dp1.n, hipMemcpyHostToDevice, st));
-            OptArgs oa{};
-            oa.P = P;
-            oa.O = dop;
-            oa.n = nf;
-            oa.level = lvl;
-            oa.kw = kw;
-            oa.list = dlist.as<int>();
-            oa.sets = dsets.as<int>();
-            oa.p1parent = dp1.as<int>();
-            oa.ostore_prev = ostore_prev.as<unsigned char>();
-            oa.ostore_cur = ostore_cur.as<unsigned char>();
-            oa.ostride = ostride;
-            oa.store = store ? 1 : 0;
-            oa.rec = drec.as<DevRecord>();
-            oa.b1 = keep_bases ? db1.as<int>() : nullptr;
-            oa.b2 = keep_bases ? db2.as<int>() : nullptr;
-            DBuf dtrace, dprof;
-            const char *prof_path = std::getenv("MPQP_DEBUG_PROF");
-            if (prof_path) {
-                dprof.alloc(sizeof(long long) * 8 * 4096);
-                HIP_OK(hipMemsetAsync(dprof.p, 0, dprof.n, st));
-                oa.prof = dprof.as<long long>();
+            DBuf dsets(sizeof(int) * (size_t)n * kw), dpar(sizeof(int) * (size_t)n);
+            DBuf drec(sizeof(DevRecord) * (size_t)n);
+            HIP_OK(hipMemcpyAsync(dsets.p, cur_sets.data(), dsets.n, hipMemcpyHostToDevice, st));
+            HIP_OK(hipMemcpyAsync(dpar.p, cur_par.data(), dpar.n, hipMemcpyHostToDevice, st));
+            DBuf fstore_cur, ostore_cur;
+            if (store) {
+                fstore_cur.alloc((size_t)fstride * n);
+                ostore_cur.alloc((size_t)ostride * n);
             }
-            const char *tr_path = std::getenv("MPQP_DEBUG_TRACE");
-            if (tr_path && lvl == 0) {
-                dtrace.alloc(sizeof(int) * 4 * 256);
-                HIP_OK(hipMemsetAsync(dtrace.p, 0xff, dtrace.n, st));
-                oa.trace = dtrace.as<int>();
+            DBuf dfb, db1, db2;
+            if (bases) {
+                dfb.alloc(sizeof(int) * (size_t)n * M1);
+                db1.alloc(sizeof(int) * (size_t)n * M2);
+                db2.alloc(sizeof(int) * (size_t)n * M2);
+                const int nb1 = n * M2;
+                hipLaunchKernelGGL(fill_i32, dim3((nb1 + 255) / 256), dim3(256), 0, st, db1.as<int>(), nb1, -1);
+                hipLaunchKernelGGL(fill_i32, dim3((nb1 + 255) / 256), dim3(256), 0, st, db2.as<int>(), nb1, -1);
+                HIP_OK(hipGetLastError());
             }
-            HIP_OK(hipEventRecord(e1, st));
-            launch_opt(opt_nt, red_slots(std::max(M2, N2 - n_eqs)), nf, (size_t)olds, st, oa);
+
+            FeasArgs fa{};
+            fa.P = P;
+            fa.O = dop;
+            fa.n = n;
+            fa.level = lvl;
+            fa.kw = kw;
+            fa.sets = dsets.as<int>();
+            fa.parent = dpar.as<int>();
+            fa.fstore_prev = fstore_prev.as<unsigned char>();
+            fa.fstore_cur = fstore_cur.as<unsigned char>();
+            fa.fstride = fstride;
+            fa.store = store ? 1 : 0;
+            fa.rec = drec.as<DevRecord>();
+            fa.fbasis = bases ? dfb.as<int>() : nullptr;
+            HIP_OK(hipEventRecord(e0, st));
+            hipLaunchKernelGGL(feas_kernel<FEAS_NT>, dim3(n), dim3(FEAS_NT), (size_t)flds, st, fa);
             HIP_OK(hipGetLastError());
-            HIP_OK(hipEventRecord(e2, st));
+            HIP_OK(hipEventRecord(e1, st));
+            hrec.resize(n);
             HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
-            HIP_OK(hipEventElapsedTime(&ms, e1, e2));
-            to += ms * 1e-3;
-            C.stats.ms_opt += ms;
-            if (oa.prof) {
-                std::vector<long long> hp(8 * 4096);
-                HIP_OK(hipMemcpy(hp.data(), oa.prof, hp.size() * 8, hipMemcpyDeviceToHost));
-                std::string pth = std::string(prof_path) + "." + std::to_string(lvl);
-                if (FILE *f = std::fopen(pth.c_str(), "wb")) {
-                    std::fwrite(hp.data(), 8, hp.size(), f);
-                    std::fclose(f);
-                }
-            }
-            if (oa.trace) {
-                std::vector<int> ht(4 * 256);
-                HIP_OK(hipMemcpy(ht.data(), oa.trace, ht.size() * 4, hipMemcpyDeviceToHost));
-                if (FILE *f = std::fopen(tr_path, "wb")) {
-                    std::fwrite(ht.data(), 4, ht.size(), f);
-                    std::fclose(f);
-                }
-            }
-        }
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            tf += ms * 1e-3;
+            C.stats.ms_feas += ms;
 
-        {
-            MpqpStats &S = C.stats;
-            S.levels += 1;
-            S.launches_feas += 1;
-            S.launches_opt += nf > 0 ? 1 : 0;
-            S.candidates += n;
-            S.opt_lps += nf;
-            long long pf = 0, p1 = 0, p2 = 0;
+            // optimality LPs of the feasible candidates; warm start from the
+            // parent's phase-1 state when that state is clean
+            std::vector<int> list, p1par(n, -1);
             for (int c = 0; c < n; ++c) {
-                pf += hrec[c].feas_iters;
-                p1 += hrec[c].p1_iters;
-                p2 += hrec[c].p2_iters;
+                if (hrec[c].feasible) list.push_back(c);
+                const int pi = cur_par[c];
+                if (pi >= 0 && prev_clean[pi]) p1par[c] = pi;
             }
-            S.pivots_feas += pf;
-            S.pivots_p1 += p1;
-            S.pivots_p2 += p2;
-            const double s8 = 8.0;
-            auto bytes = [&](double M, double K) { return 2.0 * (M + 1) * (K + 1) * s8 + (M + K + 2) * s8; };
-            auto flops = [&](double M, double K) { return 2.0 * (M + 1) * (K + 1); };
-            const double Kp2 = (double)(N2 - n_eqs - M2);
-            S.bytes_feas += pf * bytes(M1, K1);
-            S.flops_feas += pf * flops(M1, K1);
-            S.bytes_opt += p1 * bytes(M2, K2) + p2 * bytes(M2, Kp2);
-            S.flops_opt += p1 * flops(M2, K2) + p2 * flops(M2, Kp2);
-        }
-        const int base = (int)all.size();
-        std::vector<int> kkt(n);
-        for (int c = 0; c < n; ++c) {
-            MpqpRecord r;
-            std::memcpy(&r, &hrec[c], sizeof r);
-            r.level = lvl;
-            r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
-            all.push_back(r);
-            kkt[c] = r.kkt_feasible;
-        }
-        allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
-        if (keep_bases) {
-            size_t o1 = allfb.size(), o2 = allb1.size();
-            allfb.resize(o1 + (size_t)n * M1);
-            allb1.resize(o2 + (size_t)n * M2);
-            allb2.resize(o2 + (size_t)n * M2);
-            HIP_OK(hipMemcpyAsync(allfb.data() + o1, dfb.p, dfb.n, hipMemcpyDeviceToHost, st));
-            HIP_OK(hipMemcpyAsync(allb1.data() + o2, db1.p, db1.n, hipMemcpyDeviceToHost, st));
-            HIP_OK(hipMemcpyAsync(allb2.data() + o2, db2.p, db2.n, hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            // bases of LPs that did not run are reported as -1
-            for (int c = 0; c < n; ++c) {
-                if (!hrec[c].feasible) {
-                    std::fill(allb1.begin() + o2 + (size_t)c * M2, allb1.begin() + o2 + (size_t)(c + 1) * M2, -1);
-                    std::fill(allb2.begin() + o2 + (size_t)c * M2, allb2.begin() + o2 + (size_t)(c + 1) * M2, -1);
-                } else if (!hrec[c].kkt_feasible) {
-                    std::fill(allb2.begin() + o2 + (size_t)c * M2, allb2.begin() + o2 + (size_t)(c + 1) * M2, -1);
+            const int nf = (int)list.size();
+            int n_fb = 0;
+            if (nf > 0) {
+                DBuf dlist(sizeof(int) * (size_t)nf), dp1(sizeof(int) * (size_t)n), dfbl;
+                HIP_OK(hipMemcpyAsync(dlist.p, list.data(), dlist.n, hipMemcpyHostToDevice, st));
+                HIP_OK(hipMemcpyAsync(dp1.p, p1par.data(), dp1.n, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
+                HIP_OK(hipGetLastError());
+                OptArgs oa{};
+                oa.P = P;
+                oa.O = dop;
+                oa.n = nf;
+                oa.level = lvl;
+                oa.kw = kw;
+                oa.list = dlist.as<int>();
+                oa.sets = dsets.as<int>();
+                oa.p1parent = dp1.as<int>();
+                oa.ostore_prev = ostore_prev.as<unsigned char>();
+                oa.ostore_cur = ostore_cur.as<unsigned char>();
+                oa.ostride = ostride;
+                oa.store = store ? 1 : 0;
+                oa.rec = drec.as<DevRecord>();
+                oa.b1 = bases ? db1.as<int>() : nullptr;
+                oa.b2 = bases ? db2.as<int>() : nullptr;
+                HIP_OK(hipEventRecord(e1, st));
+                if (mixed) {
+                    dfbl.alloc(sizeof(int) * (size_t)nf);
+                    oa.counter = dctr.as<int>();
+                    oa.fb_list = dfbl.as<int>();
+                    oa.fb_count = dctr.as<int>() + 1;
+                    oa.cert = dcert.as<double>();
+                    launch(fn32, std::min(nf, cap32), lds32, oa);
+                    // FP64 re-solve from the cold start of what failed certification
+                    OptArgs ob = oa;
+                    ob.list = dfbl.as<int>();
+                    ob.n_dev = dctr.as<int>() + 1;
+                    ob.counter = dctr.as<int>() + 2;
+                    ob.force_cold = 1;
+                    ob.store32 = 1;
+                    ob.fb_list = nullptr;
+                    ob.fb_count = nullptr;
+                    ob.cert = nullptr;
+                    HIP_OK(hipEventRecord(e3, st));
+                    launch(fn64, std::min(nf, cap64), lds64, ob);
+                } else if (o.batching == MPQP_PERSISTENT) {
+                    oa.counter = dctr.as<int>();
+                    launch(fn64, std::min(nf, cap64), lds64, oa);
+                } else {
+                    launch(fn64, nf, lds64, oa);
+                }
+                HIP_OK(hipEventRecord(e2, st));
+                HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                HIP_OK(hipEventElapsedTime(&ms, e1, e2));
+                to += ms * 1e-3;
+                C.stats.ms_opt += ms;
+                C.stats.launches_opt += mixed ? 2 : 1;
+                if (mixed) {
+                    HIP_OK(hipEventElapsedTime(&ms, e3, e2));
+                    C.stats.ms_fallback += ms;
+                    C.stats.launches_fallback += 1;
+                }
+                for (int c = 0; c < n; ++c) n_fb += hrec[c].repivots;
+            }
+            if (mixed && lvl <= 1) {
+                probe_opt += nf;
+                probe_fb += n_fb;
+                if (lvl == 1 && probe_opt > 0 && (double)probe_fb > 0.25 * (double)probe_opt) {
+                    restart = true;
+                    break;
                 }
             }
+            if (probe_only) break;
+            n_feas_total += nf;
+            ++n_batches;
+            {
+                MpqpStats &St = C.stats;
+                St.levels += 1;
+                St.launches_feas += 1;
+                St.candidates += n;
+                St.opt_lps += nf;
+                St.fallbacks += n_fb;
+                long long pf = 0, p1 = 0, p2 = 0;
+                double b_opt = 0.0, f_opt = 0.0;
+                const double Kp2 = (double)(N2 - n_eqs - M2);
+                auto bytes = [&](double Mr, double Kc, double s) {
+                    return 2.0 * (Mr + 1) * (Kc + 1) * s + (Mr + Kc + 2) * s;
+                };
+                auto flops = [&](double Mr, double Kc) { return 2.0 * (Mr + 1) * (Kc + 1); };
+                for (int c = 0; c < n; ++c) {
+                    pf += hrec[c].feas_iters;
+                    p1 += hrec[c].p1_iters;
+                    p2 += hrec[c].p2_iters;
+                    // FP32 pivots unless the LP was re-solved in FP64
+                    const double s = (mixed && !hrec[c].repivots) ? 4.0 : 8.0;
+                    const double bc = hrec[c].p1_iters * bytes(M2, K2, s) + hrec[c].p2_iters * bytes(M2, Kp2, s);
+                    const double fc = hrec[c].p1_iters * flops(M2, K2) + hrec[c].p2_iters * flops(M2, Kp2);
+                    b_opt += bc;
+                    f_opt += fc;
+                    if (mixed && hrec[c].repivots) {
+                        St.bytes_fallback += bc;
+                        St.flops_fallback += fc;
+                    }
+                }
+                St.pivots_feas += pf;
+                St.pivots_p1 += p1;
+                St.pivots_p2 += p2;
+                St.bytes_feas += pf * bytes(M1, K1, 8.0);
+                St.flops_feas += pf * flops(M1, K1);
+                St.bytes_opt += b_opt;
+                St.flops_opt += f_opt;
+            }
+            const int base = (int)all.size();
+            std::vector<int> clean(n);
+            for (int c = 0; c < n; ++c) {
+                MpqpRecord r;
+                std::memcpy(&r, &hrec[c], sizeof r);
+                r.level = lvl;
+                r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
+                all.push_back(r);
+                clean[c] = r.p1_clean;
+            }
+            allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
+            if (bases) {
+                size_t o1 = allfb.size(), o2 = allb1.size();
+                allfb.resize(o1 + (size_t)n * M1);
+                allb1.resize(o2 + (size_t)n * M2);
+                allb2.resize(o2 + (size_t)n * M2);
+                HIP_OK(hipMemcpyAsync(allfb.data() + o1, dfb.p, dfb.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(allb1.data() + o2, db1.p, db1.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(allb2.data() + o2, db2.p, db2.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+            }
+            prev_sets.swap(cur_sets);
+            prev_feas.assign(n, 0);
+            for (int c = 0; c < n; ++c) prev_feas[c] = hrec[c].feasible;
+            prev_clean.swap(clean);
+            prev_recidx.resize(n);
+            for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
+            prev_n = n;
+            fstore_prev = std::move(fstore_cur);
+            ostore_prev = std::move(ostore_cur);
         }
-        prev_sets.swap(cur_sets);
-        prev_feas.assign(n, 0);
-        for (int c = 0; c < n; ++c) prev_feas[c] = hrec[c].feasible;
-        prev_kkt.swap(kkt);
-        prev_recidx.resize(n);
-        for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
-        prev_n = n;
-        fstore_prev = std::move(fstore_cur);
-        ostore_prev = std::move(ostore_cur);
+        if (!restart) break;
+        mixed = false;
     }
+    C.stats.precision_used = mixed ? MPQP_MIXED : MPQP_FP64;
     HIP_OK(hipStreamSynchronize(st));
     const double t_gpu = now_s() - t_start;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
 
     GPUResults *res = static_cast<GPUResults *>(std::calloc(1, sizeof(GPUResults)));
     int nfin = 0;

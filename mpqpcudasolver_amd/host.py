@@ -129,6 +129,11 @@ class MpqpStats(ctypes.Structure):
         ("flops_feas", ctypes.c_double),
         ("flops_opt", ctypes.c_double),
         ("fallbacks", ctypes.c_longlong),
+        ("launches_fallback", ctypes.c_int),
+        ("reserved1", ctypes.c_int),
+        ("ms_fallback", ctypes.c_double),
+        ("bytes_fallback", ctypes.c_double),
+        ("flops_fallback", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

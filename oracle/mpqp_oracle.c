@@ -50,7 +50,22 @@
  * Re-specified behaviour (the reference is racy, batch-global or undefined):
  *   - n_cycles_same_lps (:1542-1546) counts batch iterations in which no LP
  *     finished; here each LP counts its own iterations since its phase-1
- *     primal infeasibility sum(min(x_B,0)) last reached a new best.
+ *     primal infeasibility sum(min(x_B,0)) last reached a new best.  The
+ *     rhs perturbation starts at 20 such iterations (:1550); the stall exit
+ *     (300 batch iterations, :1561) becomes a per-LP limit that defaults to
+ *     the phase-1 cap: with 300 per LP, 1 LP of the n=30 m=62 p=8 problem
+ *     and 155 of its ill-conditioned variant stop short of an optimum that
+ *     they reach within 4669 iterations, which loses final sets.
+ *   - a stop on the perturbed rhs re-checks the real rhs; if it is still
+ *     infeasible the LP continues without perturbation.
+ *   - the counter modes may accept non-minimal dual ratios, leaving the
+ *     final basis dual infeasible; phase 1 then finishes with primal simplex
+ *     pivots on the phase-1 cost (Dantzig pricing, lowest variable on ties)
+ *     until every reduced cost exceeds tol_minus.
+ *   - ratio tests also require |entry| > pivot_rel * (largest |entry| of
+ *     the column, or of the row in the dual ratio test), default 1e-9: tiny
+ *     pivots admitted by the absolute 1e-9 threshold alone produced
+ *     near-singular bases and wrong margins.
  *   - the artificial-in-basis test of :238 compares row positions with N_
  *     and never fires; here phase 1 is KKT-feasible iff its optimum
  *     |z| < dual_feasible_tol (a property of the LP, not of the pivot path),
@@ -61,6 +76,14 @@
  *   - an empty ratio test (stale th_idxs :871-876, idx -1 write :941)
  *     ends the LP with status UNBOUNDED (phase 1: infeasible).
  *   - LPs stopped by a cap or the stall exit keep verdict 0 with a status.
+ *
+ * Mixed precision (no counterpart in the reference, which is FP64 only):
+ * the optimality LP pivots in FP32 (tolerances tol32 / licq32 /
+ * pivot_rel32) from the parent's FP32 phase-1 state; its final bases are
+ * certified in FP64 by iterative refinement (primal x_B, dual y, reduced
+ * costs) and the certified z decides the verdicts; an LP that fails
+ * certification is solved again in FP64 from the cold start.  Levels 0 and
+ * 1 decide per problem whether FP32 pivots are used at all (mixed_probe).
  *
  * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
  */
@@ -99,7 +122,7 @@ void mpqp_default_options(MpqpOptions *o) {
     o->max_iter_phase1 = 10000;
     o->max_iter_phase2 = 800;
     o->stall_perturb = 20;
-    o->stall_exit = 300;
+    o->stall_exit = 10000;
     o->opt_threads = 0;
     o->warm_start = 1;
     o->tol_plus = 1e-9;

@@ -4,7 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 INT_FIELDS = ["level", "parent", "feas_status", "feas_iters", "feasible", "p1_status", "p1_iters",
-              "kkt_feasible", "p1_warm", "p2_status", "p2_iters", "final_set"]
+              "kkt_feasible", "p1_warm", "p2_status", "p2_iters", "final_set", "repivots", "p1_clean"]
 FLOAT_FIELDS = ["z_feas", "z1", "z2"]
 # objectives and margins: 1e-9 relative (scale floor 1)
 REL_TOL = 1e-9

@@ -50,14 +50,14 @@ def test_gpuresults_layout_matches_reference():
 def test_struct_sizes():
     assert ctypes.sizeof(MpqpRecord) == 80
     assert ctypes.sizeof(MpqpOptions) == 48 + 15 * 8
-    assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8
+    assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8 + 8 + 3 * 8
 
 
 def test_backend_and_options(hip):
     assert hip.backend == "hip-gfx950"
     d = hip.default_options()
     assert (d.max_iter_feas, d.max_iter_phase1, d.max_iter_phase2) == (500, 10000, 800)
-    assert (d.stall_perturb, d.stall_exit) == (20, 300)
+    assert (d.stall_perturb, d.stall_exit) == (20, 10000)
     assert d.feas_z_tol == 1e-12 and d.perturb_step == 1e-6
     hip.set_options(d, tol_plus=1e-8)
     assert hip.get_options().tol_plus == 1e-8

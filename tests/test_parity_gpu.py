@@ -175,6 +175,107 @@ def test_parity_full_config_sampled(hip, orc, cfg):
             assert abs(rec[f] - r[f][c]) <= REL_TOL * max(1.0, abs(rec[f])), (sets[c], f)
 
 
+# ------------------------------------------------------- mixed precision
+
+@pytest.mark.parametrize("name", ["config0", "small8x24", "stress10x24"])
+def test_parity_small_mixed(hip, orc, name):
+    """FP32 pivots + FP64 certification + FP64 re-solve: identical to the
+    oracle's mixed path record for record (FP32 arithmetic is IEEE on both)."""
+    _log(f"case: mixed parity {name}")
+    prog, k = {
+        "config0": lambda: config_problem(0),
+        "small8x24": lambda: (synthetic_mpqp(8, 24, 3, 11), 3),
+        "stress10x24": lambda: (precision_stress_mpqp(10, 24, 3, 13, n_dup=4), 3),
+    }[name]()
+    g, o = _run_pair(hip, orc, assemble_lps(prog), k, precision=1)
+    assert compare_runs(g, o) == []
+    assert hip.stats()["precision_used"] == orc.stats()["precision_used"]
+    hip.set_options()
+    orc.set_options()
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_parity_config_first_levels_mixed(hip, orc, cfg):
+    _log(f"case: mixed parity config {cfg}, |S| <= 2")
+    prog, _ = config_problem(cfg)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 2, precision=1)
+    assert compare_runs(g, o) == []
+    assert hip.stats()["precision_used"] == orc.stats()["precision_used"]
+    hip.set_options()
+    orc.set_options()
+
+
+def test_mixed_k0_probe(hip, orc):
+    """k_max = 0: level 1 is solved only to decide the precision."""
+    _log("case: mixed parity k_max = 0")
+    prog = synthetic_mpqp(8, 24, 3, 11)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 0, precision=1)
+    assert compare_runs(g, o) == []
+    assert len(g.records) == 1
+    hip.set_options()
+    orc.set_options()
+
+
+@pytest.mark.parametrize("cfg", [1, 3])
+def test_mixed_verdicts_match_fp64(hip, cfg):
+    """Full |S| <= 4 run: the mixed path gives the FP64 path's verdicts
+    (feasible, KKT-feasible, final) on every candidate."""
+    _log(f"case: config {cfg} full run, mixed vs FP64 verdicts")
+    prog, k = config_problem(cfg)
+    lp = assemble_lps(prog)
+    hip.set_options(record=1)
+    f = hip.run(lp, k_max=k, records=True)
+    hip.set_options(record=1, precision=1)
+    m = hip.run(lp, k_max=k, records=True)
+    st = hip.stats()
+    hip.set_options()
+    assert np.array_equal(f.sets, m.sets)
+    for fld in ("feasible", "kkt_feasible", "final_set"):
+        d = np.flatnonzero(f.records[fld] != m.records[fld])
+        assert len(d) == 0, (fld, len(d), d[:5])
+    assert f.active_sets == m.active_sets
+    _log(f"  precision_used {st['precision_used']}, fallbacks {st['fallbacks']} of {st['opt_lps']}")
+    if cfg == 1:
+        assert st["precision_used"] == 1
+        assert st["fallbacks"] <= 0.05 * st["opt_lps"]
+
+
+def test_mixed_full_config_sampled(hip, orc):
+    """Config 1 full run in mixed precision; 60 random candidates re-evaluated
+    by the oracle's mixed path along their prefix chains."""
+    _log("case: config 1 full mixed run, sampled parity")
+    prog, k = config_problem(1)
+    lp = assemble_lps(prog)
+    hip.set_options(record=1, precision=1)
+    orc.set_options(precision=1)
+    g = hip.run(lp, k_max=k, records=True)
+    r = g.records
+    sets = [[int(v) for v in row if v >= 0] for row in g.sets]
+    rng = np.random.default_rng(7)
+    for c in rng.choice(len(sets), 60, replace=False):
+        rec = orc.eval_set(lp, sets[c])[0]
+        for f in INT_FIELDS:
+            if f == "parent":
+                continue
+            assert rec[f] == r[f][c], (sets[c], f, rec[f], r[f][c])
+        for f in FLOAT_FIELDS:
+            assert abs(rec[f] - r[f][c]) <= REL_TOL * max(1.0, abs(rec[f])), (sets[c], f)
+    hip.set_options()
+    orc.set_options()
+
+
+def test_persistent_batching_matches_plain(hip):
+    _log("case: persistent FP64 launch vs plain")
+    prog, _ = config_problem(1)
+    lp = assemble_lps(prog)
+    hip.set_options(record=2)
+    a = hip.run(lp, k_max=2, records=True, bases=True)
+    hip.set_options(record=2, batching=1)
+    b = hip.run(lp, k_max=2, records=True, bases=True)
+    hip.set_options()
+    assert compare_runs(a, b) == []
+
+
 def test_errors_are_reported(hip):
     _log("case: invalid input")
     prog, k = config_problem(0)
