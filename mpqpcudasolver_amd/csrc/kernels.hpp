@@ -52,8 +52,7 @@ struct OptArgs {
     int *b1, *b2;         // by candidate x M2, or null
     int *fb_list;         // mixed: candidates whose FP32 result failed
     int *fb_count;        //        certification (appended), and their count
-    double *cert;         // mixed: 6 x (M2 + 8) doubles of scratch per workgroup
-    long long *prof;      // phase cycle counts, 8 per workgroup (MPQP_PROFILE builds), or null
+    long long *prof;      // phase cycle counts, 16 per workgroup (MPQP_PROFILE builds), or null
 };
 
 // Byte layout of one stored state: T (rows x cols doubles, contiguous) then

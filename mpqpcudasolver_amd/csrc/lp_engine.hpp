@@ -55,6 +55,15 @@ struct DevProb {
     const int *root_basis;   // M2
     const int *root_nonb;    // K2
     const int *root_ok;      // 1 int
+    double cert_ptol;        // primal certification: 1e-12 * (1 + max|xb2|)
+    // A2 (column-major, M2 x N2) in sparse form for the FP64 certification,
+    // ELL layout (entry t of line l at t * lines + l, so that consecutive
+    // lines are adjacent in memory): rows with their variables ascending,
+    // columns with their rows ascending.
+    const int *csr_n, *csr_var;    // per row: count; M2 * csr_w
+    const double *csr_val;
+    const int *csc_n, *csc_row;    // per column: count; N2 * csc_w
+    const double *csc_val;
 };
 
 // Mirrors MpqpRecord of the public header.
@@ -120,13 +129,26 @@ __device__ __forceinline__ double lane_down_v(double x) { return lane_down_d<I>(
 template <int I>
 __device__ __forceinline__ float lane_down_v(float x) { return lane_down_f<I>(x); }
 
-// Wave maximum (order-free), broadcast to all lanes.
+template <int I>
+__device__ __forceinline__ double max_step(double v, int lane) {
+    const double o = lane_down_d<I>(v);
+    return ((lane < I) & (o > v)) ? o : v;
+}
+
+// Wave maximum (order-free): DPP / swizzle / permlane steps into lane 0,
+// then broadcast with readfirstlane (all lanes active).
 __device__ __forceinline__ double wave_max(double v) {
-    for (int d = 32; d > 0; d >>= 1) {
-        const double o = __shfl_xor(v, d, MPQP_WAVE);
-        v = o > v ? o : v;
-    }
-    return v;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    v = max_step<32>(v, lane);
+    v = max_step<16>(v, lane);
+    v = max_step<8>(v, lane);
+    v = max_step<4>(v, lane);
+    v = max_step<2>(v, lane);
+    v = max_step<1>(v, lane);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // ---------------------------------------------------------------------
@@ -219,8 +241,9 @@ __device__ __forceinline__ void sum_step(V (&v)[S], int n, int lane) {
     v[0] = ((lane < I) & (lane + I < n)) ? v[0] + ov : v[0];
 }
 
+// Tree sum left in lane 0, slot 0 (other lanes hold partial sums).
 template <int S, typename V>
-__device__ __forceinline__ V tree_sum_s(V (&v)[S], int n, int lane) {
+__device__ __forceinline__ void tree_sum_lane0(V (&v)[S], int n, int lane) {
     const int half = half_pow2(n);
 #pragma unroll
     for (int lvl = 2; lvl >= 0; --lvl) {
@@ -242,6 +265,53 @@ __device__ __forceinline__ V tree_sum_s(V (&v)[S], int n, int lane) {
     if (half >= 4) sum_step<S, 4, V>(v, n, lane);
     if (half >= 2) sum_step<S, 2, V>(v, n, lane);
     if (half >= 1) sum_step<S, 1, V>(v, n, lane);
+}
+
+// Four independent tree sums (each exactly tree_sum_lane0 of its own n),
+// evaluated step by step together with selects so that their cross-lane
+// latencies overlap.
+template <int S>
+__device__ __forceinline__ void tree_sum4(double (&t)[4][S], const int (&n)[4], int lane) {
+    int half[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) half[q] = half_pow2(n[q]);
+#pragma unroll
+    for (int lvl = 2; lvl >= 0; --lvl) {
+        const int i = 64 << lvl;
+        const int ds = 1 << lvl;
+#pragma unroll
+        for (int s = 0; s < ds; ++s) {
+            if (s + ds < S) {
+                const int u = lane + 64 * (s + ds);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool take = (i <= half[q]) & (u < n[q]);
+                    t[q][s] = take ? t[q][s] + t[q][s + ds] : t[q][s];
+                }
+            }
+        }
+    }
+#define MPQP_SUM4_STEP(I)                                                          \
+    {                                                                              \
+        double o[4];                                                               \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) o[q] = lane_down_d<I>(t[q][0]); \
+        _Pragma("unroll") for (int q = 0; q < 4; ++q) {                            \
+            const bool take = (half[q] >= I) & (lane < I) & (lane + I < n[q]);     \
+            t[q][0] = take ? t[q][0] + o[q] : t[q][0];                             \
+        }                                                                          \
+    }
+    MPQP_SUM4_STEP(32)
+    MPQP_SUM4_STEP(16)
+    MPQP_SUM4_STEP(8)
+    MPQP_SUM4_STEP(4)
+    MPQP_SUM4_STEP(2)
+    MPQP_SUM4_STEP(1)
+#undef MPQP_SUM4_STEP
+}
+
+template <int S, typename V>
+__device__ __forceinline__ V tree_sum_s(V (&v)[S], int n, int lane) {
+    tree_sum_lane0<S, V>(v, n, lane);
     return __shfl(v[0], 0, MPQP_WAVE);
 }
 

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -33,23 +34,26 @@ namespace {
 constexpr int FEAS_NT = 64;
 constexpr int OPT_NT = 512;
 
-template <typename Real, int NT, bool MIXED>
+template <int NT, bool FP32>
 const void *opt_fn_nt(int S) {
+#define MPQP_OPT_FN(s) \
+    (FP32 ? reinterpret_cast<const void *>(&opt_kernel32<NT, s>) : reinterpret_cast<const void *>(&opt_kernel64<NT, s>))
     switch (S) {
-    case 1: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 1, MIXED>);
-    case 2: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 2, MIXED>);
-    case 3: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 3, MIXED>);
-    case 4: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 4, MIXED>);
-    case 6: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 6, MIXED>);
-    default: return reinterpret_cast<const void *>(&opt_kernel_t<Real, NT, 8, MIXED>);
+    case 1: return MPQP_OPT_FN(1);
+    case 2: return MPQP_OPT_FN(2);
+    case 3: return MPQP_OPT_FN(3);
+    case 4: return MPQP_OPT_FN(4);
+    case 6: return MPQP_OPT_FN(6);
+    default: return MPQP_OPT_FN(8);
     }
+#undef MPQP_OPT_FN
 }
 
 // The optimality kernel for a precision (FP32 pivots with certification, or
 // FP64), workgroup size and reduction slot count.
 const void *opt_fn(bool fp32, int nt, int S) {
-    if (fp32) return nt == 256 ? opt_fn_nt<float, 256, true>(S) : opt_fn_nt<float, 512, true>(S);
-    return nt == 256 ? opt_fn_nt<double, 256, false>(S) : opt_fn_nt<double, 512, false>(S);
+    if (fp32) return nt == 256 ? opt_fn_nt<256, true>(S) : opt_fn_nt<512, true>(S);
+    return nt == 256 ? opt_fn_nt<256, false>(S) : opt_fn_nt<512, false>(S);
 }
 
 __global__ void fill_i32(int *p, int n, int v) {
@@ -296,7 +300,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     if (feas_lds_bytes(M1, N1 - M1, n_c) > C.lds_limit)
         throw HipError{"mpqp: feasibility tableau exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
-    if (opt_lds_bytes_t<double>(M2, N2 - M2, N2) > C.lds_limit)
+    if (opt_lds_bytes_t<double>(M2, N2 - M2, N2, false) > C.lds_limit)
         throw HipError{"mpqp: optimality tableau exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
     hipStream_t st = C.stream;
@@ -319,6 +323,55 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     P.M2 = M2; P.N2 = N2; P.K2 = K2; P.Nn = N2 - n_eqs; P.n_eqs = n_eqs;
     P.A1 = dA1.as<double>(); P.xb1 = dxb1.as<double>();
     P.A2 = dA2.as<double>(); P.xb2 = dxb2.as<double>();
+    {
+        double bmax = 0.0;
+        for (int rho = 0; rho < M2; ++rho) bmax = std::max(bmax, std::fabs(xb2[rho]));
+        P.cert_ptol = 1e-12 * (1.0 + bmax);
+    }
+
+    // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification, in
+    // ELL layout: entry t of row rho at t * M2 + rho (variables ascending),
+    // entry t of column v at t * N2 + v (rows ascending)
+    std::vector<int> rn(M2, 0), cn(N2, 0);
+    for (int v = 0; v < N2; ++v)
+        for (int rho = 0; rho < M2; ++rho)
+            if (A2[(size_t)v * M2 + rho] != 0.0) {
+                rn[rho]++;
+                cn[v]++;
+            }
+    int rw = 1, cw = 1;
+    for (int rho = 0; rho < M2; ++rho) rw = std::max(rw, rn[rho]);
+    for (int v = 0; v < N2; ++v) cw = std::max(cw, cn[v]);
+    std::vector<int> rvar((size_t)rw * M2, -1), crow((size_t)cw * N2, -1), rfill(M2, 0), cfill(N2, 0);
+    std::vector<double> rval((size_t)rw * M2, 0.0), cval((size_t)cw * N2, 0.0);
+    for (int v = 0; v < N2; ++v)
+        for (int rho = 0; rho < M2; ++rho) {
+            const double a = A2[(size_t)v * M2 + rho];
+            if (a == 0.0) continue;
+            const size_t er = (size_t)rfill[rho]++ * M2 + rho, ec = (size_t)cfill[v]++ * N2 + v;
+            rvar[er] = v;
+            rval[er] = a;
+            crow[ec] = rho;
+            cval[ec] = a;
+        }
+    DBuf dsp_i(sizeof(int) * (M2 + N2 + rvar.size() + crow.size()));
+    DBuf dsp_d(sizeof(double) * (rval.size() + cval.size()));
+    {
+        int *ib0 = dsp_i.as<int>();
+        double *db0 = dsp_d.as<double>();
+        P.csr_n = ib0;
+        P.csc_n = ib0 + M2;
+        P.csr_var = ib0 + M2 + N2;
+        P.csc_row = P.csr_var + rvar.size();
+        P.csr_val = db0;
+        P.csc_val = db0 + rval.size();
+        HIP_OK(hipMemcpyAsync((void *)P.csr_n, rn.data(), sizeof(int) * rn.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csc_n, cn.data(), sizeof(int) * cn.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csr_var, rvar.data(), sizeof(int) * rvar.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csc_row, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csr_val, rval.data(), sizeof(double) * rval.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csc_val, cval.data(), sizeof(double) * cval.size(), hipMemcpyHostToDevice, st));
+    }
 
     // root tableau
     DBuf dG(sizeof(double) * (size_t)M2 * (K2 + 2)), droot(sizeof(double) * (size_t)M2 * (K2 + 2));
@@ -359,8 +412,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
 
     const long long fstride = state_bytes(M1, K1 + 1, M1, K1);
     const long long flds = feas_lds_bytes(M1, K1, n_c);
-    const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2);
-    const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2);
+    const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false);
+    const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true);
     const int opt_nt = (o.opt_threads == 256) ? 256 : OPT_NT;
     const int S = red_slots(std::max(M2, N2 - n_eqs));
     const void *fn64 = opt_fn(false, opt_nt, S);
@@ -374,26 +427,24 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     const int cap32 = o.precision == MPQP_MIXED ? capacity(fn32, lds32) : 0;
     // counters: [0] FP32 work, [1] fallback count, [2] FP64 work
     DBuf dctr(sizeof(int) * 4);
-    DBuf dcert;
-    if (o.precision == MPQP_MIXED) dcert.alloc(sizeof(double) * 6 * (size_t)(M2 + 8) * cap32);
     auto launch = [&](const void *fn, int grid, long long lds, OptArgs &oa) {
 #ifdef MPQP_PROFILE
         // diagnostic builds: per-phase cycle totals of this launch on stderr
-        DBuf dprof(sizeof(long long) * 8 * (size_t)grid);
+        DBuf dprof(sizeof(long long) * 16 * (size_t)grid);
         HIP_OK(hipMemsetAsync(dprof.p, 0, dprof.n, st));
         oa.prof = dprof.as<long long>();
 #endif
         void *args[] = {&oa};
         HIP_OK(hipLaunchKernel(fn, dim3(grid), dim3(opt_nt), args, (size_t)lds, st));
 #ifdef MPQP_PROFILE
-        std::vector<long long> hp(8 * (size_t)grid);
+        std::vector<long long> hp(16 * (size_t)grid);
         HIP_OK(hipMemcpyAsync(hp.data(), dprof.p, dprof.n, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        long long tot[8] = {0};
+        long long tot[16] = {0};
         for (size_t b = 0; b < (size_t)grid; ++b)
-            for (int k = 0; k < 8; ++k) tot[k] += hp[b * 8 + k];
+            for (int k = 0; k < 16; ++k) tot[k] += hp[b * 16 + k];
         std::fprintf(stderr, "prof grid=%d fp32=%d:", grid, fn == fn32 ? 1 : 0);
-        for (int k = 0; k < 8; ++k) std::fprintf(stderr, " %lld", tot[k]);
+        for (int k = 0; k < 16; ++k) std::fprintf(stderr, " %lld", tot[k]);
         std::fprintf(stderr, "\n");
         oa.prof = nullptr;
 #endif
@@ -543,7 +594,6 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     oa.counter = dctr.as<int>();
                     oa.fb_list = dfbl.as<int>();
                     oa.fb_count = dctr.as<int>() + 1;
-                    oa.cert = dcert.as<double>();
                     launch(fn32, std::min(nf, cap32), lds32, oa);
                     // FP64 re-solve from the cold start of what failed certification
                     OptArgs ob = oa;
@@ -554,7 +604,6 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     ob.store32 = 1;
                     ob.fb_list = nullptr;
                     ob.fb_count = nullptr;
-                    ob.cert = nullptr;
                     HIP_OK(hipEventRecord(e3, st));
                     launch(fn64, std::min(nf, cap64), lds64, ob);
                 } else if (o.batching == MPQP_PERSISTENT) {

@@ -12,20 +12,32 @@
 // Replaces dual_simplex_gpu_optimality (batched_simplex.cu:1260-1721).
 #pragma once
 
+// Leading dimension of the LDS tableau: K+2 columns [slots | rhs | B^-1
+// delta] padded to an odd count, so that a column walk (pivot column,
+// rhs, e-columns) by 64 lanes touches 64 distinct 4-byte banks (FP32) or 32
+// (FP64) instead of a few.
+__host__ __device__ inline int opt_ld(int K) { return (K + 2) | 1; }
+
+// LDS of one workgroup: T (M+1) x ld, colq (M+1), vcol (M) in Real; basis,
+// nonb, where, bcnt, nbcnt, rows in int; for certification (FP32 pivots)
+// x and r in double.
 template <typename Real>
-__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2) {
-    const int ld = K + 2;
-    long long b = (long long)(M + 1) * ld * (long long)sizeof(Real) + (long long)(M + 1) * (long long)sizeof(Real) +
-                  (long long)M * (long long)sizeof(Real) + (long long)(M + K) * 4 + (long long)N2 * 4 * 3 +
-                  (long long)(M + 1) * 4;
+__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert) {
+    const int ld = opt_ld(K);
+    long long b = (long long)((M + 1) * ld + (M + 1) + M) * (long long)sizeof(Real) +
+                  (long long)(M + K + 3 * N2 + M + 1) * 4;
+    if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
     return (b + 15) & ~15LL;
 }
 
-// Stored phase-1 state: T (M x (K+2) Real, contiguous) then basis (M ints)
-// then nonbasic (K ints), padded to 256 bytes.
+// Stored phase-1 state: T (M rows of ld Real, as in LDS, padded to a
+// multiple of 4 elements so that copies move 16-byte vectors) then basis (M
+// ints) then nonbasic (K ints); states are 256-byte aligned.
+__host__ __device__ inline int opt_tw(int M, int K) { return (M * opt_ld(K) + 3) & ~3; }
+
 template <typename Real>
 __host__ __device__ inline long long opt_state_bytes(int M, int K) {
-    long long b = (long long)M * (K + 2) * (long long)sizeof(Real) + (long long)(M + K) * 4;
+    long long b = (long long)opt_tw(M, K) * (long long)sizeof(Real) + (long long)(M + K) * 4;
     return (b + 255) & ~255LL;
 }
 
@@ -52,16 +64,27 @@ struct OptShared {
     do {                    \
     } while (0)
 #endif
+// finer marks inside the phase-1 decision (-DMPQP_PROFILE_DEC, slots 8-11)
+#ifdef MPQP_PROFILE_DEC
+#define OPT_PROF_DEC(shv, slot) OPT_PROF(shv, slot)
+#else
+#define OPT_PROF_DEC(shv, slot) \
+    do {                        \
+    } while (0)
+#endif
 
 enum { ACT_PIVOT = 0, ACT_STOP = 1, ACT_UNPERT = 2, ACT_UNBOUNDED = 3 };
 
 template <typename Real>
 struct OptLds {
+    int ld;
     Real *T, *colq, *vcol;
     int *basis, *nonb, *where, *bcnt, *nbcnt, *rows;
-    __device__ void carve(unsigned char *smem, int M, int K, int N2) {
+    double *cx, *cr;  // certification vectors (FP32 pivots only)
+    __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert) {
+        ld = opt_ld(K);
         T = reinterpret_cast<Real *>(smem);
-        colq = T + (M + 1) * (K + 2);
+        colq = T + (M + 1) * ld;
         vcol = colq + (M + 1);
         basis = reinterpret_cast<int *>(vcol + M);
         nonb = basis + M;
@@ -69,6 +92,12 @@ struct OptLds {
         bcnt = where + N2;
         nbcnt = bcnt + N2;
         rows = nbcnt + N2;
+        cx = cr = nullptr;
+        if (cert) {
+            const size_t off = ((size_t)(reinterpret_cast<unsigned char *>(rows + M + 1) - smem) + 7) & ~(size_t)7;
+            cx = reinterpret_cast<double *>(smem + off);
+            cr = cx + M;
+        }
     }
 };
 
@@ -76,25 +105,41 @@ struct OptLds {
 // start: load a state (same precision store, or the FP64 root), lambda
 // swaps; returns 1 if the start basis is nonsingular.
 // ---------------------------------------------------------------------
+// Loads T from a stored state (same element type, rows of L.ld: a flat
+// 16-byte-vector copy) or from the FP64 root (rows of K+2, rounded to
+// Real), then basis, nonbasic and the position map where[].
 template <typename Real, int NT, typename Src>
-__device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, const Src *sT, const int *sb,
-                                         const int *sn) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, ne = M * (K + 2);
+__device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, const Src *sT, int src_ld,
+                                         const int *sb, const int *sn) {
+    const int M = P.M2, K = P.K2, N2 = P.N2;
     const int tid = threadIdx.x;
-    if constexpr (sizeof(Src) == sizeof(Real)) {
-        if ((ne & 1) == 0 && sizeof(Real) == 8) {
-            const double2 *s2 = reinterpret_cast<const double2 *>(sT);
-            double2 *d2 = reinterpret_cast<double2 *>(L.T);
-            for (int e = tid; e < ne / 2; e += NT) d2[e] = s2[e];
-        } else if ((ne & 3) == 0 && sizeof(Real) == 4) {
-            const float4 *s4 = reinterpret_cast<const float4 *>(sT);
-            float4 *d4 = reinterpret_cast<float4 *>(L.T);
-            for (int e = tid; e < ne / 4; e += NT) d4[e] = s4[e];
-        } else {
-            for (int e = tid; e < ne; e += NT) L.T[e] = (Real)sT[e];
+    constexpr int V = 16 / sizeof(Real);
+    if (sizeof(Src) == sizeof(Real) && src_ld == L.ld) {
+        // opt_tw(M, K) elements: at most 3 beyond T's M rows, which land in
+        // the cost row (row M, rebuilt before use)
+        const int nv = opt_tw(M, K) / V;
+        const int4 *s4 = reinterpret_cast<const int4 *>(sT);
+        int4 *d4 = reinterpret_cast<int4 *>(L.T);
+        constexpr int UL = 8;  // loads in flight per thread
+        for (int e0 = tid; e0 < nv; e0 += NT * UL) {
+            int4 vv[UL];
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                const int e = e0 + u * NT;
+                if (e < nv) vv[u] = s4[e];
+            }
+#pragma unroll
+            for (int u = 0; u < UL; ++u) {
+                const int e = e0 + u * NT;
+                if (e < nv) d4[e] = vv[u];
+            }
         }
     } else {
-        for (int e = tid; e < ne; e += NT) L.T[e] = (Real)sT[e];
+        const int sw = K + 2, ne = M * sw;
+        for (int e = tid; e < ne; e += NT) {
+            const int i = e / sw, j = e - i * sw;
+            L.T[i * L.ld + j] = (Real)sT[i * src_ld + j];
+        }
     }
     for (int i = tid; i < M; i += NT) L.basis[i] = sb[i];
     for (int kk = tid; kk < K; kk += NT) L.nonb[kk] = sn[kk];
@@ -111,7 +156,7 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
 template <typename Real, int NT>
 __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, const DevTol &tl, const int *Sset,
                                           int j0, int j1) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, ld = K + 2;
+    const int M = P.M2, K = P.K2, N2 = P.N2, ld = L.ld;
     const int r4 = M - P.n_c - P.n_x;
     const int tid = threadIdx.x;
     for (int jj = j0; jj < j1; ++jj) {
@@ -192,8 +237,8 @@ struct P1Result {
 };
 
 template <typename Real, int NT, int S>
-__device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = K + 2;
+__device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = L.ld;
     const int RHS = K, PERT = K + 1;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
     Real *T = L.T;
@@ -230,6 +275,7 @@ __device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts 
             }
             tree_argmin_s<S>(v, id, cn, M, lane, DminPred{(n_same % 20) >= 10 ? 1 : 0, tl.dual_stop_tol});
             const int r = rfl(id[0]);
+            OPT_PROF_DEC(sh, 8);
             if (lane == 0) L.bcnt[L.basis[r]] += 1;
             const bool go_on = __any(neg);
             int action = ACT_PIVOT, q = -1, stall = 0;
@@ -261,6 +307,7 @@ __device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts 
                     const Real rl = -(Real)tl.pivot_rel * amax;
                     dlim = rl < dlim ? rl : dlim;
                 }
+                OPT_PROF_DEC(sh, 9);
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int kk = lane + MPQP_WAVE * s;
@@ -279,6 +326,7 @@ __device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts 
                 }
                 tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
                 const int wv = rfl(id[0]);
+                OPT_PROF_DEC(sh, 10);
                 if (wv < 0) {
                     action = ACT_UNBOUNDED;
                 } else {
@@ -290,6 +338,7 @@ __device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts 
                         ps[s] = (i < M && xv[s] < (Real)0) ? xv[s] : (Real)0;
                     }
                     const double pinf = (double)tree_sum_s<S, Real>(ps, M, lane);
+                    OPT_PROF_DEC(sh, 11);
                     if (pinf > best_p) {
                         best_p = pinf;
                         n_same = 0;
@@ -380,8 +429,8 @@ __device__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts 
 // (ascending positions; entering slot = largest |T[i][k]| > tol_plus among
 // non-artificial slots, lowest slot on ties).  Returns 1 if none stays.
 template <typename Real, int NT, int S>
-__device__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, Nn = P.Nn, ld = K + 2;
+__device__ __forceinline__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, Nn = P.Nn, ld = L.ld;
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
     const int n = sh.nrows;
@@ -421,8 +470,8 @@ struct P2Result {
 };
 
 template <typename Real, int NT, int S>
-__device__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = K + 2, RHS = K;
+__device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = L.ld, RHS = K;
     const int EPS = P.n_x + P.n_t + P.n_c;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
     Real *T = L.T;
@@ -499,178 +548,242 @@ __device__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P, const DevOpts 
 
 // ---------------------------------------------------------------------
 // FP64 certification of an FP32 basis (mixed precision)
+//
+// The basis B of the FP32 tableau is checked against A2 in FP64: x solves
+// B x = xb2 and y solves B^T y = cb by iterative refinement, using the FP32
+// B^-1 held in the tableau's e-columns as the preconditioner; the certified
+// x, y give the verdict quantities.  A2 is read through its CSR (rows, for
+// B x) and CSC (columns, for B^T y and reduced costs) copies; the S-mask of
+// the w columns (initialise_A_kernel_1, batched_simplex.cu:635-650) is
+// applied on the fly.  Vectors x, r live in LDS.
 // ---------------------------------------------------------------------
 #define MPQP_CERT_STEPS 3
 
-// A2 entry of variable v at row rho under the S-mask of the w columns
-// (initialise_A_kernel_1, batched_simplex.cu:635-650)
-__device__ __forceinline__ double a2m(const DevProb &P, const unsigned *inS, int rho, int v) {
-    const int w0 = P.n_x + P.n_t;
-    if (v >= w0 && v < w0 + P.n_c) {
-        const int j = v - w0;
-        if ((inS[j >> 5] >> (j & 31)) & 1u) {
-            if (rho == P.M2 - P.n_c + j) return 0.0;
-        } else {
-            const int r4 = P.M2 - P.n_c - P.n_x;
-            if (rho >= r4 && rho < P.M2 - P.n_c) return 0.0;
-        }
-    }
-    return P.A2[(long long)v * P.M2 + rho];
+// true if entry (rho, v) of A2 is masked to zero for the current S
+__device__ __forceinline__ bool w_masked(const DevProb &P, const unsigned *inS, int rho, int v) {
+    const int j = v - (P.n_x + P.n_t);
+    if ((unsigned)j >= (unsigned)P.n_c) return false;
+    if ((inS[j >> 5] >> (j & 31)) & 1u) return rho == P.M2 - P.n_c + j;
+    return rho >= P.M2 - P.n_c - P.n_x && rho < P.M2 - P.n_c;
 }
 
-struct CertBuf {
-    double *x, *r, *d, *y, *cb, *red;  // M each (red: M for the block max)
-};
-
 template <int NT>
-__device__ __forceinline__ double block_max(double v, double *red, OptShared &sh) {
-    // wave max then across waves through LDS-free global scratch red[0..NW)
-    v = wave_max(v);
-    const int lane = threadIdx.x & (MPQP_WAVE - 1), wave = threadIdx.x / MPQP_WAVE;
+__device__ __forceinline__ double block_max(double v, double *red) {
     constexpr int NW = NT / MPQP_WAVE;
-    if (lane == 0) red[wave] = v;
+    v = wave_max(v);
+    if ((threadIdx.x & (MPQP_WAVE - 1)) == 0) red[threadIdx.x / MPQP_WAVE] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int w = 1; w < NW; ++w) m = red[w] > m ? red[w] : m;
-        sh.z = m;
-    }
-    __syncthreads();
-    const double r = sh.z;
-    __syncthreads();  // read by all before the next writer of sh.z
-    return r;
+    double m = red[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = red[w] > m ? red[w] : m;
+    __syncthreads();  // red is free again
+    return m;
 }
 
-// dx = B32^-1 r (B^-1 columns = e-columns of the FP32 tableau)
-template <int NT>
-__device__ __forceinline__ void binv_apply(const OptLds<float> &L, const DevProb &P, const double *r, double *dx) {
-    const int M = P.M2, ld = P.K2 + 2;
-    for (int i = threadIdx.x; i < M; i += NT) {
-        double acc = 0.0;
-        for (int rho = 0; rho < M; ++rho) {
-            const double rv = r[rho];
-            if (rv == 0.0) continue;
-            const int w = L.where[P.N2 - M + rho];
-            if (w >= 0) {
-                if (i == w) acc = fma(rv, 1.0, acc);
-            } else {
-                acc = fma(rv, (double)L.T[i * ld + (-w - 1)], acc);
+// Every product below is a sequential fma chain in a fixed order, one
+// thread per output entry: A2 through its CSR rows (variables ascending)
+// and CSC columns (rows ascending), masked entries and nonbasic variables
+// skipped; B^-1 through the e-columns of the FP32 tableau.  Sparse walks
+// load MPQP_CB entries at a time so that their latencies overlap.
+#define MPQP_CB 4
+
+// Compact list of the nonbasic e-variables, rho ascending: elist_rho[k],
+// elist_slot[k] (in the phase-1 use counters, free during certification);
+// count in sh.best.
+template <int S>
+__device__ __forceinline__ void elist_build(OptLds<float> &L, const DevProb &P, OptShared &sh) {
+    const int M = P.M2, e0 = P.N2 - M;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    if (threadIdx.x < MPQP_WAVE) {
+        int base = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int rho = lane + MPQP_WAVE * s;
+            const int w = rho < M ? L.where[e0 + rho] : 0;
+            const bool f = w < 0;
+            const unsigned long long m = __ballot(f);
+            if (f) {
+                const int k = base + __popcll(m & ((1ull << lane) - 1ull));
+                L.bcnt[k] = rho;
+                L.nbcnt[k] = -w - 1;
             }
+            base += __popcll(m);
         }
-        dx[i] = acc;
+        if (lane == 0) sh.best = base;
     }
     __syncthreads();
 }
 
-// out = B32^-T v (v by position, out by row)
-template <int NT>
-__device__ __forceinline__ void binvT_apply(const OptLds<float> &L, const DevProb &P, const double *v, double *out) {
-    const int M = P.M2, ld = P.K2 + 2;
-    for (int rho = threadIdx.x; rho < M; rho += NT) {
-        const int w = L.where[P.N2 - M + rho];
-        double acc = 0.0;
-        if (w >= 0) {
-            if (v[w] != 0.0) acc = fma(v[w], 1.0, acc);
-        } else {
-            const int kk = -w - 1;
-            for (int i = 0; i < M; ++i)
-                if (v[i] != 0.0) acc = fma(v[i], (double)L.T[i * ld + kk], acc);
+// c - sum over column v of A2 (rows ascending, masked skipped) of a * y[row]
+__device__ __forceinline__ double col_resid(const DevProb &P, const unsigned *inS, int v, double c, const double *y) {
+    double acc = c;
+    const int n = P.csc_n[v], N2 = P.N2;
+    for (int t0 = 0; t0 < n; t0 += MPQP_CB) {
+        int rr[MPQP_CB];
+        double av[MPQP_CB], yv[MPQP_CB];
+#pragma unroll
+        for (int u = 0; u < MPQP_CB; ++u) {
+            const int t = t0 + u;
+            rr[u] = t < n ? P.csc_row[t * N2 + v] : -1;
+            av[u] = t < n ? P.csc_val[t * N2 + v] : 0.0;
         }
-        out[rho] = acc;
+#pragma unroll
+        for (int u = 0; u < MPQP_CB; ++u) yv[u] = rr[u] >= 0 ? y[rr[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < MPQP_CB; ++u)
+            if (rr[u] >= 0 && !w_masked(P, inS, rr[u], v)) acc = fma(-av[u], yv[u], acc);
     }
-    __syncthreads();
+    return acc;
 }
 
-template <int NT>
-__device__ int cert_primal(const OptLds<float> &L, const DevProb &P, const unsigned *inS, CertBuf &B, OptShared &sh) {
-    const int M = P.M2, ld = P.K2 + 2, RHS = P.K2;
-    double bm = 0.0;
-    for (int rho = threadIdx.x; rho < M; rho += NT) bm = fmax(bm, fabs(P.xb2[rho]));
-    const double bmax = block_max<NT>(bm, B.red, sh);
-    const double tol = 1e-12 * (1.0 + bmax);
-    for (int i = threadIdx.x; i < M; i += NT) B.x[i] = (double)L.T[i * ld + RHS];
+// x (LDS) := refined solution of B x = xb2; 1 once max|xb2 - B x| <= cert_ptol
+template <int NT, int S>
+__device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
+                                           OptShared &sh) {
+    const int M = P.M2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
+    double *x = L.cx, *r = L.cr;
+    elist_build<S>(L, P, sh);
+    const int ne = sh.best;
+    for (int i = threadIdx.x; i < M; i += NT) x[i] = (double)L.T[i * ld + RHS];
     __syncthreads();
     for (int it = 0;; ++it) {
         double rn = 0.0;
         for (int rho = threadIdx.x; rho < M; rho += NT) {
+            // (B x)_rho: row rho of A2, variables ascending, basic only
             double acc = P.xb2[rho];
-            for (int i = 0; i < M; ++i) {
-                const double a = a2m(P, inS, rho, L.basis[i]);
-                if (a != 0.0) acc = fma(-B.x[i], a, acc);
+            const int n = P.csr_n[rho];
+            for (int t0 = 0; t0 < n; t0 += MPQP_CB) {
+                int vv[MPQP_CB], ww[MPQP_CB];
+                double av[MPQP_CB], xv[MPQP_CB];
+#pragma unroll
+                for (int u = 0; u < MPQP_CB; ++u) {
+                    const int t = t0 + u;
+                    vv[u] = t < n ? P.csr_var[t * M + rho] : -1;
+                    av[u] = t < n ? P.csr_val[t * M + rho] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < MPQP_CB; ++u) ww[u] = vv[u] >= 0 ? L.where[vv[u]] : -1;
+#pragma unroll
+                for (int u = 0; u < MPQP_CB; ++u) xv[u] = ww[u] >= 0 ? x[ww[u]] : 0.0;
+#pragma unroll
+                for (int u = 0; u < MPQP_CB; ++u)
+                    if (ww[u] >= 0 && !w_masked(P, inS, rho, vv[u])) acc = fma(-xv[u], av[u], acc);
             }
-            B.r[rho] = acc;
+            r[rho] = acc;
             rn = fmax(rn, fabs(acc));
         }
-        const double rmax = block_max<NT>(rn, B.red, sh);
-        if (rmax <= tol) return 1;
+        const double rmax = block_max<NT>(rn, red);
+        OPT_PROF(sh, 8);
+        if (rmax <= P.cert_ptol) return 1;
         if (it == MPQP_CERT_STEPS) return 0;
-        binv_apply<NT>(L, P, B.r, B.d);
-        for (int i = threadIdx.x; i < M; i += NT) B.x[i] = B.x[i] + B.d[i];
+        for (int i = threadIdx.x; i < M; i += NT) {
+            // (B32^-1 r)_i, rho ascending: the unit term of a basic e at
+            // position i slotted in among the nonbasic e-columns
+            const float *Ti = L.T + i * ld;
+            const int bv = L.basis[i];
+            const int ru = bv >= e0 ? bv - e0 : M;
+            bool done = ru >= M;
+            double acc = 0.0;
+            for (int k = 0; k < ne; ++k) {
+                const int rk = L.bcnt[k];
+                const double tv = (double)Ti[L.nbcnt[k]];
+                if (!done && ru < rk) {
+                    acc = fma(r[ru], 1.0, acc);
+                    done = true;
+                }
+                acc = fma(r[rk], tv, acc);
+            }
+            if (!done) acc = fma(r[ru], 1.0, acc);
+            x[i] = x[i] + acc;
+        }
         __syncthreads();
+        OPT_PROF(sh, 9);
     }
 }
 
-template <int NT>
-__device__ int cert_dual(const OptLds<float> &L, const DevProb &P, const unsigned *inS, CertBuf &B, OptShared &sh) {
-    const int M = P.M2;
-    binvT_apply<NT>(L, P, B.cb, B.y);
+// y (in L.cx) := refined solution of B^T y = c_B, c = cost(v), starting from
+// the FP32 cost row (y_rho = c(e_rho) - d(e_rho), d = 0 for a basic e: the
+// e-columns of A2 are the identity); 1 once max|c_B - B^T y| <= 2e-12
+template <int NT, typename CostF>
+__device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
+                                         CostF cost, OptShared &sh) {
+    const int M = P.M2, ld = L.ld, e0 = P.N2 - M;
+    double *y = L.cx, *r = L.cr;
+    for (int rho = threadIdx.x; rho < M; rho += NT) {
+        const int w = L.where[e0 + rho];
+        const double c = cost(e0 + rho);
+        y[rho] = w >= 0 ? c : c - (double)L.T[M * ld + (-w - 1)];
+    }
+    __syncthreads();
+    OPT_PROF(sh, 11);
     for (int it = 0;; ++it) {
         double rn = 0.0;
         for (int i = threadIdx.x; i < M; i += NT) {
-            double acc = B.cb[i];
             const int bv = L.basis[i];
-            for (int rho = 0; rho < M; ++rho) {
-                const double a = a2m(P, inS, rho, bv);
-                if (a != 0.0) acc = fma(-a, B.y[rho], acc);
-            }
-            B.r[i] = acc;
+            const double acc = col_resid(P, inS, bv, cost(bv), y);
+            r[i] = acc;
             rn = fmax(rn, fabs(acc));
         }
-        const double rmax = block_max<NT>(rn, B.red, sh);
+        const double rmax = block_max<NT>(rn, red);
+        OPT_PROF(sh, 12);
         if (rmax <= 2e-12) return 1;
         if (it == MPQP_CERT_STEPS) return 0;
-        binvT_apply<NT>(L, P, B.r, B.d);
-        for (int rho = threadIdx.x; rho < M; rho += NT) B.y[rho] = B.y[rho] + B.d[rho];
+        for (int rho = threadIdx.x; rho < M; rho += NT) {
+            // (B32^-T r)_rho over positions ascending
+            const int w = L.where[e0 + rho];
+            double acc = 0.0;
+            if (w >= 0) {
+                acc = fma(r[w], 1.0, 0.0);
+            } else {
+                const float *Tk = L.T + (-w - 1);
+                for (int i = 0; i < M; ++i) acc = fma(r[i], (double)Tk[i * ld], acc);
+            }
+            y[rho] = y[rho] + acc;
+        }
         __syncthreads();
+        OPT_PROF(sh, 13);
     }
 }
 
 // min over the selected nonbasic slots of the FP64 reduced cost c_v - a_v^T y
-template <int NT, typename Sel, typename Cost>
-__device__ double min_red_cost(const OptLds<float> &L, const DevProb &P, const unsigned *inS, CertBuf &B,
-                               OptShared &sh, Sel sel, Cost cost) {
+template <int NT, typename Sel, typename CostF>
+__device__ __forceinline__ double min_red_cost(const OptLds<float> &L, const DevProb &P, const unsigned *inS,
+                                               double *red, Sel sel, CostF cost) {
     double mn = INFINITY;
     for (int kk = threadIdx.x; kk < P.K2; kk += NT) {
         const int v = L.nonb[kk];
-        if (!sel(v)) continue;
-        double acc = cost(v);
-        for (int rho = 0; rho < P.M2; ++rho) {
-            const double a = a2m(P, inS, rho, v);
-            if (a != 0.0) acc = fma(-a, B.y[rho], acc);
-        }
-        mn = fmin(mn, acc);
+        if (sel(v)) mn = fmin(mn, col_resid(P, inS, v, cost(v), L.cx));
     }
-    return -block_max<NT>(-mn, B.red, sh);
+    return -block_max<NT>(-mn, red);
 }
 
-// FP64 tree sum of cb[i] * x[i] in the reference order (wave 0)
-template <int S>
-__device__ double cert_tree_z(const DevProb &P, CertBuf &B, OptShared &sh) {
+// FP64 tree sum of cb(basis[i]) * x[i] in the reference order (wave 0),
+// broadcast to the workgroup through red[0]
+template <int NT, int S, typename CbF>
+__device__ __forceinline__ double cert_z(const OptLds<float> &L, const DevProb &P, double *red, CbF cbf) {
     const int M = P.M2, lane = threadIdx.x & (MPQP_WAVE - 1);
     if (threadIdx.x < MPQP_WAVE) {
         double zs[S];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int i = lane + MPQP_WAVE * s;
-            zs[s] = i < M ? B.cb[i] * B.x[i] : 0.0;
+            zs[s] = i < M ? cbf(L.basis[i]) * L.cx[i] : 0.0;
         }
         const double z = tree_sum_s<S, double>(zs, M, lane);
-        if (lane == 0) sh.z = z;
+        if (lane == 0) red[0] = z;
     }
     __syncthreads();
-    const double r = sh.z;
+    const double z = red[0];
     __syncthreads();
-    return r;
+    return z;
+}
+
+// minimum of x over the workgroup
+template <int NT>
+__device__ __forceinline__ double cert_xmin(const OptLds<float> &L, int M, double *red) {
+    double mn = INFINITY;
+    for (int i = threadIdx.x; i < M; i += NT) mn = fmin(mn, L.cx[i]);
+    return -block_max<NT>(-mn, red);
 }
 
 // ---------------------------------------------------------------------
@@ -681,20 +794,17 @@ __device__ double cert_tree_z(const DevProb &P, CertBuf &B, OptShared &sh) {
 // level, in the store's element type.
 template <typename Real, typename Dst, int NT>
 __device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, unsigned char *dst) {
-    const int ne = M * (K + 2);
+    const int tw = opt_tw(M, K);
     Dst *dT = reinterpret_cast<Dst *>(dst);
-    if constexpr (sizeof(Dst) == sizeof(Real) && sizeof(Real) == 8) {
-        if ((ne & 1) == 0) {
-            double2 *d2 = reinterpret_cast<double2 *>(dT);
-            const double2 *s2 = reinterpret_cast<const double2 *>(L.T);
-            for (int e = threadIdx.x; e < ne / 2; e += NT) d2[e] = s2[e];
-        } else {
-            for (int e = threadIdx.x; e < ne; e += NT) dT[e] = (Dst)L.T[e];
-        }
+    constexpr int V = 16 / sizeof(Real);
+    if (sizeof(Dst) == sizeof(Real)) {
+        int4 *d4 = reinterpret_cast<int4 *>(dT);
+        const int4 *s4 = reinterpret_cast<const int4 *>(L.T);
+        for (int e = threadIdx.x; e < tw / V; e += NT) d4[e] = s4[e];
     } else {
-        for (int e = threadIdx.x; e < ne; e += NT) dT[e] = (Dst)L.T[e];
+        for (int e = threadIdx.x; e < tw; e += NT) dT[e] = (Dst)L.T[e];
     }
-    int *db = reinterpret_cast<int *>(dT + (long long)ne);
+    int *db = reinterpret_cast<int *>(dT + (long long)tw);
     for (int i = threadIdx.x; i < M; i += NT) db[i] = L.basis[i];
     for (int kk = threadIdx.x; kk < K; kk += NT) db[M + kk] = L.nonb[kk];
 }
@@ -705,24 +815,21 @@ __device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, u
 // that cannot be certified is appended to a.fb_list for the FP64 re-solve
 // (launched with force_cold and store32).
 template <typename Real, int NT, int S, bool MIXED>
-__global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
+__device__ __forceinline__ void opt_body(const OptArgs &a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ OptShared sh;
     __shared__ unsigned inS[8];
     const DevProb &P = a.P;
     const DevOpts &O = a.O;
-    const int M = P.M2, K = P.K2, N2 = P.N2, ld = K + 2;
+    __shared__ double red[NT / MPQP_WAVE];
+    const int M = P.M2, K = P.K2, N2 = P.N2;
     const int tid = threadIdx.x;
     const int k = a.level;
     const int n = a.n_dev ? *a.n_dev : a.n;
     OptLds<Real> L;
-    L.carve(smem, M, K, N2);
+    L.carve(smem, M, K, N2, MIXED);
+    const int ld = L.ld;
     const DevTol tl = MIXED ? dev_tol32(O) : dev_tol64(O);
-    CertBuf B{};
-    if constexpr (MIXED) {
-        double *cb = a.cert + (long long)blockIdx.x * 6 * (M + 8);
-        B = CertBuf{cb, cb + (M + 8), cb + 2 * (M + 8), cb + 3 * (M + 8), cb + 4 * (M + 8), cb + 5 * (M + 8)};
-    }
     for (int iter = 0;; ++iter) {
         int item;
         if (a.counter) {
@@ -739,7 +846,7 @@ __global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
         DevRecord &R = a.rec[c];
 #ifdef MPQP_PROFILE
         if (tid == 0) {
-            sh.prof = a.prof ? a.prof + (long long)blockIdx.x * 8 : nullptr;
+            sh.prof = a.prof ? a.prof + (long long)blockIdx.x * 16 : nullptr;
             sh.t0 = (long long)__builtin_amdgcn_s_memtime();
         }
 #endif
@@ -756,13 +863,13 @@ __global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
         int warm = 0, started = 0;
         if (!a.force_cold && O.warm && k > 0 && a.p1parent[c] >= 0) {
             const Real *sT = reinterpret_cast<const Real *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride);
-            const int *sb = reinterpret_cast<const int *>(sT + (long long)M * ld);
-            opt_load<Real, NT, Real>(L, P, sT, sb, sb + M);
+            const int *sb = reinterpret_cast<const int *>(sT + (long long)opt_tw(M, K));
+            opt_load<Real, NT, Real>(L, P, sT, ld, sb, sb + M);
             started = warm = opt_lambda<Real, NT>(L, P, tl, Sset, k - 1, k);
             __syncthreads();
         }
         if (!started) {
-            opt_load<Real, NT, double>(L, P, P.root, P.root_basis, P.root_nonb);
+            opt_load<Real, NT, double>(L, P, P.root, K + 2, P.root_basis, P.root_nonb);
             started = opt_lambda<Real, NT>(L, P, tl, Sset, 0, k);
         }
         OPT_PROF(sh, 0);
@@ -782,26 +889,22 @@ __global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
             if constexpr (MIXED) {
                 // FP64 certification of phase 1: x_B >= dual_stop_tol; if an
                 // artificial is basic, nonnegative reduced phase-1 costs
+                const int Nn = P.Nn;
+                auto cb1 = [Nn](int v) { return v >= Nn ? 1.0 : 0.0; };
                 int ok = p1.status == ST_OPTIMAL;
-                if (ok) ok = cert_primal<NT>(L, P, inS, B, sh);
+                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh);
+                if (ok) ok = cert_xmin<NT>(L, M, red) >= O.dual_stop_tol;
                 if (ok) {
-                    double mn = INFINITY;
-                    for (int i = tid; i < M; i += NT) mn = fmin(mn, B.x[i]);
-                    ok = -block_max<NT>(-mn, B.red, sh) >= O.dual_stop_tol;
-                }
-                if (ok) {
-                    for (int i = tid; i < M; i += NT) B.cb[i] = L.basis[i] >= P.Nn ? 1.0 : 0.0;
-                    rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= P.Nn; });
-                    if (sh.nrows > 0) {
-                        ok = cert_dual<NT>(L, P, inS, B, sh);
-                        if (ok)
-                            ok = min_red_cost<NT>(L, P, inS, B, sh, [&](int) { return true; },
-                                                  [&](int v) { return v >= P.Nn ? 1.0 : 0.0; }) >= -O.certify_tol;
-                    }
-                }
-                if (ok) {
-                    p1.z1 = cert_tree_z<S>(P, B, sh);
+                    p1.z1 = cert_z<NT, S>(L, P, red, cb1);
                     p1.kkt = (p1.z1 > -O.dual_feas_tol && p1.z1 < O.dual_feas_tol) ? 1 : 0;
+                    rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
+                    OPT_PROF(sh, 10);
+                    if (sh.nrows > 0) {
+                        ok = cert_dual<NT>(L, P, inS, red, cb1, sh);
+                        if (ok)
+                            ok = min_red_cost<NT>(L, P, inS, red, [](int) { return true; }, cb1) >= -O.certify_tol;
+                        OPT_PROF(sh, 14);
+                    }
                 }
                 fail = !ok;
             }
@@ -824,26 +927,18 @@ __global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
                 if constexpr (MIXED) {
                     // FP64 certification of phase 2: x_B >= -certify_tol and
                     // nonnegative reduced costs of the first N_ variables
-                    const int EPS = P.n_x + P.n_t + P.n_c;
+                    const int EPS = P.n_x + P.n_t + P.n_c, Nn = P.Nn;
+                    auto cb2 = [EPS](int v) { return v == EPS ? -1.0 : 0.0; };
                     int ok = p2.status == ST_OPTIMAL;
-                    if (ok) ok = cert_primal<NT>(L, P, inS, B, sh);
+                    if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh);
+                    if (ok) ok = cert_xmin<NT>(L, M, red) >= -O.certify_tol;
                     if (ok) {
-                        double mn = INFINITY;
-                        for (int i = tid; i < M; i += NT) mn = fmin(mn, B.x[i]);
-                        ok = -block_max<NT>(-mn, B.red, sh) >= -O.certify_tol;
-                    }
-                    if (ok) {
-                        for (int i = tid; i < M; i += NT) B.cb[i] = L.basis[i] == EPS ? -1.0 : 0.0;
-                        __syncthreads();
-                        ok = cert_dual<NT>(L, P, inS, B, sh);
+                        p2.z2 = cert_z<NT, S>(L, P, red, cb2);
+                        p2.final_set = -p2.z2 > O.final_tol ? 1 : 0;
+                        ok = cert_dual<NT>(L, P, inS, red, cb2, sh);
                     }
                     if (ok)
-                        ok = min_red_cost<NT>(L, P, inS, B, sh, [&](int v) { return v < P.Nn; },
-                                              [&](int v) { return v == EPS ? -1.0 : 0.0; }) >= -O.certify_tol;
-                    if (ok) {
-                        p2.z2 = cert_tree_z<S>(P, B, sh);
-                        p2.final_set = -p2.z2 > O.final_tol ? 1 : 0;
-                    }
+                        ok = min_red_cost<NT>(L, P, inS, red, [Nn](int v) { return v < Nn; }, cb2) >= -O.certify_tol;
                     fail = !ok;
                 }
                 if (!fail && a.b2)
@@ -874,4 +969,16 @@ __global__ __launch_bounds__(NT) void opt_kernel_t(OptArgs a) {
         }
         OPT_PROF(sh, 7);
     }
+}
+
+template <int NT, int S>
+__global__ __launch_bounds__(NT) void opt_kernel64(OptArgs a) {
+    opt_body<double, NT, S, false>(a);
+}
+
+// FP32 pivots: capped at 128 VGPRs so that two 512-thread workgroups (two
+// LPs, 2 x ~80 KB of LDS at n=30 m=62 p=8) share a CU.
+template <int NT, int S>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void opt_kernel32(OptArgs a) {
+    opt_body<float, NT, S, true>(a);
 }

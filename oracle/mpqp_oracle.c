@@ -231,6 +231,10 @@ typedef struct {
     double *root;
     int *root_basis, *root_nonb;
     int root_ok;
+    /* structural nonzeros of A2 by column (rows ascending) and by row
+     * (variables ascending), for the certification sums */
+    int *csc_ptr, *csc_row, *csr_ptr, *csr_var;
+    double *csc_val, *csr_val;
 } Prob;
 
 /* var ids of the optimality LP */
@@ -527,60 +531,67 @@ void oracle_cert_stats(long long *out, int reset) {
     for (int i = 0; i < 16; i++) { out[i] = g_cert_fail[i]; if (reset) g_cert_fail[i] = 0; }
 }
 
-/* A2 entry of variable v at row rho under the S-mask of the w columns
- * (initialise_A_kernel_1 :635-650). */
-static double a2m(const Prob *P, const char *inS, int rho, int v) {
-    int w0 = P->n_x + P->n_t;
-    if (v >= w0 && v < w0 + P->n_c) {
-        int j = v - w0;
-        if (inS[j]) {
-            if (rho == P->M2 - P->n_c + j) return 0.0;
-        } else {
-            int r4 = P->M2 - P->n_c - P->n_x;
-            if (rho >= r4 && rho < P->M2 - P->n_c) return 0.0;
-        }
-    }
-    return P->A2[(size_t)v * P->M2 + rho];
+/* Every certification product below is a sequential fma chain in a fixed
+ * order (the GPU computes each output entry on one thread, in the same
+ * order).  A2 is read through its CSR rows (variables ascending) and CSC
+ * columns (rows ascending); masked entries and nonbasic variables are
+ * skipped. */
+
+/* true if entry (rho, v) of A2 is masked to zero for S (:635-650) */
+static int w_masked(const Prob *P, const char *inS, int rho, int v) {
+    int j = v - (P->n_x + P->n_t);
+    if (j < 0 || j >= P->n_c) return 0;
+    if (inS[j]) return rho == P->M2 - P->n_c + j;
+    return rho >= P->M2 - P->n_c - P->n_x && rho < P->M2 - P->n_c;
 }
 
-/* dx = B32^-1 r: columns of B^-1 are the e-variable tableau columns (unit
- * vectors for basic e variables), FP32 values accumulated in FP64. */
-static void binv_apply(const Prob *P, const OSt_f *O, const int *where, const double *r, double *dx) {
-    int M = P->M2, ld = P->ld2;
-    for (int i = 0; i < M; i++) {
-        double acc = 0.0;
-        for (int rho = 0; rho < M; rho++) {
-            if (r[rho] == 0.0) continue;
-            int w = where[P->N2 - M + rho];
-            if (w >= 0) {
-                if (i == w) acc = fma(r[rho], 1.0, acc);
-            } else {
-                acc = fma(r[rho], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
-            }
-        }
-        dx[i] = acc;
-    }
+/* cost of variable v in the phase being certified: phase 1 = 1 on the
+ * artificials (v >= N_), phase 2 = -1 on eps */
+static double cert_cost(const Prob *P, int phase, int v) {
+    if (phase == 1) return v >= P->N_ ? 1.0 : 0.0;
+    return v == EPS_VAR(P) ? -1.0 : 0.0;
 }
 
-/* out = B32^-T v (v indexed by basis position, out by row). */
-static void binvT_apply(const Prob *P, const OSt_f *O, const int *where, const double *v, double *out) {
+/* (B32^-1 r)_i over rho ascending: e_rho basic at position i contributes
+ * r[rho] (unit column), e_rho basic elsewhere nothing, e_rho nonbasic
+ * r[rho] * T32[i][slot of e_rho]. */
+static double binv_row(const Prob *P, const OSt_f *O, const int *where, const double *r, int i) {
     int M = P->M2, ld = P->ld2;
+    double acc = 0.0;
     for (int rho = 0; rho < M; rho++) {
         int w = where[P->N2 - M + rho];
-        double acc = 0.0;
         if (w >= 0) {
-            if (v[w] != 0.0) acc = fma(v[w], 1.0, acc);
+            if (w == i) acc = fma(r[rho], 1.0, acc);
         } else {
-            int k = -w - 1;
-            for (int i = 0; i < M; i++)
-                if (v[i] != 0.0) acc = fma(v[i], (double)O->T[(size_t)i * ld + k], acc);
+            acc = fma(r[rho], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
         }
-        out[rho] = acc;
     }
+    return acc;
+}
+
+/* (B32^-T s)_rho over positions i ascending */
+static double binvT_col(const Prob *P, const OSt_f *O, const int *where, const double *s, int rho) {
+    int M = P->M2, ld = P->ld2;
+    int w = where[P->N2 - M + rho];
+    if (w >= 0) return fma(s[w], 1.0, 0.0);
+    double acc = 0.0;
+    for (int i = 0; i < M; i++) acc = fma(s[i], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
+    return acc;
+}
+
+/* c - sum over column v of A2 (rows ascending, masked skipped) of a * y[row] */
+static double col_resid(const Prob *P, const char *inS, int v, double c, const double *y) {
+    double acc = c;
+    for (int e = P->csc_ptr[v]; e < P->csc_ptr[v + 1]; e++) {
+        int rho = P->csc_row[e];
+        if (!w_masked(P, inS, rho, v)) acc = fma(-P->csc_val[e], y[rho], acc);
+    }
+    return acc;
 }
 
 /* x := FP64 solution of B x = xb2 by iterative refinement from the FP32
- * rhs column; returns 1 once max|xb2 - B x| <= 1e-12 (1 + max|xb2|). */
+ * rhs column; returns 1 once max|xb2 - B x| <= 1e-12 (1 + max|xb2|).
+ * (B x)_rho walks row rho of A2 (variables ascending, basic only). */
 static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const char *inS, double *x,
                        double *r, double *dx) {
     int M = P->M2, ld = P->ld2, RHS = P->K2;
@@ -593,9 +604,9 @@ static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const ch
         double rn = 0.0;
         for (int rho = 0; rho < M; rho++) {
             double acc = P->xb2[rho];
-            for (int i = 0; i < M; i++) {
-                double a = a2m(P, inS, rho, O->basis[i]);
-                if (a != 0.0) acc = fma(-x[i], a, acc);
+            for (int e = P->csr_ptr[rho]; e < P->csr_ptr[rho + 1]; e++) {
+                int v = P->csr_var[e], w = where[v];
+                if (w >= 0 && !w_masked(P, inS, rho, v)) acc = fma(-x[w], P->csr_val[e], acc);
             }
             r[rho] = acc;
             if (fabs(acc) > rn) rn = fabs(acc);
@@ -603,43 +614,36 @@ static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const ch
         if (g_trace) fprintf(stderr, "cert_primal it %d rn %.3e tol %.3e\n", it, rn, tol);
         if (rn <= tol) return 1;
         if (it == CERT_STEPS) return 0;
-        binv_apply(P, O, where, r, dx);
+        for (int i = 0; i < M; i++) dx[i] = binv_row(P, O, where, r, i);
         for (int i = 0; i < M; i++) x[i] = x[i] + dx[i];
     }
 }
 
-/* y := FP64 solution of B^T y = cb by iterative refinement; returns 1 once
- * max|cb - B^T y| <= 2e-12. */
-static int cert_dual(const Prob *P, const OSt_f *O, const int *where, const char *inS, const double *cb,
+/* y := FP64 solution of B^T y = c_B by iterative refinement, starting from
+ * the FP32 cost row (y_rho = c(e_rho) - d(e_rho), d = 0 when e_rho is
+ * basic: the e-columns of A2 are the identity); returns 1 once
+ * max|c_B - B^T y| <= 2e-12. */
+static int cert_dual(const Prob *P, const OSt_f *O, const int *where, const char *inS, int phase,
                      double *y, double *r, double *dy) {
-    int M = P->M2;
-    binvT_apply(P, O, where, cb, y);
+    int M = P->M2, ld = P->ld2, e0 = P->N2 - M;
+    for (int rho = 0; rho < M; rho++) {
+        int w = where[e0 + rho];
+        double c = cert_cost(P, phase, e0 + rho);
+        y[rho] = w >= 0 ? c : c - (double)O->T[(size_t)M * ld + (-w - 1)];
+    }
     for (int it = 0;; it++) {
         double rn = 0.0;
         for (int i = 0; i < M; i++) {
-            double acc = cb[i];
-            for (int rho = 0; rho < M; rho++) {
-                double a = a2m(P, inS, rho, O->basis[i]);
-                if (a != 0.0) acc = fma(-a, y[rho], acc);
-            }
+            int bv = O->basis[i];
+            double acc = col_resid(P, inS, bv, cert_cost(P, phase, bv), y);
             r[i] = acc;
             if (fabs(acc) > rn) rn = fabs(acc);
         }
         if (rn <= 2e-12) return 1;
         if (it == CERT_STEPS) return 0;
-        binvT_apply(P, O, where, r, dy);
-        for (int rho = 0; rho < P->M2; rho++) y[rho] = y[rho] + dy[rho];
+        for (int rho = 0; rho < M; rho++) dy[rho] = binvT_col(P, O, where, r, rho);
+        for (int rho = 0; rho < M; rho++) y[rho] = y[rho] + dy[rho];
     }
-}
-
-/* reduced cost c_v - a_v^T y of variable v */
-static double red_cost(const Prob *P, const char *inS, int v, double cv, const double *y) {
-    double acc = cv;
-    for (int rho = 0; rho < P->M2; rho++) {
-        double a = a2m(P, inS, rho, v);
-        if (a != 0.0) acc = fma(-a, y[rho], acc);
-    }
-    return acc;
 }
 
 /* Certifies the FP32 phase-1 result in FP64 and sets out->kkt_feasible,
@@ -658,10 +662,10 @@ static int certify_phase1(const Prob *P, const MpqpOptions *o, const OSt_f *O, S
         any_art |= cb[i] != 0.0;
     }
     if (any_art) {
-        if (!cert_dual(P, O, s->where, s->inS, cb, y, r, d)) CFAIL(3);
+        if (!cert_dual(P, O, s->where, s->inS, 1, y, r, d)) CFAIL(3);
         for (int k = 0; k < P->K2; k++) {
             int v = O->nonb[k];
-            if (red_cost(P, s->inS, v, v >= P->N_ ? 1.0 : 0.0, y) < -o->certify_tol) CFAIL(4);
+            if (col_resid(P, s->inS, v, cert_cost(P, 1, v), y) < -o->certify_tol) CFAIL(4);
         }
     }
     for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
@@ -681,11 +685,11 @@ static int certify_phase2(const Prob *P, const MpqpOptions *o, const OSt_f *O, S
     for (int i = 0; i < M; i++)
         if (x[i] < -o->certify_tol) CFAIL(7);
     for (int i = 0; i < M; i++) cb[i] = O->basis[i] == eps ? -1.0 : 0.0;
-    if (!cert_dual(P, O, s->where, s->inS, cb, y, r, d)) CFAIL(8);
+    if (!cert_dual(P, O, s->where, s->inS, 2, y, r, d)) CFAIL(8);
     for (int k = 0; k < P->K2; k++) {
         int v = O->nonb[k];
         if (v >= P->N_) continue;
-        if (red_cost(P, s->inS, v, v == eps ? -1.0 : 0.0, y) < -o->certify_tol) CFAIL(9);
+        if (col_resid(P, s->inS, v, cert_cost(P, 2, v), y) < -o->certify_tol) CFAIL(9);
     }
     for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
     out->z2 = tree_sum_d(r, M);
@@ -900,6 +904,38 @@ static int prob_init(Prob *P, int n_c, int n_x, int n_t, const double *A1, const
     }
     for (int i = 0; i < M1; i++)
         if (!(xb1[i] >= 0.0)) { set_err("mpqp: xb1 must be >= 0 for the slack basis"); return -1; }
+    {
+        int nnz = 0;
+        for (size_t e = 0; e < (size_t)M2 * N2; e++) nnz += A2[e] != 0.0;
+        P->csc_ptr = (int *)calloc((size_t)N2 + 1, sizeof(int));
+        P->csr_ptr = (int *)calloc((size_t)M2 + 1, sizeof(int));
+        P->csc_row = (int *)malloc(sizeof(int) * (size_t)(nnz + 1));
+        P->csr_var = (int *)malloc(sizeof(int) * (size_t)(nnz + 1));
+        P->csc_val = (double *)malloc(sizeof(double) * (size_t)(nnz + 1));
+        P->csr_val = (double *)malloc(sizeof(double) * (size_t)(nnz + 1));
+        int e = 0;
+        for (int v = 0; v < N2; v++) {
+            for (int rho = 0; rho < M2; rho++) {
+                double a = A2[(size_t)v * M2 + rho];
+                if (a == 0.0) continue;
+                P->csc_row[e] = rho;
+                P->csc_val[e] = a;
+                P->csr_ptr[rho + 1]++;
+                e++;
+            }
+            P->csc_ptr[v + 1] = e;
+        }
+        for (int rho = 0; rho < M2; rho++) P->csr_ptr[rho + 1] += P->csr_ptr[rho];
+        int *fill = (int *)malloc(sizeof(int) * (size_t)M2);
+        for (int rho = 0; rho < M2; rho++) fill[rho] = P->csr_ptr[rho];
+        for (int v = 0; v < N2; v++)
+            for (int k = P->csc_ptr[v]; k < P->csc_ptr[v + 1]; k++) {
+                int at = fill[P->csc_row[k]]++;
+                P->csr_var[at] = v;
+                P->csr_val[at] = P->csc_val[k];
+            }
+        free(fill);
+    }
     P->root = (double *)calloc((size_t)M2 * P->ld2, sizeof(double));
     P->root_basis = (int *)malloc(sizeof(int) * (size_t)M2);
     P->root_nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
@@ -909,6 +945,8 @@ static int prob_init(Prob *P, int n_c, int n_x, int n_t, const double *A1, const
 
 static void prob_free(Prob *P) {
     free(P->root); free(P->root_basis); free(P->root_nonb);
+    free(P->csc_ptr); free(P->csc_row); free(P->csc_val);
+    free(P->csr_ptr); free(P->csr_var); free(P->csr_val);
 }
 
 static void fstate_alloc(FState *F, const Prob *P) {
