@@ -69,15 +69,17 @@ def cpu_baseline(lp, k_sample: int) -> dict:
             "sample": f"levels |S|<={k_sample} of the same problem, FP64: {n} checks in {dt:.2f} s"}
 
 
-def timed(lib, lp, k_max, steps, warmup, **opts):
+def timed(lib, lps, k_max, steps, warmup, **opts):
+    """A step is one mpqp() call per problem of the workload (one problem
+    except for the 64-problem workload); returns the last step's summed stats
+    and the wall time of the timed steps."""
     lib.set_options(**opts)
     for _ in range(warmup):
-        lib.run(lp, k_max=k_max)
+        lib.run_many(lps, k_max)
     t0 = time.perf_counter()
     for _ in range(steps):
-        lib.run(lp, k_max=k_max)
+        _, st = lib.run_many(lps, k_max)
     dt = time.perf_counter() - t0
-    st = lib.stats()
     return st, dt
 
 
@@ -87,7 +89,7 @@ def roofline(st: dict, lds_peak: float, fp32: bool) -> dict:
     longer in the last step.  Times are HIP events on the library's stream."""
     ms_opt = st["ms_opt"] - st["ms_fallback"]
     if ms_opt >= st["ms_feas"]:
-        name = "opt_kernel_t<float> (FP32 pivots + FP64 certification)" if fp32 else "opt_kernel_t<double>"
+        name = "opt_kernel32 (FP32 pivots + FP64 certification)" if fp32 else "opt_kernel64"
         launches = st["launches_opt"] - st["launches_fallback"]
         ms, byts, flops = ms_opt, st["bytes_opt"] - st["bytes_fallback"], st["flops_opt"] - st["flops_fallback"]
     else:
@@ -146,13 +148,12 @@ def main() -> int:
     if lib.warm_up() != 0:
         raise RuntimeError(lib.last_error())
     prog, k_max = config_problem(args.config)
-    if isinstance(prog, list):
-        raise SystemExit("the many-problem workload is measured by its parity tests, not here")
-    lp = assemble_lps(prog)
+    lps = [assemble_lps(p) for p in prog] if isinstance(prog, list) else [assemble_lps(prog)]
+    lp = lps[0]
     mixed = args.precision == "mixed"
     opts = dict(precision=1 if mixed else 0, batching=1 if args.batching == "persistent" else 0)
 
-    st, dt = timed(lib, lp, k_max, args.steps, args.warmup, **opts)
+    st, dt = timed(lib, lps, k_max, args.steps, args.warmup, **opts)
     checks = st["candidates"]
     value = checks * args.steps / dt
     lds_peak = lib.measure_lds_bandwidth()
@@ -172,26 +173,29 @@ def main() -> int:
         "dtype": "f32-pivot+f64-certify" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": _workload(args.config), "variant": f"{args.precision}-{args.batching}",
-                   "checks_per_step": checks, "precision_used": "mixed" if st["precision_used"] else "fp64",
+                   "checks_per_step": checks, "problems": len(lps),
+                   "precision_used": "mixed" if st["precision_used"] else "fp64",
                    "fallbacks_fp64": st["fallbacks"], "optimality_lps": st["opt_lps"]},
         "roofline": roofline(st, lds_peak, mixed and st["precision_used"] == 1),
         "stats": st,
     }
     if not args.no_compare and (mixed or args.batching != "plain"):
         # the FP64 plain variant: throughput and candidate-by-candidate verdicts
-        st64, dt64 = timed(lib, lp, k_max, args.steps, args.warmup, precision=0, batching=0)
+        st64, dt64 = timed(lib, lps, k_max, args.steps, args.warmup, precision=0, batching=0)
         v64 = st64["candidates"] * args.steps / dt64
         lib.set_options(record=1, **opts)
-        a = lib.run(lp, k_max=k_max, records=True)
+        ra, _ = lib.run_many(lps, k_max, records=True)
         lib.set_options(record=1, precision=0, batching=0)
-        b = lib.run(lp, k_max=k_max, records=True)
+        rb, _ = lib.run_many(lps, k_max, records=True)
         lib.set_options()
-        same_sets = a.sets.shape == b.sets.shape and bool(np.array_equal(a.sets, b.sets))
-        mism = {f: int(np.count_nonzero(a.records[f] != b.records[f])) if same_sets else -1 for f in VERDICTS}
+        same_sets = all(a.sets.shape == b.sets.shape and bool(np.array_equal(a.sets, b.sets))
+                        for a, b in zip(ra, rb))
+        mism = {f: (sum(int(np.count_nonzero(a.records[f] != b.records[f])) for a, b in zip(ra, rb))
+                    if same_sets else -1) for f in VERDICTS}
         out["fp64_plain"] = {"value": v64, "ms_per_step": dt64 / args.steps * 1e3,
                              "speedup": value / v64 if v64 > 0 else None,
                              "verdict_mismatches": mism, "same_candidates": same_sets,
-                             "same_active_sets": a.active_sets == b.active_sets}
+                             "same_active_sets": all(a.active_sets == b.active_sets for a, b in zip(ra, rb))}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(lp, min(args.cpu_k, k_max))
     print(json.dumps(out), flush=True)

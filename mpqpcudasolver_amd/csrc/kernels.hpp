@@ -53,6 +53,8 @@ struct OptArgs {
     int *fb_list;         // mixed: candidates whose FP32 result failed
     int *fb_count;        //        certification (appended), and their count
     long long *prof;      // phase cycle counts, 16 per workgroup (MPQP_PROFILE builds), or null
+    unsigned char *gtab;  // global-memory tableaus (one per workgroup) when LDS is too small, or null
+    long long gtab_stride;
 };
 
 // Byte layout of one stored state: T (rows x cols doubles, contiguous) then

@@ -34,10 +34,11 @@ namespace {
 constexpr int FEAS_NT = 64;
 constexpr int OPT_NT = 512;
 
-template <int NT, bool FP32>
+template <int NT, bool FP32, bool GT>
 const void *opt_fn_nt(int S) {
-#define MPQP_OPT_FN(s) \
-    (FP32 ? reinterpret_cast<const void *>(&opt_kernel32<NT, s>) : reinterpret_cast<const void *>(&opt_kernel64<NT, s>))
+#define MPQP_OPT_FN(s)                                                       \
+    (FP32 ? reinterpret_cast<const void *>(&opt_kernel32<NT, s, GT>)         \
+          : reinterpret_cast<const void *>(&opt_kernel64<NT, s, GT>))
     switch (S) {
     case 1: return MPQP_OPT_FN(1);
     case 2: return MPQP_OPT_FN(2);
@@ -50,10 +51,12 @@ const void *opt_fn_nt(int S) {
 }
 
 // The optimality kernel for a precision (FP32 pivots with certification, or
-// FP64), workgroup size and reduction slot count.
-const void *opt_fn(bool fp32, int nt, int S) {
-    if (fp32) return nt == 256 ? opt_fn_nt<256, true>(S) : opt_fn_nt<512, true>(S);
-    return nt == 256 ? opt_fn_nt<256, false>(S) : opt_fn_nt<512, false>(S);
+// FP64), workgroup size, reduction slot count and tableau placement (LDS,
+// or global memory when it does not fit: 512 threads only).
+const void *opt_fn(bool fp32, int nt, int S, bool gt) {
+    if (gt) return fp32 ? opt_fn_nt<512, true, true>(S) : opt_fn_nt<512, false, true>(S);
+    if (fp32) return nt == 256 ? opt_fn_nt<256, true, false>(S) : opt_fn_nt<512, true, false>(S);
+    return nt == 256 ? opt_fn_nt<256, false, false>(S) : opt_fn_nt<512, false, false>(S);
 }
 
 __global__ void fill_i32(int *p, int n, int v) {
@@ -300,8 +303,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     if (feas_lds_bytes(M1, N1 - M1, n_c) > C.lds_limit)
         throw HipError{"mpqp: feasibility tableau exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
-    if (opt_lds_bytes_t<double>(M2, N2 - M2, N2, false) > C.lds_limit)
-        throw HipError{"mpqp: optimality tableau exceeds the LDS of one workgroup (" +
+    if (opt_lds_bytes_t<double>(M2, N2 - M2, N2, false, true) > C.lds_limit)
+        throw HipError{"mpqp: optimality LP bookkeeping exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
     hipStream_t st = C.stream;
     const int K1 = N1 - M1, K2 = N2 - M2;
@@ -412,12 +415,16 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
 
     const long long fstride = state_bytes(M1, K1 + 1, M1, K1);
     const long long flds = feas_lds_bytes(M1, K1, n_c);
-    const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false);
-    const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true);
-    const int opt_nt = (o.opt_threads == 256) ? 256 : OPT_NT;
+    // tableau in LDS when it fits, else in global memory (one per resident
+    // workgroup, persistent grid)
+    const bool gt64 = opt_lds_bytes_t<double>(M2, K2, N2, false) > C.lds_limit;
+    const bool gt32 = opt_lds_bytes_t<float>(M2, K2, N2, true) > C.lds_limit;
+    const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false, gt64);
+    const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true, gt32);
+    const int opt_nt = (gt64 || o.opt_threads != 256) ? OPT_NT : 256;
     const int S = red_slots(std::max(M2, N2 - n_eqs));
-    const void *fn64 = opt_fn(false, opt_nt, S);
-    const void *fn32 = opt_fn(true, opt_nt, S);
+    const void *fn64 = opt_fn(false, opt_nt, S, gt64);
+    const void *fn32 = opt_fn(true, opt_nt, S, gt32);
     auto capacity = [&](const void *fn, long long lds) {
         int nb = 0;
         HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, opt_nt, (size_t)lds));
@@ -425,6 +432,9 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     };
     const int cap64 = capacity(fn64, lds64);
     const int cap32 = o.precision == MPQP_MIXED ? capacity(fn32, lds32) : 0;
+    DBuf gtab64, gtab32;
+    if (gt64) gtab64.alloc((size_t)opt_gtab_bytes<double>(M2, K2) * cap64);
+    if (gt32 && o.precision == MPQP_MIXED) gtab32.alloc((size_t)opt_gtab_bytes<float>(M2, K2) * cap32);
     // counters: [0] FP32 work, [1] fallback count, [2] FP64 work
     DBuf dctr(sizeof(int) * 4);
     auto launch = [&](const void *fn, int grid, long long lds, OptArgs &oa) {
@@ -594,6 +604,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     oa.counter = dctr.as<int>();
                     oa.fb_list = dfbl.as<int>();
                     oa.fb_count = dctr.as<int>() + 1;
+                    oa.gtab = gtab32.as<unsigned char>();
+                    oa.gtab_stride = opt_gtab_bytes<float>(M2, K2);
                     launch(fn32, std::min(nf, cap32), lds32, oa);
                     // FP64 re-solve from the cold start of what failed certification
                     OptArgs ob = oa;
@@ -604,10 +616,14 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     ob.store32 = 1;
                     ob.fb_list = nullptr;
                     ob.fb_count = nullptr;
+                    ob.gtab = gtab64.as<unsigned char>();
+                    ob.gtab_stride = opt_gtab_bytes<double>(M2, K2);
                     HIP_OK(hipEventRecord(e3, st));
                     launch(fn64, std::min(nf, cap64), lds64, ob);
-                } else if (o.batching == MPQP_PERSISTENT) {
+                } else if (o.batching == MPQP_PERSISTENT || gt64) {
                     oa.counter = dctr.as<int>();
+                    oa.gtab = gtab64.as<unsigned char>();
+                    oa.gtab_stride = opt_gtab_bytes<double>(M2, K2);
                     launch(fn64, std::min(nf, cap64), lds64, oa);
                 } else {
                     launch(fn64, nf, lds64, oa);

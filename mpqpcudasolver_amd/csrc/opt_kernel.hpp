@@ -18,16 +18,23 @@
 // (FP64) instead of a few.
 __host__ __device__ inline int opt_ld(int K) { return (K + 2) | 1; }
 
-// LDS of one workgroup: T (M+1) x ld, colq (M+1), vcol (M) in Real; basis,
-// nonb, where, bcnt, nbcnt, rows in int; for certification (FP32 pivots)
-// x and r in double.
+// LDS of one workgroup: T (M+1) x ld (unless the tableau lives in global
+// memory: gt), colq (M+1), vcol (M) in Real; basis, nonb, where, bcnt,
+// nbcnt, rows in int; for certification (FP32 pivots) x and r in double.
 template <typename Real>
-__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert) {
+__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, bool gt = false) {
     const int ld = opt_ld(K);
-    long long b = (long long)((M + 1) * ld + (M + 1) + M) * (long long)sizeof(Real) +
+    long long b = (long long)((gt ? 0 : (M + 1) * ld) + (M + 1) + M) * (long long)sizeof(Real) +
                   (long long)(M + K + 3 * N2 + M + 1) * 4;
     if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
     return (b + 15) & ~15LL;
+}
+
+// Global-memory tableau of one workgroup (when the LDS one does not fit):
+// (M+1) x ld Real, 256-byte aligned.
+template <typename Real>
+__host__ __device__ inline long long opt_gtab_bytes(int M, int K) {
+    return ((long long)(M + 1) * opt_ld(K) * (long long)sizeof(Real) + 255) & ~255LL;
 }
 
 // Stored phase-1 state: T (M rows of ld Real, as in LDS, padded to a
@@ -81,10 +88,11 @@ struct OptLds {
     Real *T, *colq, *vcol;
     int *basis, *nonb, *where, *bcnt, *nbcnt, *rows;
     double *cx, *cr;  // certification vectors (FP32 pivots only)
-    __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert) {
+    // gT: the tableau in global memory, or null for the LDS tableau
+    __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT) {
         ld = opt_ld(K);
-        T = reinterpret_cast<Real *>(smem);
-        colq = T + (M + 1) * ld;
+        T = gT ? gT : reinterpret_cast<Real *>(smem);
+        colq = gT ? reinterpret_cast<Real *>(smem) : T + (M + 1) * ld;
         vcol = colq + (M + 1);
         basis = reinterpret_cast<int *>(vcol + M);
         nonb = basis + M;
@@ -814,7 +822,7 @@ __device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, u
 // Real = float, MIXED = true: FP32 pivots, FP64 certification; a candidate
 // that cannot be certified is appended to a.fb_list for the FP64 re-solve
 // (launched with force_cold and store32).
-template <typename Real, int NT, int S, bool MIXED>
+template <typename Real, int NT, int S, bool MIXED, bool GT>
 __device__ __forceinline__ void opt_body(const OptArgs &a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ OptShared sh;
@@ -827,7 +835,8 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
     const int k = a.level;
     const int n = a.n_dev ? *a.n_dev : a.n;
     OptLds<Real> L;
-    L.carve(smem, M, K, N2, MIXED);
+    L.carve(smem, M, K, N2, MIXED,
+            GT ? reinterpret_cast<Real *>(a.gtab + (long long)blockIdx.x * a.gtab_stride) : nullptr);
     const int ld = L.ld;
     const DevTol tl = MIXED ? dev_tol32(O) : dev_tol64(O);
     for (int iter = 0;; ++iter) {
@@ -971,14 +980,14 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
     }
 }
 
-template <int NT, int S>
+template <int NT, int S, bool GT>
 __global__ __launch_bounds__(NT) void opt_kernel64(OptArgs a) {
-    opt_body<double, NT, S, false>(a);
+    opt_body<double, NT, S, false, GT>(a);
 }
 
 // FP32 pivots: capped at 128 VGPRs so that two 512-thread workgroups (two
 // LPs, 2 x ~80 KB of LDS at n=30 m=62 p=8) share a CU.
-template <int NT, int S>
+template <int NT, int S, bool GT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void opt_kernel32(OptArgs a) {
-    opt_body<float, NT, S, true>(a);
+    opt_body<float, NT, S, true, GT>(a);
 }

@@ -335,6 +335,23 @@ class MpqpLib:
         self.lib.mpqp_free_results(res)
         return run
 
+    def run_many(self, lps, k_max: int, records: bool = False) -> tuple[list, dict]:
+        """One mpqp() call per problem, as the reference's host loops over its
+        problems (main.py:60-170); returns the runs and the summed stats
+        (counts and kernel times added, precision_used of the last call)."""
+        runs, tot = [], None
+        for lp in lps:
+            runs.append(self.run(lp, k_max=k_max, records=records))
+            st = self.stats()
+            if tot is None:
+                tot = dict(st)
+            else:
+                for key, val in st.items():
+                    if key != "precision_used":
+                        tot[key] += val
+                tot["precision_used"] = st["precision_used"]
+        return runs, tot or {}
+
 
 def load_library(path: str | None = None) -> MpqpLib:
     """Loads libmpqp_hip.so (the HIP engine).  Raises if it is missing."""

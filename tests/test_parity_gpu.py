@@ -264,6 +264,86 @@ def test_mixed_full_config_sampled(hip, orc):
     orc.set_options()
 
 
+# ---------------------------------------- tableau larger than LDS (n=60 m=120 p=12)
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_parity_config4_first_levels(hip, orc, precision):
+    """The optimality tableau (497 KB FP64 / 257 KB FP32) lives in global
+    memory; records and bases must still match the oracle exactly."""
+    _log(f"case: config 4 |S| <= 1, precision {precision}")
+    prog, _ = config_problem(4)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 1, precision=precision)
+    assert compare_runs(g, o) == []
+    hip.set_options()
+    orc.set_options()
+
+
+def test_config4_full_sampled_and_mixed_verdicts(hip, orc):
+    """Full |S| <= 3 run: 40 sampled candidates against the oracle along their
+    prefix chains, and the mixed path's verdicts equal the FP64 path's."""
+    _log("case: config 4 full run, sampled parity and mixed verdicts")
+    prog, k = config_problem(4)
+    lp = assemble_lps(prog)
+    hip.set_options(record=1)
+    f = hip.run(lp, k_max=k, records=True)
+    r = f.records
+    sets = [[int(v) for v in row if v >= 0] for row in f.sets]
+    rng = np.random.default_rng(4)
+    for c in rng.choice(len(sets), 40, replace=False):
+        rec = orc.eval_set(lp, sets[c])[0]
+        for fld in INT_FIELDS:
+            if fld == "parent":
+                continue
+            assert rec[fld] == r[fld][c], (sets[c], fld, rec[fld], r[fld][c])
+        for fld in FLOAT_FIELDS:
+            assert abs(rec[fld] - r[fld][c]) <= REL_TOL * max(1.0, abs(rec[fld])), (sets[c], fld)
+    hip.set_options(record=1, precision=1)
+    m = hip.run(lp, k_max=k, records=True)
+    hip.set_options()
+    assert np.array_equal(f.sets, m.sets)
+    for fld in ("feasible", "kkt_feasible", "final_set"):
+        d = np.flatnonzero(f.records[fld] != m.records[fld])
+        assert len(d) == 0, (fld, len(d), d[:5])
+    assert f.active_sets == m.active_sets
+
+
+# --------------------------------------------------- 64 uneven problems
+
+def test_parity_uneven_problems_subset(hip, orc):
+    """Eight of the 64 uneven problems (smallest to largest: LDS and global
+    tableaus), |S| <= 2, both precisions, record for record."""
+    progs, _ = config_problem(2)
+    lps = [assemble_lps(p) for p in progs]
+    order = sorted(range(len(lps)), key=lambda i: lps[i].M2)
+    pick = [order[int(round(t))] for t in np.linspace(0, len(order) - 1, 8)]
+    for i in pick:
+        for prec in (0, 1):
+            _log(f"case: uneven problem {i} (M2={lps[i].M2}), precision {prec}")
+            g, o = _run_pair(hip, orc, lps[i], 2, precision=prec)
+            assert compare_runs(g, o) == [], (i, prec)
+    hip.set_options()
+    orc.set_options()
+
+
+def test_uneven_problems_mixed_verdicts(hip):
+    """All 64 problems, |S| <= 3: mixed-precision verdicts equal FP64's."""
+    _log("case: 64 uneven problems, mixed vs FP64 verdicts")
+    progs, k = config_problem(2)
+    lps = [assemble_lps(p) for p in progs]
+    hip.set_options(record=1)
+    f, stf = hip.run_many(lps, k, records=True)
+    hip.set_options(record=1, precision=1)
+    m, stm = hip.run_many(lps, k, records=True)
+    hip.set_options()
+    assert stf["candidates"] == stm["candidates"]
+    for i, (a, b) in enumerate(zip(f, m)):
+        assert np.array_equal(a.sets, b.sets), i
+        for fld in ("feasible", "kkt_feasible", "final_set"):
+            d = np.flatnonzero(a.records[fld] != b.records[fld])
+            assert len(d) == 0, (i, fld, len(d))
+    _log(f"  {stf['candidates']} candidates, fallbacks {stm['fallbacks']} of {stm['opt_lps']}")
+
+
 def test_persistent_batching_matches_plain(hip):
     _log("case: persistent FP64 launch vs plain")
     prog, _ = config_problem(1)
