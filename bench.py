@@ -83,6 +83,25 @@ def timed(lib, lps, k_max, steps, warmup, **opts):
     return st, dt
 
 
+def profiled_traffic(kernel: str, cfg: int, variant: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy-kernel
+    calibration), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_config{cfg}_{variant}.md")
+    if not os.path.exists(path):
+        return None
+    in_table = False
+    for line in open(path):
+        in_table = in_table or line.startswith("HBM traffic per dispatch")
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if in_table and len(cells) == 5 and cells[0].startswith(f"`void {kernel}<"):
+            try:
+                return float(cells[4]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def roofline(st: dict, lds_peak: float, fp32: bool) -> dict:
     """Dominant kernel: the optimality kernel (FP32 pivots in a mixed run,
     fallback launches excluded) or the feasibility kernel, whichever took
@@ -179,6 +198,13 @@ def main() -> int:
         "roofline": roofline(st, lds_peak, mixed and st["precision_used"] == 1),
         "stats": st,
     }
+    fp32 = mixed and st["precision_used"] == 1
+    if out["roofline"]["kernel"].startswith("opt_kernel"):
+        kname = "opt_kernel32" if fp32 else "opt_kernel64"
+        out["roofline"]["traffic"] = profiled_traffic(kname, args.config, f"{args.precision}_{args.batching}")
+        if out["roofline"]["traffic"] is not None:
+            out["roofline"]["traffic_note"] = (f"HBM bytes per launch from profiles/{kname}_config{args.config}_"
+                                               f"{args.precision}_{args.batching}.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     if not args.no_compare and (mixed or args.batching != "plain"):
         # the FP64 plain variant: throughput and candidate-by-candidate verdicts
         st64, dt64 = timed(lib, lps, k_max, args.steps, args.warmup, precision=0, batching=0)

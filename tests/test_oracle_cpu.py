@@ -296,3 +296,25 @@ def test_empty_set_closed_form(orc):
     assert rec["kkt_feasible"] == int(kkt_lp_direct(prog, []))
     st, eps = margin_lp(prog, [])
     assert st == 0 and abs(eps + rec["z2"]) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["config0", "small8x24", "stress10x24"])
+def test_mixed_precision_verdicts_equal_fp64(oracle_path, name):
+    """The oracle's mixed path (FP32 pivots, FP64 certification, FP64
+    re-solve) gives the FP64 path's verdicts on every candidate; the
+    ill-conditioned problem exercises the certification failures."""
+    prog, k = {
+        "config0": lambda: config_problem(0),
+        "small8x24": lambda: (synthetic_mpqp(8, 24, 3, 11), 3),
+        "stress10x24": lambda: (precision_stress_mpqp(10, 24, 3, 13, n_dup=4), 3),
+    }[name]()
+    lp = assemble_lps(prog)
+    o = OracleLib(oracle_path)
+    o.set_options(record=1)
+    a = o.run(lp, k_max=k, records=True)
+    o.set_options(record=1, precision=1)
+    b = o.run(lp, k_max=k, records=True)
+    assert np.array_equal(a.sets, b.sets)
+    for f in ("feasible", "kkt_feasible", "final_set"):
+        assert np.array_equal(a.records[f], b.records[f]), f
+    assert a.active_sets == b.active_sets
