@@ -489,7 +489,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         const int last_lvl = (mixed && k_max == 0) ? 1 : k_max;
         const long long ostride = mixed ? opt_state_bytes<float>(M2, K2) : opt_state_bytes<double>(M2, K2);
 
-        std::vector<int> prev_sets, prev_feas, prev_clean;
+        std::vector<int> prev_sets, prev_feas, prev_warm;
         std::vector<int> prev_recidx;
         DBuf fstore_prev, ostore_prev;
         int prev_n = 0;
@@ -567,12 +567,12 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             C.stats.ms_feas += ms;
 
             // optimality LPs of the feasible candidates; warm start from the
-            // parent's phase-1 state when that state is clean
+            // parent's final phase-1 state
             std::vector<int> list, p1par(n, -1);
             for (int c = 0; c < n; ++c) {
                 if (hrec[c].feasible) list.push_back(c);
                 const int pi = cur_par[c];
-                if (pi >= 0 && prev_clean[pi]) p1par[c] = pi;
+                if (pi >= 0 && prev_warm[pi]) p1par[c] = pi;
             }
             const int nf = (int)list.size();
             int n_fb = 0;
@@ -691,14 +691,15 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 St.flops_opt += f_opt;
             }
             const int base = (int)all.size();
-            std::vector<int> clean(n);
+            // a phase-1 state warm-starts the children whenever phase 1 ended optimal
+            std::vector<int> warm_ok(n);
             for (int c = 0; c < n; ++c) {
                 MpqpRecord r;
                 std::memcpy(&r, &hrec[c], sizeof r);
                 r.level = lvl;
                 r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
                 all.push_back(r);
-                clean[c] = r.p1_clean;
+                warm_ok[c] = r.p1_status == ST_OPTIMAL;
             }
             allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
             if (bases) {
@@ -714,7 +715,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             prev_sets.swap(cur_sets);
             prev_feas.assign(n, 0);
             for (int c = 0; c < n; ++c) prev_feas[c] = hrec[c].feasible;
-            prev_clean.swap(clean);
+            prev_warm.swap(warm_ok);
             prev_recidx.resize(n);
             for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
             prev_n = n;

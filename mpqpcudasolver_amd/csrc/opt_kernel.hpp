@@ -129,19 +129,15 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
         const int4 *s4 = reinterpret_cast<const int4 *>(sT);
         int4 *d4 = reinterpret_cast<int4 *>(L.T);
         constexpr int UL = 8;  // loads in flight per thread
-        for (int e0 = tid; e0 < nv; e0 += NT * UL) {
+        int e0 = tid;
+        for (; e0 + (UL - 1) * NT < nv; e0 += NT * UL) {
             int4 vv[UL];
 #pragma unroll
-            for (int u = 0; u < UL; ++u) {
-                const int e = e0 + u * NT;
-                if (e < nv) vv[u] = s4[e];
-            }
+            for (int u = 0; u < UL; ++u) vv[u] = s4[e0 + u * NT];
 #pragma unroll
-            for (int u = 0; u < UL; ++u) {
-                const int e = e0 + u * NT;
-                if (e < nv) d4[e] = vv[u];
-            }
+            for (int u = 0; u < UL; ++u) d4[e0 + u * NT] = vv[u];
         }
+        for (int e = e0; e < nv; e += NT) d4[e] = s4[e];
     } else {
         const int sw = K + 2, ne = M * sw;
         for (int e = tid; e < ne; e += NT) {
@@ -691,17 +687,31 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
             const int bv = L.basis[i];
             const int ru = bv >= e0 ? bv - e0 : M;
             bool done = ru >= M;
+            const double rU = done ? 0.0 : r[ru];
             double acc = 0.0;
-            for (int k = 0; k < ne; ++k) {
-                const int rk = L.bcnt[k];
-                const double tv = (double)Ti[L.nbcnt[k]];
-                if (!done && ru < rk) {
-                    acc = fma(r[ru], 1.0, acc);
-                    done = true;
+            constexpr int B = 4;  // LDS loads in flight
+            for (int k0 = 0; k0 < ne; k0 += B) {
+                int rk[B];
+                double rv[B], tv[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) rk[u] = k0 + u < ne ? L.bcnt[k0 + u] : M;
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const bool in = k0 + u < ne;
+                    rv[u] = in ? r[rk[u]] : 0.0;
+                    tv[u] = in ? (double)Ti[L.nbcnt[k0 + u]] : 0.0;
                 }
-                acc = fma(r[rk], tv, acc);
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    if (k0 + u >= ne) break;
+                    if (!done && ru < rk[u]) {
+                        acc = fma(rU, 1.0, acc);
+                        done = true;
+                    }
+                    acc = fma(rv[u], tv[u], acc);
+                }
             }
-            if (!done) acc = fma(r[ru], 1.0, acc);
+            if (!done) acc = fma(rU, 1.0, acc);
             x[i] = x[i] + acc;
         }
         __syncthreads();
@@ -744,7 +754,19 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
                 acc = fma(r[w], 1.0, 0.0);
             } else {
                 const float *Tk = L.T + (-w - 1);
-                for (int i = 0; i < M; ++i) acc = fma(r[i], (double)Tk[i * ld], acc);
+                constexpr int B = 4;  // LDS loads in flight
+                for (int i0 = 0; i0 < M; i0 += B) {
+                    double rv[B], tv[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        const bool in = i0 + u < M;
+                        rv[u] = in ? r[i0 + u] : 0.0;
+                        tv[u] = in ? (double)Tk[(i0 + u) * ld] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < B; ++u)
+                        if (i0 + u < M) acc = fma(rv[u], tv[u], acc);
+                }
             }
             y[rho] = y[rho] + acc;
         }
@@ -923,7 +945,7 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
             if (p1.kkt) clean = opt_drive_out<Real, NT, S>(L, P, tl, sh);
             if (a.b1)
                 for (int i = tid; i < M; i += NT) a.b1[(long long)c * M + i] = L.basis[i];
-            if (p1.kkt && clean && a.store) {
+            if (p1.status == ST_OPTIMAL && a.store) {
                 unsigned char *dst = a.ostore_cur + (long long)c * a.ostride;
                 if (a.store32) opt_store<Real, float, NT>(L, M, K, dst);
                 else opt_store<Real, Real, NT>(L, M, K, dst);

@@ -76,6 +76,11 @@
  *   - an empty ratio test (stale th_idxs :871-876, idx -1 write :941)
  *     ends the LP with status UNBOUNDED (phase 1: infeasible).
  *   - LPs stopped by a cap or the stall exit keep verdict 0 with a status.
+ *   - warm starts: copy_basis_dual_1 (:553-590) takes a root only if its
+ *     phase-1 objective is below 1e-12; here every parent whose phase 1
+ *     ended optimal hands its phase-1 state to its children (half the
+ *     phase-1 pivots of the n=30 m=62 p=8 problem; the cleanup above makes
+ *     any start basis valid).
  *
  * Mixed precision (no counterpart in the reference, which is FP64 only):
  * the optimality LP pivots in FP32 (tolerances tol32 / licq32 /
@@ -515,7 +520,7 @@ typedef struct {
 } BasisOut;
 
 /* Evaluate one candidate S (|S| = k, ascending) given its parent's final
- * states (fpar / opar may be NULL; opar_ok: parent phase 1 feasible).
+ * states (fpar / opar may be NULL; opar_ok: parent phase 1 ended optimal).
  * The candidate's final feasibility state is left in F and its phase-1
  * final state in O1 (when kkt feasible). */
 /* ------------------------------------------------------------------ */
@@ -810,7 +815,7 @@ static void opt_mixed(const Prob *P, const MpqpOptions *o, const int *S, int k, 
     rec->p2_status = rec->p2_iters = rec->final_set = 0;
     rec->z2 = 0.0;
     opt_fp64(P, o, S, k, NULL, 0, O1d, O2d, s, rec, bo);
-    if (rec->kkt_feasible) {
+    if (rec->p1_status == MPQP_ST_OPTIMAL) {
         for (size_t e = 0; e < (size_t)(P->M2 + 1) * P->ld2; e++) O1f->T[e] = (float)O1d->T[e];
         memcpy(O1f->basis, O1d->basis, sizeof(int) * (size_t)P->M2);
         memcpy(O1f->nonb, O1d->nonb, sizeof(int) * (size_t)P->K2);
@@ -818,7 +823,7 @@ static void opt_mixed(const Prob *P, const MpqpOptions *o, const int *S, int k, 
 }
 
 /* Evaluate one candidate S (|S| = k, ascending) given its parent's final
- * states (fpar / opar may be NULL; opar_ok: parent phase-1 state clean).
+ * states (fpar / opar may be NULL; opar_ok: parent phase 1 ended optimal).
  * opar is an OSt_d (FP64) or an OSt_f (mixed precision).  The candidate's
  * feasibility state is left in F and its phase-1 state in O1d / O1f. */
 static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, int k,
@@ -1031,6 +1036,11 @@ static double now_s(void) {
  * ill-conditioned for FP32 pivots and is solved in FP64 throughout.
  * Returns 1 to keep mixed precision. */
 #define MIXED_PROBE_MAX_FALLBACK 0.25
+/* A candidate's phase-1 state warm-starts its children whenever its phase 1
+ * ended optimal (copy_basis_dual_1 :553-590 takes only roots with
+ * |z| < 1e-12; phase 1 here ends with the dual-feasibility cleanup, so any
+ * start basis is valid, and the parent's optimum is close to the child's). */
+#define WARM_OK(rec) ((rec).p1_status == MPQP_ST_OPTIMAL)
 static int mixed_probe(const Prob *P, const MpqpOptions *o) {
     Scratch s;
     scratch_init(&s, P);
@@ -1048,7 +1058,7 @@ static int mixed_probe(const Prob *P, const MpqpOptions *o) {
     if (r0.feasible) {
         for (int j = 0; j < P->n_c; j++) {
             int S[1] = {j};
-            eval_candidate(P, o, S, 1, &F0, &W0.f, r0.p1_clean, &F1, &W1.d, &W2.d, &W1.f, &W2.f, &s, &r1, NULL);
+            eval_candidate(P, o, S, 1, &F0, &W0.f, WARM_OK(r0), &F1, &W1.d, &W2.d, &W1.f, &W2.f, &s, &r1, NULL);
             if (r1.feasible) { n_opt++; n_fb += r1.repivots; }
         }
     }
@@ -1162,13 +1172,13 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
                                pi >= 0 ? prev_kkt[pi] : 0, &F, &W1.d, &W2.d, &W1.f, &W2.f, &s, rec, &bo);
                 rec->parent = pi >= 0 ? prev_rec[pi] : -1;
                 feas[c] = rec->feasible;
-                kkt[c] = rec->p1_clean;
+                kkt[c] = WARM_OK(*rec);
                 if (keep && rec->feasible) {
                     fstate_alloc(&curF[c], &P);
                     memcpy(curF[c].T, F.T, sizeof(double) * (size_t)(M1 + 1) * P.ld1);
                     memcpy(curF[c].basis, F.basis, sizeof(int) * (size_t)M1);
                     memcpy(curF[c].nonb, F.nonb, sizeof(int) * (size_t)P.K1);
-                    if (rec->p1_clean) opack_keep(&curO[c], &P, mixed, &W1.d, &W1.f);
+                    if (WARM_OK(*rec)) opack_keep(&curO[c], &P, mixed, &W1.d, &W1.f);
                 }
             }
             fstate_free(&F);
@@ -1312,7 +1322,7 @@ int oracle_eval_set(const int *S, int k, int n_c, int n_x, int n_t, double *A1, 
         if (t < k && !r.feasible) { ret = 0; break; }
         fpar = fcur; { FState *tmp = fspare; fspare = fcur; fcur = tmp; }
         opar = ocur; { OPack *tmp = ospare; ospare = ocur; ocur = tmp; }
-        opar_ok = r.p1_clean;
+        opar_ok = WARM_OK(r);
     }
     if (ret == 1 && rec) { *rec = r; rec->parent = -1; }
     fstate_free(&Fa); fstate_free(&Fb);
