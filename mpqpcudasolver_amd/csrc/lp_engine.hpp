@@ -56,13 +56,14 @@ struct DevProb {
     const int *root_nonb;    // K2
     const int *root_ok;      // 1 int
     double cert_ptol;        // primal certification: 1e-12 * (1 + max|xb2|)
-    // A2 (column-major, M2 x N2) in sparse form for the FP64 certification,
-    // ELL layout (entry t of line l at t * lines + l, so that consecutive
-    // lines are adjacent in memory): rows with their variables ascending,
-    // columns with their rows ascending.
-    const int *csr_n, *csr_var;    // per row: count; M2 * csr_w
+    // A2 (column-major, M2 x N2) in sparse form for the FP64 certification.
+    // Rows, variables ascending, in an ELL layout interleaved for the row
+    // teams (GR lanes): entry t of row rho at ((t / GR) * M2 + rho) * GR +
+    // t % GR.  Columns, rows ascending, contiguous from csc_ptr[v] (a
+    // multiple of the column team size).
+    const int *csr_n, *csr_var;    // per row: count; entries
     const double *csr_val;
-    const int *csc_n, *csc_row;    // per column: count; N2 * csc_w
+    const int *csc_n, *csc_ptr, *csc_row;  // per column: count, start; entries
     const double *csc_val;
 };
 

@@ -332,44 +332,55 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         P.cert_ptol = 1e-12 * (1.0 + bmax);
     }
 
-    // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification, in
-    // ELL layout: entry t of row rho at t * M2 + rho (variables ascending),
-    // entry t of column v at t * N2 + v (rows ascending)
-    std::vector<int> rn(M2, 0), cn(N2, 0);
+    // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification:
+    // rows (variables ascending) in an ELL layout interleaved for teams of
+    // GR lanes, entry t of row rho at ((t / GR) * M2 + rho) * GR + t % GR;
+    // columns (rows ascending) contiguous, each starting at a multiple of
+    // the column team size GC
+    constexpr int GR = MPQP_GR, GC = MPQP_GC;
+    std::vector<int> rn(M2, 0), cn(N2, 0), cptr(N2, 0);
     for (int v = 0; v < N2; ++v)
         for (int rho = 0; rho < M2; ++rho)
             if (A2[(size_t)v * M2 + rho] != 0.0) {
                 rn[rho]++;
                 cn[v]++;
             }
-    int rw = 1, cw = 1;
-    for (int rho = 0; rho < M2; ++rho) rw = std::max(rw, rn[rho]);
-    for (int v = 0; v < N2; ++v) cw = std::max(cw, cn[v]);
-    std::vector<int> rvar((size_t)rw * M2, -1), crow((size_t)cw * N2, -1), rfill(M2, 0), cfill(N2, 0);
-    std::vector<double> rval((size_t)rw * M2, 0.0), cval((size_t)cw * N2, 0.0);
-    for (int v = 0; v < N2; ++v)
+    int rw = GR, csz = 0;
+    for (int rho = 0; rho < M2; ++rho) rw = std::max(rw, (rn[rho] + GR - 1) / GR * GR);
+    for (int v = 0; v < N2; ++v) {
+        cptr[v] = csz;
+        csz += std::max(GC, (cn[v] + GC - 1) / GC * GC);
+    }
+    std::vector<int> rvar((size_t)rw * M2, -1), crow((size_t)csz, -1), rfill(M2, 0);
+    std::vector<double> rval((size_t)rw * M2, 0.0), cval((size_t)csz, 0.0);
+    for (int v = 0; v < N2; ++v) {
+        int cf = 0;
         for (int rho = 0; rho < M2; ++rho) {
             const double a = A2[(size_t)v * M2 + rho];
             if (a == 0.0) continue;
-            const size_t er = (size_t)rfill[rho]++ * M2 + rho, ec = (size_t)cfill[v]++ * N2 + v;
+            const int t = rfill[rho]++;
+            const size_t er = ((size_t)(t / GR) * M2 + rho) * GR + (t % GR), ec = (size_t)cptr[v] + cf++;
             rvar[er] = v;
             rval[er] = a;
             crow[ec] = rho;
             cval[ec] = a;
         }
-    DBuf dsp_i(sizeof(int) * (M2 + N2 + rvar.size() + crow.size()));
+    }
+    DBuf dsp_i(sizeof(int) * (M2 + 2 * N2 + rvar.size() + crow.size()));
     DBuf dsp_d(sizeof(double) * (rval.size() + cval.size()));
     {
         int *ib0 = dsp_i.as<int>();
         double *db0 = dsp_d.as<double>();
         P.csr_n = ib0;
         P.csc_n = ib0 + M2;
-        P.csr_var = ib0 + M2 + N2;
+        P.csc_ptr = ib0 + M2 + N2;
+        P.csr_var = ib0 + M2 + 2 * N2;
         P.csc_row = P.csr_var + rvar.size();
         P.csr_val = db0;
         P.csc_val = db0 + rval.size();
         HIP_OK(hipMemcpyAsync((void *)P.csr_n, rn.data(), sizeof(int) * rn.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csc_n, cn.data(), sizeof(int) * cn.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csc_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csr_var, rvar.data(), sizeof(int) * rvar.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csc_row, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csr_val, rval.data(), sizeof(double) * rval.size(), hipMemcpyHostToDevice, st));

@@ -584,18 +584,50 @@ __device__ __forceinline__ double block_max(double v, double *red) {
     return m;
 }
 
-// Every product below is a sequential fma chain in a fixed order, one
-// thread per output entry: A2 through its CSR rows (variables ascending)
-// and CSC columns (rows ascending), masked entries and nonbasic variables
-// skipped; B^-1 through the e-columns of the FP32 tableau.  Sparse walks
-// load MPQP_CB entries at a time so that their latencies overlap.
-#define MPQP_CB 4
+// Team sums.  Every product entry is the sum of an ordered term list (A2
+// through its CSR rows, variables ascending, or CSC columns, rows
+// ascending; B^-1 through the nonbasic e-columns, rho ascending, or the
+// positions, ascending), computed by a team of G adjacent lanes: lane g
+// runs the fma chain of terms t = g, g + G, ... (lane 0 starts from the
+// constant term, the others from +0.0; masked entries and nonbasic
+// variables are skipped in place), and the team adds its partials by lane
+// exchange, xor 1 then xor 2, so every lane ends with (p0+p1)+(p2+p3).
+// Each lane keeps MPQP_CB (A2, global memory) or MPQP_CBL (tableau, LDS)
+// terms in flight.  Team sizes: rows of A2 2,
+// columns of A2 4, rows of B^-1 2, columns of B^-1 4 (a team never
+// straddles a wave: G divides 64).
+#define MPQP_CB 4   // A2 entries in flight per lane
+#define MPQP_CBL 4  // tableau entries in flight per lane
+#define MPQP_GR 2
+#define MPQP_GC 4
+#define MPQP_GB 2
+#define MPQP_GT 4
+
+// lane g receives lane g ^ X (X = 1, 2: DPP quad permutes)
+template <int X>
+__device__ __forceinline__ double lane_xor_d(double v) {
+    static_assert(X == 1 || X == 2, "xor");
+    constexpr int ctrl = X == 1 ? 0xB1 : 0x4E;  // quad_perm [1,0,3,2] / [2,3,0,1]
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)(b & 0xffffffffLL), (int)(unsigned)(b & 0xffffffffLL),
+                                               ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int G>
+__device__ __forceinline__ double team_add(double v) {
+    static_assert(G == 1 || G == 2 || G == 4, "team size");
+    if constexpr (G >= 2) v = v + lane_xor_d<1>(v);
+    if constexpr (G >= 4) v = v + lane_xor_d<2>(v);
+    return v;
+}
 
 // Compact list of the nonbasic e-variables, rho ascending: elist_rho[k],
-// elist_slot[k] (in the phase-1 use counters, free during certification);
-// count in sh.best.
+// elist_slot[k] (in the phase-1 use counters, free during certification).
+// Returns their count.
 template <int S>
-__device__ __forceinline__ void elist_build(OptLds<float> &L, const DevProb &P, OptShared &sh) {
+__device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, OptShared &sh) {
     const int M = P.M2, e0 = P.N2 - M;
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     if (threadIdx.x < MPQP_WAVE) {
@@ -616,20 +648,25 @@ __device__ __forceinline__ void elist_build(OptLds<float> &L, const DevProb &P, 
         if (lane == 0) sh.best = base;
     }
     __syncthreads();
+    const int ne = sh.best;
+    __syncthreads();  // sh.best is free again
+    return ne;
 }
 
-// c - sum over column v of A2 (rows ascending, masked skipped) of a * y[row]
-__device__ __forceinline__ double col_resid(const DevProb &P, const unsigned *inS, int v, double c, const double *y) {
-    double acc = c;
-    const int n = P.csc_n[v], N2 = P.N2;
-    for (int t0 = 0; t0 < n; t0 += MPQP_CB) {
+// Lane g's partial of c - sum over column v of A2 (rows ascending, masked
+// skipped) of a * y[row]: entries t = g, g + MPQP_GC, ... (c on lane 0).
+__device__ __forceinline__ double col_resid(const DevProb &P, const unsigned *inS, int v, int g, double c,
+                                            const double *y) {
+    double acc = g == 0 ? c : 0.0;
+    const unsigned n = (unsigned)P.csc_n[v], base = (unsigned)P.csc_ptr[v];
+    for (unsigned t0 = (unsigned)g; t0 < n; t0 += MPQP_GC * MPQP_CB) {
         int rr[MPQP_CB];
         double av[MPQP_CB], yv[MPQP_CB];
 #pragma unroll
         for (int u = 0; u < MPQP_CB; ++u) {
-            const int t = t0 + u;
-            rr[u] = t < n ? P.csc_row[t * N2 + v] : -1;
-            av[u] = t < n ? P.csc_val[t * N2 + v] : 0.0;
+            const unsigned t = t0 + u * MPQP_GC;
+            rr[u] = t < n ? P.csc_row[base + t] : -1;
+            av[u] = t < n ? P.csc_val[base + t] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < MPQP_CB; ++u) yv[u] = rr[u] >= 0 ? y[rr[u]] : 0.0;
@@ -640,30 +677,35 @@ __device__ __forceinline__ double col_resid(const DevProb &P, const unsigned *in
     return acc;
 }
 
-// x (LDS) := refined solution of B x = xb2; 1 once max|xb2 - B x| <= cert_ptol
+// x (LDS) := refined solution of B x = xb2; 1 once max|xb2 - B x| <=
+// cert_ptol.  *ne_out: the number of nonbasic e-variables (the e-list).
 template <int NT, int S>
 __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
-                                           OptShared &sh) {
+                                           OptShared &sh, int *ne_out) {
     const int M = P.M2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
     double *x = L.cx, *r = L.cr;
-    elist_build<S>(L, P, sh);
-    const int ne = sh.best;
+    const int ne = elist_build<S>(L, P, sh);
+    *ne_out = ne;
     for (int i = threadIdx.x; i < M; i += NT) x[i] = (double)L.T[i * ld + RHS];
     __syncthreads();
     for (int it = 0;; ++it) {
         double rn = 0.0;
-        for (int rho = threadIdx.x; rho < M; rho += NT) {
-            // (B x)_rho: row rho of A2, variables ascending, basic only
-            double acc = P.xb2[rho];
-            const int n = P.csr_n[rho];
-            for (int t0 = 0; t0 < n; t0 += MPQP_CB) {
+        for (int q = threadIdx.x; q < M * MPQP_GR; q += NT) {
+            // (xb2 - B x)_rho: row rho of A2, variables ascending, basic only;
+            // ELL layout interleaved by team lane (entry t of row rho at
+            // ((t / GR) * M + rho) * GR + t % GR)
+            const int rho = q / MPQP_GR, g = q % MPQP_GR;
+            double acc = g == 0 ? P.xb2[rho] : 0.0;
+            const unsigned ns = (unsigned)(P.csr_n[rho] + MPQP_GR - 1) / MPQP_GR;
+            const unsigned stride = (unsigned)(M * MPQP_GR);
+            for (unsigned s0 = 0; s0 < ns; s0 += MPQP_CB) {
                 int vv[MPQP_CB], ww[MPQP_CB];
                 double av[MPQP_CB], xv[MPQP_CB];
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) {
-                    const int t = t0 + u;
-                    vv[u] = t < n ? P.csr_var[t * M + rho] : -1;
-                    av[u] = t < n ? P.csr_val[t * M + rho] : 0.0;
+                    const unsigned s = s0 + u, e = (unsigned)q + s * stride;
+                    vv[u] = s < ns ? P.csr_var[e] : -1;
+                    av[u] = s < ns ? P.csr_val[e] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) ww[u] = vv[u] >= 0 ? L.where[vv[u]] : -1;
@@ -673,46 +715,41 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
                 for (int u = 0; u < MPQP_CB; ++u)
                     if (ww[u] >= 0 && !w_masked(P, inS, rho, vv[u])) acc = fma(-xv[u], av[u], acc);
             }
-            r[rho] = acc;
+            acc = team_add<MPQP_GR>(acc);
+            if (g == 0) r[rho] = acc;
             rn = fmax(rn, fabs(acc));
         }
         const double rmax = block_max<NT>(rn, red);
         OPT_PROF(sh, 8);
         if (rmax <= P.cert_ptol) return 1;
         if (it == MPQP_CERT_STEPS) return 0;
-        for (int i = threadIdx.x; i < M; i += NT) {
-            // (B32^-1 r)_i, rho ascending: the unit term of a basic e at
-            // position i slotted in among the nonbasic e-columns
+        for (int q = threadIdx.x; q < M * MPQP_GB; q += NT) {
+            // (B32^-1 r)_i: the unit term of a basic e at position i, then
+            // the nonbasic e-columns, rho ascending
+            const int i = q / MPQP_GB, g = q % MPQP_GB;
             const float *Ti = L.T + i * ld;
             const int bv = L.basis[i];
-            const int ru = bv >= e0 ? bv - e0 : M;
-            bool done = ru >= M;
-            const double rU = done ? 0.0 : r[ru];
-            double acc = 0.0;
-            constexpr int B = 4;  // LDS loads in flight
-            for (int k0 = 0; k0 < ne; k0 += B) {
-                int rk[B];
-                double rv[B], tv[B];
+            double acc = (g == 0 && bv >= e0) ? r[bv - e0] : 0.0;
+            for (int k0 = g; k0 < ne; k0 += MPQP_GB * MPQP_CBL) {
+                int rk[MPQP_CBL], sk[MPQP_CBL];
+                double rv[MPQP_CBL], tv[MPQP_CBL];
 #pragma unroll
-                for (int u = 0; u < B; ++u) rk[u] = k0 + u < ne ? L.bcnt[k0 + u] : M;
-#pragma unroll
-                for (int u = 0; u < B; ++u) {
-                    const bool in = k0 + u < ne;
-                    rv[u] = in ? r[rk[u]] : 0.0;
-                    tv[u] = in ? (double)Ti[L.nbcnt[k0 + u]] : 0.0;
+                for (int u = 0; u < MPQP_CBL; ++u) {
+                    const int k = k0 + u * MPQP_GB;
+                    rk[u] = k < ne ? L.bcnt[k] : -1;
+                    sk[u] = k < ne ? L.nbcnt[k] : 0;
                 }
 #pragma unroll
-                for (int u = 0; u < B; ++u) {
-                    if (k0 + u >= ne) break;
-                    if (!done && ru < rk[u]) {
-                        acc = fma(rU, 1.0, acc);
-                        done = true;
-                    }
-                    acc = fma(rv[u], tv[u], acc);
+                for (int u = 0; u < MPQP_CBL; ++u) {
+                    rv[u] = rk[u] >= 0 ? r[rk[u]] : 0.0;
+                    tv[u] = rk[u] >= 0 ? (double)Ti[sk[u]] : 0.0;
                 }
+#pragma unroll
+                for (int u = 0; u < MPQP_CBL; ++u)
+                    if (rk[u] >= 0) acc = fma(rv[u], tv[u], acc);
             }
-            if (!done) acc = fma(rU, 1.0, acc);
-            x[i] = x[i] + acc;
+            acc = team_add<MPQP_GB>(acc);
+            if (g == 0) x[i] = x[i] + acc;
         }
         __syncthreads();
         OPT_PROF(sh, 9);
@@ -721,10 +758,11 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
 
 // y (in L.cx) := refined solution of B^T y = c_B, c = cost(v), starting from
 // the FP32 cost row (y_rho = c(e_rho) - d(e_rho), d = 0 for a basic e: the
-// e-columns of A2 are the identity); 1 once max|c_B - B^T y| <= 2e-12
+// e-columns of A2 are the identity); 1 once max|c_B - B^T y| <= 2e-12.
+// ne: the e-list of cert_primal (same basis).
 template <int NT, typename CostF>
 __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
-                                         CostF cost, OptShared &sh) {
+                                         CostF cost, OptShared &sh, int ne) {
     const int M = P.M2, ld = L.ld, e0 = P.N2 - M;
     double *y = L.cx, *r = L.cr;
     for (int rho = threadIdx.x; rho < M; rho += NT) {
@@ -736,39 +774,44 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
     OPT_PROF(sh, 11);
     for (int it = 0;; ++it) {
         double rn = 0.0;
-        for (int i = threadIdx.x; i < M; i += NT) {
+        for (int q = threadIdx.x; q < M * MPQP_GC; q += NT) {
+            const int i = q / MPQP_GC, g = q % MPQP_GC;
             const int bv = L.basis[i];
-            const double acc = col_resid(P, inS, bv, cost(bv), y);
-            r[i] = acc;
+            double acc = col_resid(P, inS, bv, g, cost(bv), y);
+            acc = team_add<MPQP_GC>(acc);
+            if (g == 0) r[i] = acc;
             rn = fmax(rn, fabs(acc));
         }
         const double rmax = block_max<NT>(rn, red);
         OPT_PROF(sh, 12);
         if (rmax <= 2e-12) return 1;
         if (it == MPQP_CERT_STEPS) return 0;
-        for (int rho = threadIdx.x; rho < M; rho += NT) {
-            // (B32^-T r)_rho over positions ascending
-            const int w = L.where[e0 + rho];
+        for (int q = threadIdx.x; q < ne * MPQP_GT; q += NT) {
+            // (B32^-T r)_rho of a nonbasic e_rho: its tableau column,
+            // positions ascending
+            const int k = q / MPQP_GT, g = q % MPQP_GT;
+            const int rho = L.bcnt[k];
+            const float *Tk = L.T + L.nbcnt[k];
             double acc = 0.0;
-            if (w >= 0) {
-                acc = fma(r[w], 1.0, 0.0);
-            } else {
-                const float *Tk = L.T + (-w - 1);
-                constexpr int B = 4;  // LDS loads in flight
-                for (int i0 = 0; i0 < M; i0 += B) {
-                    double rv[B], tv[B];
+            for (int i0 = g; i0 < M; i0 += MPQP_GT * MPQP_CBL) {
+                double rv[MPQP_CBL], tv[MPQP_CBL];
 #pragma unroll
-                    for (int u = 0; u < B; ++u) {
-                        const bool in = i0 + u < M;
-                        rv[u] = in ? r[i0 + u] : 0.0;
-                        tv[u] = in ? (double)Tk[(i0 + u) * ld] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < B; ++u)
-                        if (i0 + u < M) acc = fma(rv[u], tv[u], acc);
+                for (int u = 0; u < MPQP_CBL; ++u) {
+                    const int i = i0 + u * MPQP_GT;
+                    rv[u] = i < M ? r[i] : 0.0;
+                    tv[u] = i < M ? (double)Tk[i * ld] : 0.0;
                 }
+#pragma unroll
+                for (int u = 0; u < MPQP_CBL; ++u)
+                    if (i0 + u * MPQP_GT < M) acc = fma(rv[u], tv[u], acc);
             }
-            y[rho] = y[rho] + acc;
+            acc = team_add<MPQP_GT>(acc);
+            if (g == 0) y[rho] = y[rho] + acc;
+        }
+        for (int i = threadIdx.x; i < M; i += NT) {
+            // basic e_rho at position i: unit column
+            const int bv = L.basis[i];
+            if (bv >= e0) y[bv - e0] = y[bv - e0] + r[i];
         }
         __syncthreads();
         OPT_PROF(sh, 13);
@@ -780,9 +823,13 @@ template <int NT, typename Sel, typename CostF>
 __device__ __forceinline__ double min_red_cost(const OptLds<float> &L, const DevProb &P, const unsigned *inS,
                                                double *red, Sel sel, CostF cost) {
     double mn = INFINITY;
-    for (int kk = threadIdx.x; kk < P.K2; kk += NT) {
+    for (int q = threadIdx.x; q < P.K2 * MPQP_GC; q += NT) {
+        const int kk = q / MPQP_GC, g = q % MPQP_GC;
         const int v = L.nonb[kk];
-        if (sel(v)) mn = fmin(mn, col_resid(P, inS, v, cost(v), L.cx));
+        if (sel(v)) {
+            const double d = team_add<MPQP_GC>(col_resid(P, inS, v, g, cost(v), L.cx));
+            mn = fmin(mn, d);
+        }
     }
     return -block_max<NT>(-mn, red);
 }
@@ -922,8 +969,8 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                 // artificial is basic, nonnegative reduced phase-1 costs
                 const int Nn = P.Nn;
                 auto cb1 = [Nn](int v) { return v >= Nn ? 1.0 : 0.0; };
-                int ok = p1.status == ST_OPTIMAL;
-                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh);
+                int ok = p1.status == ST_OPTIMAL, ne = 0;
+                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, &ne);
                 if (ok) ok = cert_xmin<NT>(L, M, red) >= O.dual_stop_tol;
                 if (ok) {
                     p1.z1 = cert_z<NT, S>(L, P, red, cb1);
@@ -931,7 +978,7 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
                     OPT_PROF(sh, 10);
                     if (sh.nrows > 0) {
-                        ok = cert_dual<NT>(L, P, inS, red, cb1, sh);
+                        ok = cert_dual<NT>(L, P, inS, red, cb1, sh, ne);
                         if (ok)
                             ok = min_red_cost<NT>(L, P, inS, red, [](int) { return true; }, cb1) >= -O.certify_tol;
                         OPT_PROF(sh, 14);
@@ -960,13 +1007,13 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                     // nonnegative reduced costs of the first N_ variables
                     const int EPS = P.n_x + P.n_t + P.n_c, Nn = P.Nn;
                     auto cb2 = [EPS](int v) { return v == EPS ? -1.0 : 0.0; };
-                    int ok = p2.status == ST_OPTIMAL;
-                    if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh);
+                    int ok = p2.status == ST_OPTIMAL, ne = 0;
+                    if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, &ne);
                     if (ok) ok = cert_xmin<NT>(L, M, red) >= -O.certify_tol;
                     if (ok) {
                         p2.z2 = cert_z<NT, S>(L, P, red, cb2);
                         p2.final_set = -p2.z2 > O.final_tol ? 1 : 0;
-                        ok = cert_dual<NT>(L, P, inS, red, cb2, sh);
+                        ok = cert_dual<NT>(L, P, inS, red, cb2, sh, ne);
                     }
                     if (ok)
                         ok = min_red_cost<NT>(L, P, inS, red, [Nn](int v) { return v < Nn; }, cb2) >= -O.certify_tol;

@@ -536,11 +536,26 @@ void oracle_cert_stats(long long *out, int reset) {
     for (int i = 0; i < 16; i++) { out[i] = g_cert_fail[i]; if (reset) g_cert_fail[i] = 0; }
 }
 
-/* Every certification product below is a sequential fma chain in a fixed
- * order (the GPU computes each output entry on one thread, in the same
- * order).  A2 is read through its CSR rows (variables ascending) and CSC
- * columns (rows ascending); masked entries and nonbasic variables are
- * skipped. */
+/* Every certification product entry is a team sum over an ordered term
+ * list: A2 through its CSR rows (variables ascending) or CSC columns (rows
+ * ascending), B^-1 through the nonbasic e-columns (rho ascending) or the
+ * positions (ascending).  Term t of the list goes to partial t % G, each
+ * partial is a sequential fma chain (partial 0 starts from the constant
+ * term, the others from +0.0; masked entries and nonbasic variables are
+ * skipped but keep their place t), and the G partials are added by the
+ * butterfly p0+p1 (G = 2) or (p0+p1)+(p2+p3) (G = 4).  The GPU computes
+ * the partials on G adjacent lanes and adds them by lane exchange (xor 1,
+ * then xor 2), which gives the same sums bit for bit. */
+#define CERT_GR 2 /* rows of A2: primal residual */
+#define CERT_GC 4 /* columns of A2: dual residual, reduced costs */
+#define CERT_GB 2 /* rows of B^-1: primal correction */
+#define CERT_GT 4 /* columns of B^-1: dual correction */
+
+static double team_total(const double *p, int G) {
+    if (G == 1) return p[0];
+    if (G == 2) return p[0] + p[1];
+    return (p[0] + p[1]) + (p[2] + p[3]);
+}
 
 /* true if entry (rho, v) of A2 is masked to zero for S (:635-650) */
 static int w_masked(const Prob *P, const char *inS, int rho, int v) {
@@ -557,41 +572,44 @@ static double cert_cost(const Prob *P, int phase, int v) {
     return v == EPS_VAR(P) ? -1.0 : 0.0;
 }
 
-/* (B32^-1 r)_i over rho ascending: e_rho basic at position i contributes
- * r[rho] (unit column), e_rho basic elsewhere nothing, e_rho nonbasic
- * r[rho] * T32[i][slot of e_rho]. */
+/* (B32^-1 r)_i: the unit term r[rho] when e_rho is basic at position i
+ * (starts partial 0), then r[rho] * T32[i][slot of e_rho] over the
+ * nonbasic e_rho, rho ascending (term k = rank among them). */
 static double binv_row(const Prob *P, const OSt_f *O, const int *where, const double *r, int i) {
-    int M = P->M2, ld = P->ld2;
-    double acc = 0.0;
+    int M = P->M2, ld = P->ld2, e0 = P->N2 - M, bv = O->basis[i];
+    double p[4] = {bv >= e0 ? r[bv - e0] : 0.0, 0.0, 0.0, 0.0};
+    int k = 0;
     for (int rho = 0; rho < M; rho++) {
-        int w = where[P->N2 - M + rho];
-        if (w >= 0) {
-            if (w == i) acc = fma(r[rho], 1.0, acc);
-        } else {
-            acc = fma(r[rho], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
-        }
+        int w = where[e0 + rho];
+        if (w >= 0) continue;
+        int g = k++ % CERT_GB;
+        p[g] = fma(r[rho], (double)O->T[(size_t)i * ld + (-w - 1)], p[g]);
     }
-    return acc;
+    return team_total(p, CERT_GB);
 }
 
-/* (B32^-T s)_rho over positions i ascending */
+/* (B32^-T s)_rho: s[w] when e_rho is basic at position w, else the sum
+ * over positions i ascending of s[i] * T32[i][slot of e_rho] */
 static double binvT_col(const Prob *P, const OSt_f *O, const int *where, const double *s, int rho) {
     int M = P->M2, ld = P->ld2;
     int w = where[P->N2 - M + rho];
-    if (w >= 0) return fma(s[w], 1.0, 0.0);
-    double acc = 0.0;
-    for (int i = 0; i < M; i++) acc = fma(s[i], (double)O->T[(size_t)i * ld + (-w - 1)], acc);
-    return acc;
+    if (w >= 0) return s[w];
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < M; i++) {
+        int g = i % CERT_GT;
+        p[g] = fma(s[i], (double)O->T[(size_t)i * ld + (-w - 1)], p[g]);
+    }
+    return team_total(p, CERT_GT);
 }
 
 /* c - sum over column v of A2 (rows ascending, masked skipped) of a * y[row] */
 static double col_resid(const Prob *P, const char *inS, int v, double c, const double *y) {
-    double acc = c;
+    double p[4] = {c, 0.0, 0.0, 0.0};
     for (int e = P->csc_ptr[v]; e < P->csc_ptr[v + 1]; e++) {
-        int rho = P->csc_row[e];
-        if (!w_masked(P, inS, rho, v)) acc = fma(-P->csc_val[e], y[rho], acc);
+        int rho = P->csc_row[e], g = (e - P->csc_ptr[v]) % CERT_GC;
+        if (!w_masked(P, inS, rho, v)) p[g] = fma(-P->csc_val[e], y[rho], p[g]);
     }
-    return acc;
+    return team_total(p, CERT_GC);
 }
 
 /* x := FP64 solution of B x = xb2 by iterative refinement from the FP32
@@ -608,11 +626,12 @@ static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const ch
     for (int it = 0;; it++) {
         double rn = 0.0;
         for (int rho = 0; rho < M; rho++) {
-            double acc = P->xb2[rho];
+            double p[4] = {P->xb2[rho], 0.0, 0.0, 0.0};
             for (int e = P->csr_ptr[rho]; e < P->csr_ptr[rho + 1]; e++) {
-                int v = P->csr_var[e], w = where[v];
-                if (w >= 0 && !w_masked(P, inS, rho, v)) acc = fma(-x[w], P->csr_val[e], acc);
+                int v = P->csr_var[e], w = where[v], g = (e - P->csr_ptr[rho]) % CERT_GR;
+                if (w >= 0 && !w_masked(P, inS, rho, v)) p[g] = fma(-x[w], P->csr_val[e], p[g]);
             }
+            double acc = team_total(p, CERT_GR);
             r[rho] = acc;
             if (fabs(acc) > rn) rn = fabs(acc);
         }
