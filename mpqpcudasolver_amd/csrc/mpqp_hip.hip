@@ -182,6 +182,53 @@ struct DBuf {
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
 
+// A view of a pooled device buffer.
+struct BufView {
+    void *p = nullptr;
+    size_t n = 0;
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Device buffers reused across levels and calls (the per-level candidate
+// arrays, records and stored states): grown on demand, never shrunk, held
+// until the process ends.
+struct PoolBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    BufView get(size_t bytes) {
+        if (bytes > cap) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+            HIP_OK(hipMalloc(&p, bytes));
+            cap = bytes;
+        }
+        return BufView{bytes ? p : nullptr, bytes};
+    }
+};
+enum { PB_SETS, PB_PAR, PB_REC, PB_LIST, PB_P1, PB_FBL, PB_FB, PB_B1, PB_B2, PB_FS, PB_OS = PB_FS + 2, PB_N = PB_OS + 2 };
+PoolBuf &pool(int k) {
+    static PoolBuf b[PB_N];
+    return b[k];
+}
+
+// Pinned host staging for the per-level records (allocated with
+// hipHostMalloc, grown on demand).
+template <typename T>
+T *pinned_records(size_t count) {
+    static void *p = nullptr;
+    static size_t cap = 0;
+    const size_t bytes = count * sizeof(T);
+    if (bytes > cap) {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        HIP_OK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        cap = bytes;
+    }
+    return static_cast<T *>(p);
+}
+
 // HIP event with RAII (released on every exit path, errors included).
 struct Event {
     hipEvent_t e = nullptr;
@@ -334,19 +381,29 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
 
     // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification:
     // rows (variables ascending) in an ELL layout interleaved for teams of
-    // GR lanes, entry t of row rho at ((t / GR) * M2 + rho) * GR + t % GR;
-    // columns (rows ascending) contiguous, each starting at a multiple of
-    // the column team size GC
-    constexpr int GR = MPQP_GR, GC = MPQP_GC;
-    std::vector<int> rn(M2, 0), cn(N2, 0), cptr(N2, 0);
+    // GR lanes, entry t of row rho at ((t / GR) * M2 + rho) * GR + t % GR,
+    // csr_n[rho] = the step count (entries per lane) of the row's wave slice
+    // rounded up to whole batches of MPQP_CB; columns (rows ascending)
+    // contiguous, each starting at a multiple of the column team size GC and
+    // padded to whole steps.  Padding entries: index -1, value 0.
+    constexpr int GR = MPQP_GR, GC = MPQP_GC, CB = MPQP_CB, SL = MPQP_WAVE / GR;
+    std::vector<int> rn(M2, 0), cn(N2, 0), cptr(N2, 0), rsteps(M2, 0);
     for (int v = 0; v < N2; ++v)
         for (int rho = 0; rho < M2; ++rho)
             if (A2[(size_t)v * M2 + rho] != 0.0) {
                 rn[rho]++;
                 cn[v]++;
             }
-    int rw = GR, csz = 0;
-    for (int rho = 0; rho < M2; ++rho) rw = std::max(rw, (rn[rho] + GR - 1) / GR * GR);
+    int maxsteps = CB, csz = 0;
+    for (int r0 = 0; r0 < M2; r0 += SL) {
+        const int r1 = std::min(M2, r0 + SL);
+        int stp = 0;
+        for (int rho = r0; rho < r1; ++rho) stp = std::max(stp, (rn[rho] + GR - 1) / GR);
+        stp = (stp + CB - 1) / CB * CB;
+        for (int rho = r0; rho < r1; ++rho) rsteps[rho] = stp;
+        maxsteps = std::max(maxsteps, stp);
+    }
+    const int rw = maxsteps * GR;
     for (int v = 0; v < N2; ++v) {
         cptr[v] = csz;
         csz += std::max(GC, (cn[v] + GC - 1) / GC * GC);
@@ -378,7 +435,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         P.csc_row = P.csr_var + rvar.size();
         P.csr_val = db0;
         P.csc_val = db0 + rval.size();
-        HIP_OK(hipMemcpyAsync((void *)P.csr_n, rn.data(), sizeof(int) * rn.size(), hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync((void *)P.csr_n, rsteps.data(), sizeof(int) * rsteps.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csc_n, cn.data(), sizeof(int) * cn.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csc_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice, st));
         HIP_OK(hipMemcpyAsync((void *)P.csr_var, rvar.data(), sizeof(int) * rvar.size(), hipMemcpyHostToDevice, st));
@@ -476,6 +533,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
 
     std::vector<MpqpRecord> all;
     std::vector<int> allsets, allfb, allb1, allb2;
+    std::vector<int> fin;  // final active sets, rows of kw ints
     const bool keep_rec = o.record >= 1, keep_bases = o.record >= 2;
     int n_batches = 0, n_feas_total = 0;
     double tf = 0.0, to = 0.0;
@@ -489,6 +547,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     for (int attempt = 0; attempt < 2; ++attempt) {
         all.clear();
         allsets.clear();
+        fin.clear();
         allfb.clear();
         allb1.clear();
         allb2.clear();
@@ -502,10 +561,10 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
 
         std::vector<int> prev_sets, prev_feas, prev_warm;
         std::vector<int> prev_recidx;
-        DBuf fstore_prev, ostore_prev;
+        BufView fstore_prev, ostore_prev;
         int prev_n = 0;
         std::vector<int> cur_sets, cur_par;
-        std::vector<DevRecord> hrec;
+        DevRecord *hrec = nullptr;
 
         for (int lvl = 0; lvl <= last_lvl; ++lvl) {
             cur_sets.clear();
@@ -514,14 +573,21 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 cur_sets.assign(kw, -1);
                 cur_par.push_back(-1);
             } else {
+                // children of the feasible parents: S + {j}, j above max(S)
+                size_t nch = 0;
+                for (int pi = 0; pi < prev_n; ++pi)
+                    if (prev_feas[pi]) nch += (size_t)(n_c - 1 - (lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1));
+                cur_sets.resize(nch * kw);
+                cur_par.resize(nch);
+                size_t c = 0;
                 for (int pi = 0; pi < prev_n; ++pi) {
                     if (!prev_feas[pi]) continue;
-                    const int mx = lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1;
-                    for (int j = mx + 1; j < n_c; ++j) {
-                        for (int t = 0; t < kw; ++t)
-                            cur_sets.push_back(t < lvl - 1 ? prev_sets[(size_t)pi * kw + t]
-                                                           : (t == lvl - 1 ? j : -1));
-                        cur_par.push_back(pi);
+                    const int *ps = prev_sets.data() + (size_t)pi * kw;
+                    const int mx = lvl >= 2 ? ps[lvl - 2] : -1;
+                    for (int j = mx + 1; j < n_c; ++j, ++c) {
+                        int *cs = cur_sets.data() + c * kw;
+                        for (int t = 0; t < kw; ++t) cs[t] = t < lvl - 1 ? ps[t] : (t == lvl - 1 ? j : -1);
+                        cur_par[c] = pi;
                     }
                 }
             }
@@ -531,20 +597,22 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             const bool store = lvl < last_lvl;
             const bool bases = keep_bases && !probe_only;
 
-            DBuf dsets(sizeof(int) * (size_t)n * kw), dpar(sizeof(int) * (size_t)n);
-            DBuf drec(sizeof(DevRecord) * (size_t)n);
+            const BufView dsets = pool(PB_SETS).get(sizeof(int) * (size_t)n * kw);
+            const BufView dpar = pool(PB_PAR).get(sizeof(int) * (size_t)n);
+            const BufView drec = pool(PB_REC).get(sizeof(DevRecord) * (size_t)n);
             HIP_OK(hipMemcpyAsync(dsets.p, cur_sets.data(), dsets.n, hipMemcpyHostToDevice, st));
             HIP_OK(hipMemcpyAsync(dpar.p, cur_par.data(), dpar.n, hipMemcpyHostToDevice, st));
-            DBuf fstore_cur, ostore_cur;
+            // stored states ping-pong between two pooled buffers by level parity
+            BufView fstore_cur, ostore_cur;
             if (store) {
-                fstore_cur.alloc((size_t)fstride * n);
-                ostore_cur.alloc((size_t)ostride * n);
+                fstore_cur = pool(PB_FS + (lvl & 1)).get((size_t)fstride * n);
+                ostore_cur = pool(PB_OS + (lvl & 1)).get((size_t)ostride * n);
             }
-            DBuf dfb, db1, db2;
+            BufView dfb, db1, db2;
             if (bases) {
-                dfb.alloc(sizeof(int) * (size_t)n * M1);
-                db1.alloc(sizeof(int) * (size_t)n * M2);
-                db2.alloc(sizeof(int) * (size_t)n * M2);
+                dfb = pool(PB_FB).get(sizeof(int) * (size_t)n * M1);
+                db1 = pool(PB_B1).get(sizeof(int) * (size_t)n * M2);
+                db2 = pool(PB_B2).get(sizeof(int) * (size_t)n * M2);
                 const int nb1 = n * M2;
                 hipLaunchKernelGGL(fill_i32, dim3((nb1 + 255) / 256), dim3(256), 0, st, db1.as<int>(), nb1, -1);
                 hipLaunchKernelGGL(fill_i32, dim3((nb1 + 255) / 256), dim3(256), 0, st, db2.as<int>(), nb1, -1);
@@ -569,8 +637,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             hipLaunchKernelGGL(feas_kernel<FEAS_NT>, dim3(n), dim3(FEAS_NT), (size_t)flds, st, fa);
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(e1, st));
-            hrec.resize(n);
-            HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
+            hrec = pinned_records<DevRecord>((size_t)n);
+            HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
             HIP_OK(hipStreamSynchronize(st));
             float ms = 0.f;
             HIP_OK(hipEventElapsedTime(&ms, e0, e1));
@@ -588,7 +656,9 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             const int nf = (int)list.size();
             int n_fb = 0;
             if (nf > 0) {
-                DBuf dlist(sizeof(int) * (size_t)nf), dp1(sizeof(int) * (size_t)n), dfbl;
+                const BufView dlist = pool(PB_LIST).get(sizeof(int) * (size_t)nf);
+                const BufView dp1 = pool(PB_P1).get(sizeof(int) * (size_t)n);
+                BufView dfbl;
                 HIP_OK(hipMemcpyAsync(dlist.p, list.data(), dlist.n, hipMemcpyHostToDevice, st));
                 HIP_OK(hipMemcpyAsync(dp1.p, p1par.data(), dp1.n, hipMemcpyHostToDevice, st));
                 hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
@@ -611,7 +681,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 oa.b2 = bases ? db2.as<int>() : nullptr;
                 HIP_OK(hipEventRecord(e1, st));
                 if (mixed) {
-                    dfbl.alloc(sizeof(int) * (size_t)nf);
+                    dfbl = pool(PB_FBL).get(sizeof(int) * (size_t)nf);
                     oa.counter = dctr.as<int>();
                     oa.fb_list = dfbl.as<int>();
                     oa.fb_count = dctr.as<int>() + 1;
@@ -640,7 +710,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     launch(fn64, nf, lds64, oa);
                 }
                 HIP_OK(hipEventRecord(e2, st));
-                HIP_OK(hipMemcpyAsync(hrec.data(), drec.p, drec.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
                 HIP_OK(hipStreamSynchronize(st));
                 HIP_OK(hipEventElapsedTime(&ms, e1, e2));
                 to += ms * 1e-3;
@@ -705,14 +775,20 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             // a phase-1 state warm-starts the children whenever phase 1 ended optimal
             std::vector<int> warm_ok(n);
             for (int c = 0; c < n; ++c) {
-                MpqpRecord r;
-                std::memcpy(&r, &hrec[c], sizeof r);
-                r.level = lvl;
-                r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
-                all.push_back(r);
-                warm_ok[c] = r.p1_status == ST_OPTIMAL;
+                warm_ok[c] = hrec[c].p1_status == ST_OPTIMAL;
+                if (hrec[c].final_set && lvl >= k_min)
+                    fin.insert(fin.end(), cur_sets.begin() + (size_t)c * kw, cur_sets.begin() + (size_t)(c + 1) * kw);
             }
-            allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
+            if (keep_rec) {
+                all.resize((size_t)base + n);
+                for (int c = 0; c < n; ++c) {
+                    MpqpRecord &r = all[(size_t)base + c];
+                    std::memcpy(&r, &hrec[c], sizeof r);
+                    r.level = lvl;
+                    r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
+                }
+                allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
+            }
             if (bases) {
                 size_t o1 = allfb.size(), o2 = allb1.size();
                 allfb.resize(o1 + (size_t)n * M1);
@@ -730,8 +806,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             prev_recidx.resize(n);
             for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
             prev_n = n;
-            fstore_prev = std::move(fstore_cur);
-            ostore_prev = std::move(ostore_cur);
+            fstore_prev = fstore_cur;
+            ostore_prev = ostore_cur;
         }
         if (!restart) break;
         mixed = false;
@@ -741,18 +817,11 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     const double t_gpu = now_s() - t_start;
 
     GPUResults *res = static_cast<GPUResults *>(std::calloc(1, sizeof(GPUResults)));
-    int nfin = 0;
-    for (auto &r : all)
-        if (r.final_set && r.level >= k_min) ++nfin;
+    const int nfin = (int)(fin.size() / (size_t)kw);
     res->active_sets = static_cast<int *>(std::malloc(sizeof(int) * (size_t)std::max(nfin, 1) * n_x));
-    int row = 0;
-    for (size_t c = 0; c < all.size(); ++c) {
-        if (!(all[c].final_set && all[c].level >= k_min)) continue;
+    for (int row = 0; row < nfin; ++row)
         for (int t = 0; t < n_x; ++t)
-            res->active_sets[(size_t)row * n_x + t] =
-                (t < all[c].level && t < kw) ? allsets[c * kw + t] : -1;
-        ++row;
-    }
+            res->active_sets[(size_t)row * n_x + t] = t < kw ? fin[(size_t)row * kw + t] : -1;
     res->n_batches = n_batches;
     res->n_splits = 1;
     res->n_feasible_lps = n_feas_total;

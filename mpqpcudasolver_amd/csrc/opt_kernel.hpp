@@ -19,13 +19,15 @@
 __host__ __device__ inline int opt_ld(int K) { return (K + 2) | 1; }
 
 // LDS of one workgroup: T (M+1) x ld (unless the tableau lives in global
-// memory: gt), colq (M+1), vcol (M) in Real; basis, nonb, where, bcnt,
-// nbcnt, rows in int; for certification (FP32 pivots) x and r in double.
+// memory: gt), colq (M+1), vcol (M) in Real; then (8-byte aligned) bcnt,
+// nbcnt, basis, nonb, where, rows in int; for certification (FP32 pivots)
+// x and r in double.  During certification bcnt/nbcnt hold x by variable
+// (N2 doubles) and colq/vcol the list of nonbasic e-variables.
 template <typename Real>
 __host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, bool gt = false) {
     const int ld = opt_ld(K);
-    long long b = (long long)((gt ? 0 : (M + 1) * ld) + (M + 1) + M) * (long long)sizeof(Real) +
-                  (long long)(M + K + 3 * N2 + M + 1) * 4;
+    long long b = (long long)((gt ? 0 : (M + 1) * ld) + (M + 1) + M) * (long long)sizeof(Real);
+    b = ((b + 7) & ~7LL) + (long long)(M + K + 3 * N2 + M + 1) * 4;
     if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
     return (b + 15) & ~15LL;
 }
@@ -88,23 +90,29 @@ struct OptLds {
     Real *T, *colq, *vcol;
     int *basis, *nonb, *where, *bcnt, *nbcnt, *rows;
     double *cx, *cr;  // certification vectors (FP32 pivots only)
+    double *xv;       // certification: x by variable (over bcnt / nbcnt)
+    int *elr, *els;   // certification: nonbasic e-list rho / slot (over colq / vcol)
     // gT: the tableau in global memory, or null for the LDS tableau
     __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT) {
         ld = opt_ld(K);
         T = gT ? gT : reinterpret_cast<Real *>(smem);
         colq = gT ? reinterpret_cast<Real *>(smem) : T + (M + 1) * ld;
         vcol = colq + (M + 1);
-        basis = reinterpret_cast<int *>(vcol + M);
+        const size_t ib = ((size_t)(reinterpret_cast<unsigned char *>(vcol + M) - smem) + 7) & ~(size_t)7;
+        bcnt = reinterpret_cast<int *>(smem + ib);
+        nbcnt = bcnt + N2;
+        basis = nbcnt + N2;
         nonb = basis + M;
         where = nonb + K;
-        bcnt = where + N2;
-        nbcnt = bcnt + N2;
-        rows = nbcnt + N2;
-        cx = cr = nullptr;
+        rows = where + N2;
+        cx = cr = xv = nullptr;
+        elr = reinterpret_cast<int *>(colq);
+        els = reinterpret_cast<int *>(vcol);
         if (cert) {
             const size_t off = ((size_t)(reinterpret_cast<unsigned char *>(rows + M + 1) - smem) + 7) & ~(size_t)7;
             cx = reinterpret_cast<double *>(smem + off);
             cr = cx + M;
+            xv = reinterpret_cast<double *>(bcnt);
         }
     }
 };
@@ -563,14 +571,6 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P
 // ---------------------------------------------------------------------
 #define MPQP_CERT_STEPS 3
 
-// true if entry (rho, v) of A2 is masked to zero for the current S
-__device__ __forceinline__ bool w_masked(const DevProb &P, const unsigned *inS, int rho, int v) {
-    const int j = v - (P.n_x + P.n_t);
-    if ((unsigned)j >= (unsigned)P.n_c) return false;
-    if ((inS[j >> 5] >> (j & 31)) & 1u) return rho == P.M2 - P.n_c + j;
-    return rho >= P.M2 - P.n_c - P.n_x && rho < P.M2 - P.n_c;
-}
-
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double *red) {
     constexpr int NW = NT / MPQP_WAVE;
@@ -592,10 +592,11 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 // constant term, the others from +0.0; masked entries and nonbasic
 // variables are skipped in place), and the team adds its partials by lane
 // exchange, xor 1 then xor 2, so every lane ends with (p0+p1)+(p2+p3).
-// Each lane keeps MPQP_CB (A2, global memory) or MPQP_CBL (tableau, LDS)
-// terms in flight.  Team sizes: rows of A2 2,
-// columns of A2 4, rows of B^-1 2, columns of B^-1 4 (a team never
-// straddles a wave: G divides 64).
+// Team sizes: rows of A2 2, columns of A2 4, rows of B^-1 2, columns of
+// B^-1 4 (a team never straddles a wave: G divides 64).  The sparse walks
+// are padded so that their loads need no per-entry guard: the rows of one
+// wave slice share a step count, a multiple of MPQP_CB; columns are padded
+// to whole steps.
 #define MPQP_CB 4   // A2 entries in flight per lane
 #define MPQP_CBL 4  // tableau entries in flight per lane
 #define MPQP_GR 2
@@ -623,9 +624,62 @@ __device__ __forceinline__ double team_add(double v) {
     return v;
 }
 
-// Compact list of the nonbasic e-variables, rho ascending: elist_rho[k],
-// elist_slot[k] (in the phase-1 use counters, free during certification).
-// Returns their count.
+// |r| for a convergence test; NaN counts as +inf (never certified)
+__device__ __forceinline__ double res_mag(double r) {
+    const double a = fabs(r);
+    return a == a ? a : INFINITY;
+}
+
+// The S-membership bits of the w columns (n_c <= 256), in four 64-bit
+// registers; has() is plain shift/mask arithmetic (no indexed register
+// array, which the compiler would place in scratch memory).
+struct SBits {
+    unsigned long long m0, m1, m2, m3;
+    __device__ __forceinline__ static unsigned long long word(const unsigned *inS, int k) {
+        const unsigned lo = __builtin_amdgcn_readfirstlane(inS[2 * k]);
+        const unsigned hi = __builtin_amdgcn_readfirstlane(inS[2 * k + 1]);
+        return ((unsigned long long)hi << 32) | lo;
+    }
+    __device__ __forceinline__ void load(const unsigned *inS) {
+        m0 = word(inS, 0);
+        m1 = word(inS, 1);
+        m2 = word(inS, 2);
+        m3 = word(inS, 3);
+    }
+    __device__ __forceinline__ bool has(int j) const {
+        const unsigned sh = (unsigned)j & 63u;
+        const unsigned q = (unsigned)j >> 6;
+        const unsigned long long b = ((m0 >> sh) & (unsigned long long)(q == 0u)) |
+                                     ((m1 >> sh) & (unsigned long long)(q == 1u)) |
+                                     ((m2 >> sh) & (unsigned long long)(q == 2u)) |
+                                     ((m3 >> sh) & (unsigned long long)(q == 3u));
+        return (b & 1ull) != 0ull;
+    }
+};
+
+// The S-mask of the w columns (initialise_A_kernel_1,
+// batched_simplex.cu:635-650) for one column v (a team's column walk):
+// entry (rho, v) of A2 is zero for S iff v = w_j and (j in S ? rho is the
+// row of w_j : rho is a stationarity row), i.e. iff isw && (in ? rho ==
+// jrow : xb0 <= rho < xb1).
+struct ColMask {
+    bool isw, in;
+    int jrow, xb0, xb1;
+    __device__ __forceinline__ ColMask(const DevProb &P, const SBits &sb, int v) {
+        const int j = v - (P.n_x + P.n_t);
+        isw = (unsigned)j < (unsigned)P.n_c;
+        in = isw && sb.has(isw ? j : 0);
+        jrow = P.M2 - P.n_c + j;
+        xb0 = P.M2 - P.n_c - P.n_x;
+        xb1 = P.M2 - P.n_c;
+    }
+    __device__ __forceinline__ bool masked(int rho) const {
+        return isw && (in ? rho == jrow : (rho >= xb0 && rho < xb1));
+    }
+};
+
+// Compact list of the nonbasic e-variables, rho ascending: L.elr[k] (rho),
+// L.els[k] (slot).  Returns their count.
 template <int S>
 __device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, OptShared &sh) {
     const int M = P.M2, e0 = P.N2 - M;
@@ -640,8 +694,8 @@ __device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, O
             const unsigned long long m = __ballot(f);
             if (f) {
                 const int k = base + __popcll(m & ((1ull << lane) - 1ull));
-                L.bcnt[k] = rho;
-                L.nbcnt[k] = -w - 1;
+                L.elr[k] = rho;
+                L.els[k] = -w - 1;
             }
             base += __popcll(m);
         }
@@ -655,69 +709,95 @@ __device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, O
 
 // Lane g's partial of c - sum over column v of A2 (rows ascending, masked
 // skipped) of a * y[row]: entries t = g, g + MPQP_GC, ... (c on lane 0).
-__device__ __forceinline__ double col_resid(const DevProb &P, const unsigned *inS, int v, int g, double c,
+__device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, int v, int g, double c,
                                             const double *y) {
     double acc = g == 0 ? c : 0.0;
-    const unsigned n = (unsigned)P.csc_n[v], base = (unsigned)P.csc_ptr[v];
-    for (unsigned t0 = (unsigned)g; t0 < n; t0 += MPQP_GC * MPQP_CB) {
+    const ColMask cm(P, sb, v);
+    const unsigned ns = ((unsigned)P.csc_n[v] + MPQP_GC - 1) / MPQP_GC;
+    const unsigned base = (unsigned)P.csc_ptr[v] + (unsigned)g;
+    unsigned s = 0;
+    for (; s + MPQP_CB <= ns; s += MPQP_CB) {
         int rr[MPQP_CB];
         double av[MPQP_CB], yv[MPQP_CB];
 #pragma unroll
         for (int u = 0; u < MPQP_CB; ++u) {
-            const unsigned t = t0 + u * MPQP_GC;
-            rr[u] = t < n ? P.csc_row[base + t] : -1;
-            av[u] = t < n ? P.csc_val[base + t] : 0.0;
+            const unsigned e = base + (s + u) * MPQP_GC;
+            rr[u] = P.csc_row[e];
+            av[u] = P.csc_val[e];
         }
 #pragma unroll
-        for (int u = 0; u < MPQP_CB; ++u) yv[u] = rr[u] >= 0 ? y[rr[u]] : 0.0;
+        for (int u = 0; u < MPQP_CB; ++u) yv[u] = y[rr[u] >= 0 ? rr[u] : 0];
 #pragma unroll
-        for (int u = 0; u < MPQP_CB; ++u)
-            if (rr[u] >= 0 && !w_masked(P, inS, rr[u], v)) acc = fma(-av[u], yv[u], acc);
+        for (int u = 0; u < MPQP_CB; ++u) {
+            const double t = fma(-av[u], yv[u], acc);
+            acc = (rr[u] >= 0 && !cm.masked(rr[u])) ? t : acc;
+        }
+    }
+    for (; s < ns; ++s) {
+        const unsigned e = base + s * MPQP_GC;
+        const int rho = P.csc_row[e];
+        const double t = fma(-P.csc_val[e], y[rho >= 0 ? rho : 0], acc);
+        acc = (rho >= 0 && !cm.masked(rho)) ? t : acc;
     }
     return acc;
 }
 
-// x (LDS) := refined solution of B x = xb2; 1 once max|xb2 - B x| <=
-// cert_ptol.  *ne_out: the number of nonbasic e-variables (the e-list).
+// x (LDS, by position; L.xv by variable, NaN when nonbasic, a NaN value of
+// a basic variable stored as +inf) := refined solution of B x = xb2; 1 once
+// max|xb2 - B x| <= cert_ptol.  *ne_out: the number of nonbasic e-variables.
 template <int NT, int S>
 __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
                                            OptShared &sh, int *ne_out) {
-    const int M = P.M2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
+    const int M = P.M2, N2 = P.N2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
     double *x = L.cx, *r = L.cr;
     const int ne = elist_build<S>(L, P, sh);
     *ne_out = ne;
-    for (int i = threadIdx.x; i < M; i += NT) x[i] = (double)L.T[i * ld + RHS];
+    SBits sb;
+    sb.load(inS);
+    const int woff = P.n_x + P.n_t, n_c = P.n_c;
+    const int xb0 = M - n_c - P.n_x, xb1 = M - n_c;
+    for (int v = threadIdx.x; v < N2; v += NT) {
+        const int w = L.where[v];
+        const double xw = w >= 0 ? (double)L.T[w * ld + RHS] : 0.0;
+        L.xv[v] = w >= 0 ? (xw == xw ? xw : INFINITY) : __builtin_nan("");
+        if (w >= 0) x[w] = xw;
+    }
     __syncthreads();
     for (int it = 0;; ++it) {
         double rn = 0.0;
         for (int q = threadIdx.x; q < M * MPQP_GR; q += NT) {
             // (xb2 - B x)_rho: row rho of A2, variables ascending, basic only;
-            // ELL layout interleaved by team lane (entry t of row rho at
-            // ((t / GR) * M + rho) * GR + t % GR)
+            // entry t of row rho at ((t / GR) * M + rho) * GR + t % GR, csr_n
+            // the (padded) step count of the row's wave slice
             const int rho = q / MPQP_GR, g = q % MPQP_GR;
             double acc = g == 0 ? P.xb2[rho] : 0.0;
-            const unsigned ns = (unsigned)(P.csr_n[rho] + MPQP_GR - 1) / MPQP_GR;
+            const unsigned ns = (unsigned)P.csr_n[rho];
             const unsigned stride = (unsigned)(M * MPQP_GR);
+            const bool xrow = rho >= xb0 && rho < xb1;
+            const int jr = rho - xb1;  // the w column whose own row this is
             for (unsigned s0 = 0; s0 < ns; s0 += MPQP_CB) {
-                int vv[MPQP_CB], ww[MPQP_CB];
+                int vv[MPQP_CB];
                 double av[MPQP_CB], xv[MPQP_CB];
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) {
-                    const unsigned s = s0 + u, e = (unsigned)q + s * stride;
-                    vv[u] = s < ns ? P.csr_var[e] : -1;
-                    av[u] = s < ns ? P.csr_val[e] : 0.0;
+                    const unsigned e = (unsigned)q + (s0 + u) * stride;
+                    vv[u] = P.csr_var[e];
+                    av[u] = P.csr_val[e];
                 }
 #pragma unroll
-                for (int u = 0; u < MPQP_CB; ++u) ww[u] = vv[u] >= 0 ? L.where[vv[u]] : -1;
+                for (int u = 0; u < MPQP_CB; ++u) xv[u] = L.xv[vv[u] >= 0 ? vv[u] : 0];
 #pragma unroll
-                for (int u = 0; u < MPQP_CB; ++u) xv[u] = ww[u] >= 0 ? x[ww[u]] : 0.0;
-#pragma unroll
-                for (int u = 0; u < MPQP_CB; ++u)
-                    if (ww[u] >= 0 && !w_masked(P, inS, rho, vv[u])) acc = fma(-xv[u], av[u], acc);
+                for (int u = 0; u < MPQP_CB; ++u) {
+                    const int j = vv[u] - woff;
+                    const bool isw = (unsigned)j < (unsigned)n_c;
+                    const bool masked = isw && (sb.has(isw ? j : 0) ? j == jr : xrow);
+                    const double t = fma(-xv[u], av[u], acc);
+                    acc = (vv[u] >= 0 && xv[u] == xv[u] && !masked) ? t : acc;
+                }
             }
             acc = team_add<MPQP_GR>(acc);
             if (g == 0) r[rho] = acc;
-            rn = fmax(rn, fabs(acc));
+            rn = fmax(rn, res_mag(acc));
         }
         const double rmax = block_max<NT>(rn, red);
         OPT_PROF(sh, 8);
@@ -730,26 +810,30 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
             const float *Ti = L.T + i * ld;
             const int bv = L.basis[i];
             double acc = (g == 0 && bv >= e0) ? r[bv - e0] : 0.0;
-            for (int k0 = g; k0 < ne; k0 += MPQP_GB * MPQP_CBL) {
+            int k = g;
+            for (; k + (MPQP_CBL - 1) * MPQP_GB < ne; k += MPQP_CBL * MPQP_GB) {
                 int rk[MPQP_CBL], sk[MPQP_CBL];
                 double rv[MPQP_CBL], tv[MPQP_CBL];
 #pragma unroll
                 for (int u = 0; u < MPQP_CBL; ++u) {
-                    const int k = k0 + u * MPQP_GB;
-                    rk[u] = k < ne ? L.bcnt[k] : -1;
-                    sk[u] = k < ne ? L.nbcnt[k] : 0;
+                    rk[u] = L.elr[k + u * MPQP_GB];
+                    sk[u] = L.els[k + u * MPQP_GB];
                 }
 #pragma unroll
                 for (int u = 0; u < MPQP_CBL; ++u) {
-                    rv[u] = rk[u] >= 0 ? r[rk[u]] : 0.0;
-                    tv[u] = rk[u] >= 0 ? (double)Ti[sk[u]] : 0.0;
+                    rv[u] = r[rk[u]];
+                    tv[u] = (double)Ti[sk[u]];
                 }
 #pragma unroll
-                for (int u = 0; u < MPQP_CBL; ++u)
-                    if (rk[u] >= 0) acc = fma(rv[u], tv[u], acc);
+                for (int u = 0; u < MPQP_CBL; ++u) acc = fma(rv[u], tv[u], acc);
             }
+            for (; k < ne; k += MPQP_GB) acc = fma(r[L.elr[k]], (double)Ti[L.els[k]], acc);
             acc = team_add<MPQP_GB>(acc);
-            if (g == 0) x[i] = x[i] + acc;
+            if (g == 0) {
+                const double xn = x[i] + acc;
+                x[i] = xn;
+                L.xv[bv] = xn == xn ? xn : INFINITY;
+            }
         }
         __syncthreads();
         OPT_PROF(sh, 9);
@@ -765,6 +849,8 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
                                          CostF cost, OptShared &sh, int ne) {
     const int M = P.M2, ld = L.ld, e0 = P.N2 - M;
     double *y = L.cx, *r = L.cr;
+    SBits sb;
+    sb.load(inS);
     for (int rho = threadIdx.x; rho < M; rho += NT) {
         const int w = L.where[e0 + rho];
         const double c = cost(e0 + rho);
@@ -777,10 +863,10 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
         for (int q = threadIdx.x; q < M * MPQP_GC; q += NT) {
             const int i = q / MPQP_GC, g = q % MPQP_GC;
             const int bv = L.basis[i];
-            double acc = col_resid(P, inS, bv, g, cost(bv), y);
+            double acc = col_resid(P, sb, bv, g, cost(bv), y);
             acc = team_add<MPQP_GC>(acc);
             if (g == 0) r[i] = acc;
-            rn = fmax(rn, fabs(acc));
+            rn = fmax(rn, res_mag(acc));
         }
         const double rmax = block_max<NT>(rn, red);
         OPT_PROF(sh, 12);
@@ -790,21 +876,21 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
             // (B32^-T r)_rho of a nonbasic e_rho: its tableau column,
             // positions ascending
             const int k = q / MPQP_GT, g = q % MPQP_GT;
-            const int rho = L.bcnt[k];
-            const float *Tk = L.T + L.nbcnt[k];
+            const int rho = L.elr[k];
+            const float *Tk = L.T + L.els[k];
             double acc = 0.0;
-            for (int i0 = g; i0 < M; i0 += MPQP_GT * MPQP_CBL) {
+            int i = g;
+            for (; i + (MPQP_CBL - 1) * MPQP_GT < M; i += MPQP_CBL * MPQP_GT) {
                 double rv[MPQP_CBL], tv[MPQP_CBL];
 #pragma unroll
                 for (int u = 0; u < MPQP_CBL; ++u) {
-                    const int i = i0 + u * MPQP_GT;
-                    rv[u] = i < M ? r[i] : 0.0;
-                    tv[u] = i < M ? (double)Tk[i * ld] : 0.0;
+                    rv[u] = r[i + u * MPQP_GT];
+                    tv[u] = (double)Tk[(i + u * MPQP_GT) * ld];
                 }
 #pragma unroll
-                for (int u = 0; u < MPQP_CBL; ++u)
-                    if (i0 + u * MPQP_GT < M) acc = fma(rv[u], tv[u], acc);
+                for (int u = 0; u < MPQP_CBL; ++u) acc = fma(rv[u], tv[u], acc);
             }
+            for (; i < M; i += MPQP_GT) acc = fma(r[i], (double)Tk[i * ld], acc);
             acc = team_add<MPQP_GT>(acc);
             if (g == 0) y[rho] = y[rho] + acc;
         }
@@ -819,16 +905,19 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
 }
 
 // min over the selected nonbasic slots of the FP64 reduced cost c_v - a_v^T y
+// (a NaN reduced cost counts as -inf)
 template <int NT, typename Sel, typename CostF>
 __device__ __forceinline__ double min_red_cost(const OptLds<float> &L, const DevProb &P, const unsigned *inS,
                                                double *red, Sel sel, CostF cost) {
+    SBits sb;
+    sb.load(inS);
     double mn = INFINITY;
     for (int q = threadIdx.x; q < P.K2 * MPQP_GC; q += NT) {
         const int kk = q / MPQP_GC, g = q % MPQP_GC;
         const int v = L.nonb[kk];
         if (sel(v)) {
-            const double d = team_add<MPQP_GC>(col_resid(P, inS, v, g, cost(v), L.cx));
-            mn = fmin(mn, d);
+            const double d = team_add<MPQP_GC>(col_resid(P, sb, v, g, cost(v), L.cx));
+            mn = fmin(mn, d == d ? d : -INFINITY);
         }
     }
     return -block_max<NT>(-mn, red);

@@ -207,16 +207,13 @@ class MpqpRun:
 
 def decode_active_sets(ptr, n_sets: int, num_x: int) -> list:
     """utils.py:109-115: rows of num_x ints, each ended by the first -1."""
-    out = []
-    for lp_idx in range(n_sets):
-        active_set = []
-        for ele_idx in range(num_x):
-            v = ptr[lp_idx * num_x + ele_idx]
-            if v == -1:
-                break
-            active_set.append(int(v))
-        out.append(active_set)
-    return out
+    if n_sets <= 0:
+        return []
+    rows = np.ctypeslib.as_array(ptr, shape=(n_sets * num_x,)).reshape(n_sets, num_x)
+    ended = rows == -1
+    lens = np.where(ended.any(axis=1), ended.argmax(axis=1), num_x)
+    width = int(lens.max())
+    return [row[:n] for row, n in zip(rows[:, :width].tolist(), lens.tolist())]
 
 
 class MpqpLib:

@@ -551,6 +551,12 @@ void oracle_cert_stats(long long *out, int reset) {
 #define CERT_GB 2 /* rows of B^-1: primal correction */
 #define CERT_GT 4 /* columns of B^-1: dual correction */
 
+/* |r| for a convergence test; NaN counts as +inf (never certified) */
+static double res_mag(double r) {
+    double a = fabs(r);
+    return a == a ? a : INFINITY;
+}
+
 static double team_total(const double *p, int G) {
     if (G == 1) return p[0];
     if (G == 2) return p[0] + p[1];
@@ -633,7 +639,7 @@ static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const ch
             }
             double acc = team_total(p, CERT_GR);
             r[rho] = acc;
-            if (fabs(acc) > rn) rn = fabs(acc);
+            if (res_mag(acc) > rn) rn = res_mag(acc);
         }
         if (g_trace) fprintf(stderr, "cert_primal it %d rn %.3e tol %.3e\n", it, rn, tol);
         if (rn <= tol) return 1;
@@ -661,7 +667,7 @@ static int cert_dual(const Prob *P, const OSt_f *O, const int *where, const char
             int bv = O->basis[i];
             double acc = col_resid(P, inS, bv, cert_cost(P, phase, bv), y);
             r[i] = acc;
-            if (fabs(acc) > rn) rn = fabs(acc);
+            if (res_mag(acc) > rn) rn = res_mag(acc);
         }
         if (rn <= 2e-12) return 1;
         if (it == CERT_STEPS) return 0;
@@ -689,7 +695,7 @@ static int certify_phase1(const Prob *P, const MpqpOptions *o, const OSt_f *O, S
         if (!cert_dual(P, O, s->where, s->inS, 1, y, r, d)) CFAIL(3);
         for (int k = 0; k < P->K2; k++) {
             int v = O->nonb[k];
-            if (col_resid(P, s->inS, v, cert_cost(P, 1, v), y) < -o->certify_tol) CFAIL(4);
+            if (!(col_resid(P, s->inS, v, cert_cost(P, 1, v), y) >= -o->certify_tol)) CFAIL(4);
         }
     }
     for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
@@ -713,7 +719,7 @@ static int certify_phase2(const Prob *P, const MpqpOptions *o, const OSt_f *O, S
     for (int k = 0; k < P->K2; k++) {
         int v = O->nonb[k];
         if (v >= P->N_) continue;
-        if (col_resid(P, s->inS, v, cert_cost(P, 2, v), y) < -o->certify_tol) CFAIL(9);
+        if (!(col_resid(P, s->inS, v, cert_cost(P, 2, v), y) >= -o->certify_tol)) CFAIL(9);
     }
     for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
     out->z2 = tree_sum_d(r, M);
