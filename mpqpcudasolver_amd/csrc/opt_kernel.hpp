@@ -775,14 +775,27 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
             const unsigned stride = (unsigned)(M * MPQP_GR);
             const bool xrow = rho >= xb0 && rho < xb1;
             const int jr = rho - xb1;  // the w column whose own row this is
+            // software pipeline: the loads of batch b + 1 are in flight while
+            // batch b is looked up and accumulated (the last batch is loaded
+            // twice rather than guarded)
+            int vv[MPQP_CB];
+            double av[MPQP_CB];
+            const unsigned last = ns >= MPQP_CB ? ns - MPQP_CB : 0u;
+#pragma unroll
+            for (int u = 0; u < MPQP_CB; ++u) {
+                const unsigned e = (unsigned)q + (unsigned)u * stride;
+                vv[u] = ns ? P.csr_var[e] : -1;
+                av[u] = ns ? P.csr_val[e] : 0.0;
+            }
             for (unsigned s0 = 0; s0 < ns; s0 += MPQP_CB) {
-                int vv[MPQP_CB];
-                double av[MPQP_CB], xv[MPQP_CB];
+                const unsigned sn = s0 + MPQP_CB < ns ? s0 + MPQP_CB : last;
+                int vn[MPQP_CB];
+                double an[MPQP_CB], xv[MPQP_CB];
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) {
-                    const unsigned e = (unsigned)q + (s0 + u) * stride;
-                    vv[u] = P.csr_var[e];
-                    av[u] = P.csr_val[e];
+                    const unsigned e = (unsigned)q + (sn + u) * stride;
+                    vn[u] = P.csr_var[e];
+                    an[u] = P.csr_val[e];
                 }
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) xv[u] = L.xv[vv[u] >= 0 ? vv[u] : 0];
@@ -793,6 +806,11 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
                     const bool masked = isw && (sb.has(isw ? j : 0) ? j == jr : xrow);
                     const double t = fma(-xv[u], av[u], acc);
                     acc = (vv[u] >= 0 && xv[u] == xv[u] && !masked) ? t : acc;
+                }
+#pragma unroll
+                for (int u = 0; u < MPQP_CB; ++u) {
+                    vv[u] = vn[u];
+                    av[u] = an[u];
                 }
             }
             acc = team_add<MPQP_GR>(acc);
