@@ -56,14 +56,18 @@ struct DevProb {
     const int *root_nonb;    // K2
     const int *root_ok;      // 1 int
     double cert_ptol;        // primal certification: 1e-12 * (1 + max|xb2|)
-    // A2 (column-major, M2 x N2) in sparse form for the FP64 certification.
-    // Rows, variables ascending, in an ELL layout interleaved for the row
-    // teams (GR lanes): entry t of row rho at ((t / GR) * M2 + rho) * GR +
-    // t % GR.  Columns, rows ascending, contiguous from csc_ptr[v] (a
-    // multiple of the column team size).
-    const int *csr_n, *csr_var;    // per row: count; entries
+    // A2 (column-major, M2 x N2) in sparse form for the FP64 certification,
+    // without the stationarity-row entries of the w columns in the row lists.
+    // Rows in slots ordered longest first (csr_row: slot -> row), ELL layout
+    // interleaved for the row teams (GR lanes): entry t of slot sl at
+    // ((t / GR) * M2 + sl) * GR + t % GR; csr_n: the slot's step count.
+    // Columns contiguous from csc_ptr[v] (a multiple of the column team
+    // size GC): csc_n entries (rows ascending; for a w column those outside
+    // the stationarity rows), padded to whole steps, then for a w column its
+    // csc_x stationarity-row entries.
+    const int *csr_n, *csr_row, *csr_var;
     const double *csr_val;
-    const int *csc_n, *csc_ptr, *csc_row;  // per column: count, start; entries
+    const int *csc_n, *csc_x, *csc_ptr, *csc_row;
     const double *csc_val;
 };
 

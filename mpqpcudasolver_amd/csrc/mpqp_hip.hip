@@ -379,69 +379,91 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         P.cert_ptol = 1e-12 * (1.0 + bmax);
     }
 
-    // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification:
-    // rows (variables ascending) in an ELL layout interleaved for teams of
-    // GR lanes, entry t of row rho at ((t / GR) * M2 + rho) * GR + t % GR,
-    // csr_n[rho] = the step count (entries per lane) of the row's wave slice
-    // rounded up to whole batches of MPQP_CB; columns (rows ascending)
-    // contiguous, each starting at a multiple of the column team size GC and
-    // padded to whole steps.  Padding entries: index -1, value 0.
+    // sparse copies of A2 (A2[v * M2 + rho]) for the FP64 certification.
+    // The entries of the w columns in the stationarity rows (present for S
+    // only when j in S) leave the row lists and form a second segment of
+    // each w column's list.  Rows: slots ordered longest first, ELL layout
+    // interleaved for teams of GR lanes (entry t of slot sl at ((t / GR) *
+    // M2 + sl) * GR + t % GR), csr_n[sl] = the step count of the slot's wave
+    // slice rounded up to whole batches of MPQP_CB.  Columns: contiguous
+    // from csc_ptr[v] (a multiple of GC), the first segment (rows
+    // ascending) padded to whole steps, then the second.  Padding entries:
+    // index -1, value 0.
     constexpr int GR = MPQP_GR, GC = MPQP_GC, CB = MPQP_CB, SL = MPQP_WAVE / GR;
-    std::vector<int> rn(M2, 0), cn(N2, 0), cptr(N2, 0), rsteps(M2, 0);
+    const int sr0 = M2 - n_c - n_x, sr1 = M2 - n_c, w0 = n_x + n_t;
+    auto is_w = [&](int v) { return v >= w0 && v < w0 + n_c; };
+    auto stat = [&](int rho) { return rho >= sr0 && rho < sr1; };
+    std::vector<int> rn(M2, 0), cn(N2, 0), cx(N2, 0), cptr(N2, 0);
     for (int v = 0; v < N2; ++v)
         for (int rho = 0; rho < M2; ++rho)
             if (A2[(size_t)v * M2 + rho] != 0.0) {
-                rn[rho]++;
-                cn[v]++;
+                const bool second = is_w(v) && stat(rho);
+                if (!second) rn[rho]++;
+                (second ? cx[v] : cn[v])++;
             }
+    std::vector<int> slot_row(M2), slot_of(M2), ssteps(M2, 0);
+    for (int rho = 0; rho < M2; ++rho) slot_row[rho] = rho;
+    std::stable_sort(slot_row.begin(), slot_row.end(), [&](int a, int b) { return rn[a] > rn[b]; });
+    for (int sl = 0; sl < M2; ++sl) slot_of[slot_row[sl]] = sl;
     int maxsteps = CB, csz = 0;
-    for (int r0 = 0; r0 < M2; r0 += SL) {
-        const int r1 = std::min(M2, r0 + SL);
+    for (int s0 = 0; s0 < M2; s0 += SL) {
+        const int s1 = std::min(M2, s0 + SL);
         int stp = 0;
-        for (int rho = r0; rho < r1; ++rho) stp = std::max(stp, (rn[rho] + GR - 1) / GR);
+        for (int sl = s0; sl < s1; ++sl) stp = std::max(stp, (rn[slot_row[sl]] + GR - 1) / GR);
         stp = (stp + CB - 1) / CB * CB;
-        for (int rho = r0; rho < r1; ++rho) rsteps[rho] = stp;
+        for (int sl = s0; sl < s1; ++sl) ssteps[sl] = stp;
         maxsteps = std::max(maxsteps, stp);
     }
     const int rw = maxsteps * GR;
     for (int v = 0; v < N2; ++v) {
         cptr[v] = csz;
-        csz += std::max(GC, (cn[v] + GC - 1) / GC * GC);
+        csz += std::max(GC, (cn[v] + GC - 1) / GC * GC + (cx[v] + GC - 1) / GC * GC);
     }
     std::vector<int> rvar((size_t)rw * M2, -1), crow((size_t)csz, -1), rfill(M2, 0);
     std::vector<double> rval((size_t)rw * M2, 0.0), cval((size_t)csz, 0.0);
     for (int v = 0; v < N2; ++v) {
-        int cf = 0;
+        int c1 = 0, c2 = 0;
+        const size_t seg2 = (size_t)cptr[v] + (size_t)((cn[v] + GC - 1) / GC * GC);
         for (int rho = 0; rho < M2; ++rho) {
             const double a = A2[(size_t)v * M2 + rho];
             if (a == 0.0) continue;
-            const int t = rfill[rho]++;
-            const size_t er = ((size_t)(t / GR) * M2 + rho) * GR + (t % GR), ec = (size_t)cptr[v] + cf++;
-            rvar[er] = v;
-            rval[er] = a;
+            const bool second = is_w(v) && stat(rho);
+            const size_t ec = second ? seg2 + c2++ : (size_t)cptr[v] + c1++;
             crow[ec] = rho;
             cval[ec] = a;
+            if (second) continue;
+            const int t = rfill[rho]++, sl = slot_of[rho];
+            const size_t er = ((size_t)(t / GR) * M2 + sl) * GR + (t % GR);
+            rvar[er] = v;
+            rval[er] = a;
         }
     }
-    DBuf dsp_i(sizeof(int) * (M2 + 2 * N2 + rvar.size() + crow.size()));
+    DBuf dsp_i(sizeof(int) * (2 * M2 + 3 * N2 + rvar.size() + crow.size()));
     DBuf dsp_d(sizeof(double) * (rval.size() + cval.size()));
     {
         int *ib0 = dsp_i.as<int>();
         double *db0 = dsp_d.as<double>();
         P.csr_n = ib0;
-        P.csc_n = ib0 + M2;
-        P.csc_ptr = ib0 + M2 + N2;
-        P.csr_var = ib0 + M2 + 2 * N2;
+        P.csr_row = ib0 + M2;
+        P.csc_n = ib0 + 2 * M2;
+        P.csc_x = P.csc_n + N2;
+        P.csc_ptr = P.csc_x + N2;
+        P.csr_var = P.csc_ptr + N2;
         P.csc_row = P.csr_var + rvar.size();
         P.csr_val = db0;
         P.csc_val = db0 + rval.size();
-        HIP_OK(hipMemcpyAsync((void *)P.csr_n, rsteps.data(), sizeof(int) * rsteps.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csc_n, cn.data(), sizeof(int) * cn.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csc_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csr_var, rvar.data(), sizeof(int) * rvar.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csc_row, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csr_val, rval.data(), sizeof(double) * rval.size(), hipMemcpyHostToDevice, st));
-        HIP_OK(hipMemcpyAsync((void *)P.csc_val, cval.data(), sizeof(double) * cval.size(), hipMemcpyHostToDevice, st));
+        auto up = [&](const void *dst, const void *src, size_t bytes) {
+            HIP_OK(hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, st));
+        };
+        up(P.csr_n, ssteps.data(), sizeof(int) * M2);
+        up(P.csr_row, slot_row.data(), sizeof(int) * M2);
+        up(P.csc_n, cn.data(), sizeof(int) * N2);
+        up(P.csc_x, cx.data(), sizeof(int) * N2);
+        up(P.csc_ptr, cptr.data(), sizeof(int) * N2);
+        up(P.csr_var, rvar.data(), sizeof(int) * rvar.size());
+        up(P.csc_row, crow.data(), sizeof(int) * crow.size());
+        up(P.csr_val, rval.data(), sizeof(double) * rval.size());
+        up(P.csc_val, cval.data(), sizeof(double) * cval.size());
     }
 
     // root tableau

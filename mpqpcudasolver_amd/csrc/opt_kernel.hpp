@@ -592,14 +592,18 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 // constant term, the others from +0.0; masked entries and nonbasic
 // variables are skipped in place), and the team adds its partials by lane
 // exchange, xor 1 then xor 2, so every lane ends with (p0+p1)+(p2+p3).
-// Team sizes: rows of A2 2, columns of A2 4, rows of B^-1 2, columns of
+// Team sizes: rows of A2 4, columns of A2 4, rows of B^-1 2, columns of
 // B^-1 4 (a team never straddles a wave: G divides 64).  The sparse walks
 // are padded so that their loads need no per-entry guard: the rows of one
 // wave slice share a step count, a multiple of MPQP_CB; columns are padded
-// to whole steps.
+// to whole steps.  The entries of the w columns in the stationarity rows
+// survive the S-mask only for j in S: they are kept out of the row lists
+// (a stationarity row adds the terms of w_j, j in S ascending, after its
+// team sum) and form a second segment of a w column's list, walked only
+// for j in S; the one masked entry left is (row of w_j, w_j), j in S.
 #define MPQP_CB 4   // A2 entries in flight per lane
 #define MPQP_CBL 4  // tableau entries in flight per lane
-#define MPQP_GR 2
+#define MPQP_GR 4
 #define MPQP_GC 4
 #define MPQP_GB 2
 #define MPQP_GT 4
@@ -657,27 +661,6 @@ struct SBits {
     }
 };
 
-// The S-mask of the w columns (initialise_A_kernel_1,
-// batched_simplex.cu:635-650) for one column v (a team's column walk):
-// entry (rho, v) of A2 is zero for S iff v = w_j and (j in S ? rho is the
-// row of w_j : rho is a stationarity row), i.e. iff isw && (in ? rho ==
-// jrow : xb0 <= rho < xb1).
-struct ColMask {
-    bool isw, in;
-    int jrow, xb0, xb1;
-    __device__ __forceinline__ ColMask(const DevProb &P, const SBits &sb, int v) {
-        const int j = v - (P.n_x + P.n_t);
-        isw = (unsigned)j < (unsigned)P.n_c;
-        in = isw && sb.has(isw ? j : 0);
-        jrow = P.M2 - P.n_c + j;
-        xb0 = P.M2 - P.n_c - P.n_x;
-        xb1 = P.M2 - P.n_c;
-    }
-    __device__ __forceinline__ bool masked(int rho) const {
-        return isw && (in ? rho == jrow : (rho >= xb0 && rho < xb1));
-    }
-};
-
 // Compact list of the nonbasic e-variables, rho ascending: L.elr[k] (rho),
 // L.els[k] (slot).  Returns their count.
 template <int S>
@@ -707,14 +690,10 @@ __device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, O
     return ne;
 }
 
-// Lane g's partial of c - sum over column v of A2 (rows ascending, masked
-// skipped) of a * y[row]: entries t = g, g + MPQP_GC, ... (c on lane 0).
-__device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, int v, int g, double c,
-                                            const double *y) {
-    double acc = g == 0 ? c : 0.0;
-    const ColMask cm(P, sb, v);
-    const unsigned ns = ((unsigned)P.csc_n[v] + MPQP_GC - 1) / MPQP_GC;
-    const unsigned base = (unsigned)P.csc_ptr[v] + (unsigned)g;
+// One segment of a column walk: steps s < ns of lane g's entries from
+// base (entry base + s * MPQP_GC), row rmask skipped (the masked entry).
+__device__ __forceinline__ double col_walk(const DevProb &P, unsigned base, unsigned ns, int rmask, const double *y,
+                                           double acc) {
     unsigned s = 0;
     for (; s + MPQP_CB <= ns; s += MPQP_CB) {
         int rr[MPQP_CB];
@@ -730,15 +709,29 @@ __device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, i
 #pragma unroll
         for (int u = 0; u < MPQP_CB; ++u) {
             const double t = fma(-av[u], yv[u], acc);
-            acc = (rr[u] >= 0 && !cm.masked(rr[u])) ? t : acc;
+            acc = (rr[u] >= 0 && rr[u] != rmask) ? t : acc;
         }
     }
     for (; s < ns; ++s) {
         const unsigned e = base + s * MPQP_GC;
         const int rho = P.csc_row[e];
         const double t = fma(-P.csc_val[e], y[rho >= 0 ? rho : 0], acc);
-        acc = (rho >= 0 && !cm.masked(rho)) ? t : acc;
+        acc = (rho >= 0 && rho != rmask) ? t : acc;
     }
+    return acc;
+}
+
+// Lane g's partial of c - sum over column v of A2 of a * y[row] (c on lane
+// 0): the first segment (rows ascending; for a w column, the rows outside
+// the stationarity block), then for w_j with j in S its stationarity rows.
+__device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, int v, int g, double c,
+                                            const double *y) {
+    const int j = v - (P.n_x + P.n_t);
+    const bool in = (unsigned)j < (unsigned)P.n_c && sb.has(j);
+    const unsigned base = (unsigned)P.csc_ptr[v] + (unsigned)g;
+    const unsigned ns1 = ((unsigned)P.csc_n[v] + MPQP_GC - 1) / MPQP_GC;
+    double acc = col_walk(P, base, ns1, in ? P.M2 - P.n_c + j : -2, y, g == 0 ? c : 0.0);
+    if (in) acc = col_walk(P, base + ns1 * MPQP_GC, ((unsigned)P.csc_x[v] + MPQP_GC - 1) / MPQP_GC, -2, y, acc);
     return acc;
 }
 
@@ -747,7 +740,7 @@ __device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, i
 // max|xb2 - B x| <= cert_ptol.  *ne_out: the number of nonbasic e-variables.
 template <int NT, int S>
 __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
-                                           OptShared &sh, int *ne_out) {
+                                           OptShared &sh, const int *Sset, int k, int *ne_out) {
     const int M = P.M2, N2 = P.N2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
     double *x = L.cx, *r = L.cr;
     const int ne = elist_build<S>(L, P, sh);
@@ -766,15 +759,17 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
     for (int it = 0;; ++it) {
         double rn = 0.0;
         for (int q = threadIdx.x; q < M * MPQP_GR; q += NT) {
-            // (xb2 - B x)_rho: row rho of A2, variables ascending, basic only;
-            // entry t of row rho at ((t / GR) * M + rho) * GR + t % GR, csr_n
-            // the (padded) step count of the row's wave slice
-            const int rho = q / MPQP_GR, g = q % MPQP_GR;
+            // (xb2 - B x)_rho: row rho of A2 in the term order above, basic
+            // only.  Slot sl (rows longest first) holds row csr_row[sl], its
+            // entry t at ((t / GR) * M + sl) * GR + t % GR; csr_n[sl] is the
+            // (padded) step count of the slot's wave slice.
+            const int sl = q / MPQP_GR, g = q % MPQP_GR;
+            const int rho = P.csr_row[sl];
             double acc = g == 0 ? P.xb2[rho] : 0.0;
-            const unsigned ns = (unsigned)P.csr_n[rho];
+            const unsigned ns = (unsigned)P.csr_n[sl];
             const unsigned stride = (unsigned)(M * MPQP_GR);
-            const bool xrow = rho >= xb0 && rho < xb1;
             const int jr = rho - xb1;  // the w column whose own row this is
+            const int vmask = ((unsigned)jr < (unsigned)n_c && sb.has(jr)) ? woff + jr : -2;
             // software pipeline: the loads of batch b + 1 are in flight while
             // batch b is looked up and accumulated (the last batch is loaded
             // twice rather than guarded)
@@ -801,11 +796,8 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
                 for (int u = 0; u < MPQP_CB; ++u) xv[u] = L.xv[vv[u] >= 0 ? vv[u] : 0];
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) {
-                    const int j = vv[u] - woff;
-                    const bool isw = (unsigned)j < (unsigned)n_c;
-                    const bool masked = isw && (sb.has(isw ? j : 0) ? j == jr : xrow);
                     const double t = fma(-xv[u], av[u], acc);
-                    acc = (vv[u] >= 0 && xv[u] == xv[u] && !masked) ? t : acc;
+                    acc = (vv[u] >= 0 && xv[u] == xv[u] && vv[u] != vmask) ? t : acc;
                 }
 #pragma unroll
                 for (int u = 0; u < MPQP_CB; ++u) {
@@ -814,6 +806,16 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
                 }
             }
             acc = team_add<MPQP_GR>(acc);
+            if (rho >= xb0 && rho < xb1) {
+                // stationarity row: the w_j of S (ascending), from dense A2
+                for (int t = 0; t < k; ++t) {
+                    const int v = woff + Sset[t];
+                    const double a = P.A2[(long long)v * M + rho];
+                    const double xw = L.xv[v];
+                    const double tt = fma(-xw, a, acc);
+                    acc = (a != 0.0 && xw == xw) ? tt : acc;
+                }
+            }
             if (g == 0) r[rho] = acc;
             rn = fmax(rn, res_mag(acc));
         }
@@ -1077,7 +1079,7 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                 const int Nn = P.Nn;
                 auto cb1 = [Nn](int v) { return v >= Nn ? 1.0 : 0.0; };
                 int ok = p1.status == ST_OPTIMAL, ne = 0;
-                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, &ne);
+                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, Sset, k, &ne);
                 if (ok) ok = cert_xmin<NT>(L, M, red) >= O.dual_stop_tol;
                 if (ok) {
                     p1.z1 = cert_z<NT, S>(L, P, red, cb1);
@@ -1115,7 +1117,7 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                     const int EPS = P.n_x + P.n_t + P.n_c, Nn = P.Nn;
                     auto cb2 = [EPS](int v) { return v == EPS ? -1.0 : 0.0; };
                     int ok = p2.status == ST_OPTIMAL, ne = 0;
-                    if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, &ne);
+                    if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, Sset, k, &ne);
                     if (ok) ok = cert_xmin<NT>(L, M, red) >= -O.certify_tol;
                     if (ok) {
                         p2.z2 = cert_z<NT, S>(L, P, red, cb2);

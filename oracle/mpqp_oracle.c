@@ -545,11 +545,34 @@ void oracle_cert_stats(long long *out, int reset) {
  * skipped but keep their place t), and the G partials are added by the
  * butterfly p0+p1 (G = 2) or (p0+p1)+(p2+p3) (G = 4).  The GPU computes
  * the partials on G adjacent lanes and adds them by lane exchange (xor 1,
- * then xor 2), which gives the same sums bit for bit. */
-#define CERT_GR 2 /* rows of A2: primal residual */
+ * then xor 2), which gives the same sums bit for bit.
+ *
+ * The entries of the w columns in the stationarity rows [M2-n_c-n_x,
+ * M2-n_c) survive the S-mask (initialise_A_kernel_1, batched_simplex.cu
+ * :635-650) only for j in S, so the A2 walks keep them out
+ * of their term lists and add them explicitly:
+ *  - row rho: the CSR entries without them; for a stationarity row, then
+ *    the terms of w_j, j in S ascending, as a sequential fma chain on the
+ *    team total;
+ *  - column v: the CSC entries outside the stationarity rows when v is a
+ *    w column (all entries otherwise), partial (index % CERT_GC); for w_j
+ *    with j in S, then its stationarity-row entries, partial (index among
+ *    them % CERT_GC).
+ * The one masked entry left in the lists is (row of w_j, w_j), j in S. */
+#define CERT_GR 4 /* rows of A2: primal residual */
 #define CERT_GC 4 /* columns of A2: dual residual, reduced costs */
 #define CERT_GB 2 /* rows of B^-1: primal correction */
 #define CERT_GT 4 /* columns of B^-1: dual correction */
+
+/* j if v is the column of w_j, else -1 */
+static int w_index(const Prob *P, int v) {
+    int j = v - (P->n_x + P->n_t);
+    return (j >= 0 && j < P->n_c) ? j : -1;
+}
+/* 1 for a stationarity row (the R4 rows of the S-mask) */
+static int stat_row(const Prob *P, int rho) {
+    return rho >= P->M2 - P->n_c - P->n_x && rho < P->M2 - P->n_c;
+}
 
 /* |r| for a convergence test; NaN counts as +inf (never certified) */
 static double res_mag(double r) {
@@ -561,14 +584,6 @@ static double team_total(const double *p, int G) {
     if (G == 1) return p[0];
     if (G == 2) return p[0] + p[1];
     return (p[0] + p[1]) + (p[2] + p[3]);
-}
-
-/* true if entry (rho, v) of A2 is masked to zero for S (:635-650) */
-static int w_masked(const Prob *P, const char *inS, int rho, int v) {
-    int j = v - (P->n_x + P->n_t);
-    if (j < 0 || j >= P->n_c) return 0;
-    if (inS[j]) return rho == P->M2 - P->n_c + j;
-    return rho >= P->M2 - P->n_c - P->n_x && rho < P->M2 - P->n_c;
 }
 
 /* cost of variable v in the phase being certified: phase 1 = 1 on the
@@ -608,19 +623,31 @@ static double binvT_col(const Prob *P, const OSt_f *O, const int *where, const d
     return team_total(p, CERT_GT);
 }
 
-/* c - sum over column v of A2 (rows ascending, masked skipped) of a * y[row] */
+/* c - sum over column v of A2 of a * y[row], in the term order above */
 static double col_resid(const Prob *P, const char *inS, int v, double c, const double *y) {
     double p[4] = {c, 0.0, 0.0, 0.0};
+    int j = w_index(P, v), in = j >= 0 && inS[j], jrow = P->M2 - P->n_c + j, t = 0;
     for (int e = P->csc_ptr[v]; e < P->csc_ptr[v + 1]; e++) {
-        int rho = P->csc_row[e], g = (e - P->csc_ptr[v]) % CERT_GC;
-        if (!w_masked(P, inS, rho, v)) p[g] = fma(-P->csc_val[e], y[rho], p[g]);
+        int rho = P->csc_row[e];
+        if (j >= 0 && stat_row(P, rho)) continue;
+        int g = t++ % CERT_GC;
+        if (!(in && rho == jrow)) p[g] = fma(-P->csc_val[e], y[rho], p[g]);
+    }
+    if (in) {
+        t = 0;
+        for (int e = P->csc_ptr[v]; e < P->csc_ptr[v + 1]; e++) {
+            int rho = P->csc_row[e];
+            if (!stat_row(P, rho)) continue;
+            int g = t++ % CERT_GC;
+            p[g] = fma(-P->csc_val[e], y[rho], p[g]);
+        }
     }
     return team_total(p, CERT_GC);
 }
 
 /* x := FP64 solution of B x = xb2 by iterative refinement from the FP32
  * rhs column; returns 1 once max|xb2 - B x| <= 1e-12 (1 + max|xb2|).
- * (B x)_rho walks row rho of A2 (variables ascending, basic only). */
+ * (B x)_rho walks row rho of A2 in the term order above (basic only). */
 static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const char *inS, double *x,
                        double *r, double *dx) {
     int M = P->M2, ld = P->ld2, RHS = P->K2;
@@ -633,11 +660,21 @@ static int cert_primal(const Prob *P, const OSt_f *O, const int *where, const ch
         double rn = 0.0;
         for (int rho = 0; rho < M; rho++) {
             double p[4] = {P->xb2[rho], 0.0, 0.0, 0.0};
+            int srow = stat_row(P, rho), jr = rho - (P->M2 - P->n_c), t = 0;
             for (int e = P->csr_ptr[rho]; e < P->csr_ptr[rho + 1]; e++) {
-                int v = P->csr_var[e], w = where[v], g = (e - P->csr_ptr[rho]) % CERT_GR;
-                if (w >= 0 && !w_masked(P, inS, rho, v)) p[g] = fma(-x[w], P->csr_val[e], p[g]);
+                int v = P->csr_var[e], w = where[v], j = w_index(P, v);
+                if (srow && j >= 0) continue;
+                int g = t++ % CERT_GR;
+                if (w >= 0 && !(j >= 0 && j == jr && inS[j])) p[g] = fma(-x[w], P->csr_val[e], p[g]);
             }
             double acc = team_total(p, CERT_GR);
+            if (srow)
+                for (int j = 0; j < P->n_c; j++) {
+                    if (!inS[j]) continue;
+                    int v = W_VAR(P, j), w = where[v];
+                    double a = P->A2[(size_t)v * M + rho];
+                    if (a != 0.0 && w >= 0) acc = fma(-x[w], a, acc);
+                }
             r[rho] = acc;
             if (res_mag(acc) > rn) rn = res_mag(acc);
         }
