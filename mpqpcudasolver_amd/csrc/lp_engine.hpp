@@ -35,15 +35,19 @@ struct DevOpts {
 // relative pivot threshold pivot_rel32).
 struct DevTol {
     double tol_plus, tol_minus, tol_minus_opt, dual_stop_tol, dual_feas_tol, licq, pivot_rel, final_tol;
+    // FP32 pass only: an empty ratio test in the phase-1 dual-feasibility
+    // cleanup ends the cleanup (the reduced cost is rounding noise for a
+    // phase-1 objective bounded below; the FP64 certificate checks it)
+    int cleanup_accept;
 };
 
 __host__ __device__ inline DevTol dev_tol64(const DevOpts &o) {
     return DevTol{o.tol_plus, o.tol_minus, o.tol_minus_opt, o.dual_stop_tol, o.dual_feas_tol,
-                  o.licq_tol, o.pivot_rel, o.final_tol};
+                  o.licq_tol, o.pivot_rel, o.final_tol, 0};
 }
 
 __host__ __device__ inline DevTol dev_tol32(const DevOpts &o) {
-    return DevTol{o.tol32, -o.tol32, -o.tol32, -o.tol32, o.tol32, o.licq32, o.pivot_rel32, o.final_tol};
+    return DevTol{o.tol32, -o.tol32, -o.tol32, -o.tol32, o.tol32, o.licq32, o.pivot_rel32, o.final_tol, 1};
 }
 
 struct DevProb {

@@ -401,7 +401,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             int action = ACT_STOP, r = -1;
             if (!(dv > tl.tol_minus)) {
                 r = primal_ratio_wave<Real>(T, ld, M, aux, RHS, tl);
-                action = r < 0 ? ACT_UNBOUNDED : ACT_PIVOT;
+                action = r >= 0 ? ACT_PIVOT : (tl.cleanup_accept ? ACT_STOP : ACT_UNBOUNDED);
             }
             if (lane == 0) {
                 sh.action = action;

@@ -84,7 +84,10 @@
  *
  * Mixed precision (no counterpart in the reference, which is FP64 only):
  * the optimality LP pivots in FP32 (tolerances tol32 / licq32 /
- * pivot_rel32) from the parent's FP32 phase-1 state; its final bases are
+ * pivot_rel32) from the parent's FP32 phase-1 state (an empty ratio test in
+ * the phase-1 dual-feasibility cleanup ends the cleanup instead of the LP:
+ * for a phase-1 objective bounded below that reduced cost is rounding
+ * noise, and the certification checks it in FP64); its final bases are
  * certified in FP64 by iterative refinement (primal x_B, dual y, reduced
  * costs) and the certified z decides the verdicts; an LP that fails
  * certification is solved again in FP64 from the cold start.  Levels 0 and
@@ -319,6 +322,7 @@ typedef struct {
     double tol_plus, tol_minus, tol_minus_opt, dual_stop_tol, dual_feas_tol;
     double licq, pivot_rel, final_tol;
     int max_iter_p1, max_iter_p2, stall_perturb, stall_exit;
+    int cleanup_accept; /* FP32 pass: an empty cleanup ratio test ends the cleanup */
 } LpTol;
 
 static LpTol lptol64(const MpqpOptions *o) {
@@ -335,6 +339,7 @@ static LpTol lptol64(const MpqpOptions *o) {
     t.max_iter_p2 = o->max_iter_phase2;
     t.stall_perturb = o->stall_perturb;
     t.stall_exit = o->stall_exit;
+    t.cleanup_accept = 0;
     return t;
 }
 
@@ -349,6 +354,7 @@ static LpTol lptol32(const MpqpOptions *o) {
     t.dual_feas_tol = o->tol32;
     t.licq = o->licq32;
     t.pivot_rel = o->pivot_rel32;
+    t.cleanup_accept = 1;
     return t;
 }
 
