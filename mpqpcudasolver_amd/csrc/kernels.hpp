@@ -67,6 +67,26 @@ __host__ __device__ inline long long state_bytes(int rows, int cols, int M, int 
 // ---------------------------------------------------------------------
 // Feasibility LP
 // ---------------------------------------------------------------------
+// dst[0..n) = src[0..n) (doubles; both 16-byte aligned) by the workgroup:
+// 16-byte vectors, eight in flight per thread, then the odd tail.
+template <int NT>
+__device__ __forceinline__ void copy_doubles(double *dst, const double *src, int n) {
+    const int nv = n / 2, tid = threadIdx.x;
+    const int4 *s4 = reinterpret_cast<const int4 *>(src);
+    int4 *d4 = reinterpret_cast<int4 *>(dst);
+    constexpr int UL = 8;
+    int e0 = tid;
+    for (; e0 + (UL - 1) * NT < nv; e0 += NT * UL) {
+        int4 vv[UL];
+#pragma unroll
+        for (int u = 0; u < UL; ++u) vv[u] = s4[e0 + u * NT];
+#pragma unroll
+        for (int u = 0; u < UL; ++u) d4[e0 + u * NT] = vv[u];
+    }
+    for (int e = e0; e < nv; e += NT) d4[e] = s4[e];
+    if ((n & 1) && tid == 0) dst[n - 1] = src[n - 1];
+}
+
 __host__ __device__ inline long long feas_lds_bytes(int M, int K, int n_c) {
     const int ld = K + 1;
     long long b = (long long)(M + 1) * ld * 8 + (long long)(M + 1) * 8 + (long long)(M + K) * 4 +
@@ -104,7 +124,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     if (par >= 0 && O.warm) {
         const unsigned char *src = a.fstore_prev + (long long)par * a.fstride;
         const double *sT = reinterpret_cast<const double *>(src);
-        for (int e = tid; e < M * ld; e += NT) T[e] = sT[e];
+        copy_doubles<NT>(T, sT, M * ld);
         const int *sb = reinterpret_cast<const int *>(sT + M * ld);
         for (int i = tid; i < M; i += NT) basis[i] = sb[i];
         for (int k = tid; k < K; k += NT) nonb[k] = sb[M + k];
@@ -122,9 +142,9 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
         const int s = var - K;
         return (s >= 0 && s < P.n_c && ((inS[s >> 5] >> (s & 31)) & 1u)) ? 1.0 : 0.0;
     };
-    cost_row_team<NT, double>(
-        T, ld, M, K + 1, [&](int i) { return cvar(basis[i]); },
-        [&](int j) { return j < K ? cvar(nonb[j]) : 0.0; });
+    for (int i = tid; i < M; i += NT) colq[i] = cvar(basis[i]);
+    __syncthreads();
+    cost_row_cb<NT, double>(T, ld, M, K + 1, colq, [&](int j) { return j < K ? cvar(nonb[j]) : 0.0; });
 
     int status = ST_ITER_CAP, iters = 0, feasible = 0;
     double z = 0.0;
@@ -172,7 +192,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
             break;
         }
         const int r = sh_r, q = sh_q;
-        pivot_team<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
+        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
         if (tid == 0) {
             const int t = basis[r];
             basis[r] = nonb[q];
@@ -200,7 +220,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     if (a.store && feasible) {
         unsigned char *dst = a.fstore_cur + (long long)c * a.fstride;
         double *dT = reinterpret_cast<double *>(dst);
-        for (int e = tid; e < M * ld; e += NT) dT[e] = T[e];
+        copy_doubles<NT>(dT, T, M * ld);
         int *db = reinterpret_cast<int *>(dT + M * ld);
         for (int i = tid; i < M; i += NT) db[i] = basis[i];
         for (int k = tid; k < K; k += NT) db[M + k] = nonb[k];

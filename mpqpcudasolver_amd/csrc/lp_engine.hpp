@@ -536,3 +536,31 @@ __device__ __forceinline__ void cost_row_team(Real *T, int ld, int M, int ncols,
     }
     __syncthreads();
 }
+
+// The same row with cb held in LDS (cb[0..M)): eight rows in flight per
+// lane, identical arithmetic (skip c == 0, ascending i).
+template <int NT, typename Real, typename CnF>
+__device__ __forceinline__ void cost_row_cb(Real *T, int ld, int M, int ncols, const Real *cb, CnF cN) {
+    constexpr int U = 8;
+    for (int j = threadIdx.x; j < ncols; j += NT) {
+        Real acc = cN(j);
+        int i0 = 0;
+        for (; i0 + U <= M; i0 += U) {
+            Real c[U], t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                c[u] = cb[i0 + u];
+                t[u] = T[(i0 + u) * ld + j];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (c[u] != Real(0)) acc = fma(-c[u], t[u], acc);
+        }
+        for (int i = i0; i < M; ++i) {
+            const Real c = cb[i];
+            if (c != Real(0)) acc = fma(-c, T[i * ld + j], acc);
+        }
+        T[M * ld + j] = acc;
+    }
+    __syncthreads();
+}
