@@ -4,7 +4,7 @@
 //                tableau of the optimality LP (replaces get_B_kernel_2 +
 //                getrf of batched_simplex.cu:715-726,1435,1478 for the cold
 //                start).
-//   feas_kernel  one 64-lane workgroup per candidate: the feasibility LP
+//   feas_kernel  one workgroup per candidate: the feasibility LP
 //                (replaces simplex_gpu_feasibility, batched_simplex.cu:
 //                990-1258, and its kernels a3/a4/a10-a15/a22/a23/a27).
 //   opt_kernel_t one workgroup per feasible candidate (or a persistent grid

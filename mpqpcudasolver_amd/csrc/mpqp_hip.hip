@@ -31,7 +31,7 @@ static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
 
 namespace {
 
-constexpr int FEAS_NT = 64;
+constexpr int FEAS_NT = 256;
 constexpr int OPT_NT = 512;
 
 template <int NT, bool FP32, bool GT>
