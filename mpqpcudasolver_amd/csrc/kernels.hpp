@@ -192,13 +192,11 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
             break;
         }
         const int r = sh_r, q = sh_q;
-        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq);
-        if (tid == 0) {
+        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq, [&] {
             const int t = basis[r];
             basis[r] = nonb[q];
             nonb[q] = t;
-        }
-        __syncthreads();
+        });
         ++iters;
     }
 

@@ -426,13 +426,11 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
     __syncthreads();
 }
 
-// Same pivot, rows updated four at a time per wave so that several LDS loads
-// are in flight (the tableau of a large LP leaves room for one workgroup per
-// CU only, so latency must be hidden inside the wave).  Per-entry arithmetic
-// is identical to pivot_team.
+// Three-phase form of the same pivot (column copy; division of row r and
+// zeroing of column q; update), four rows in flight per wave.
 template <int NT, typename Real>
-__device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols, int r, int q,
-                                           Real *colq) {
+__device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncols, int r, int q,
+                                            Real *colq) {
     const int tid = threadIdx.x;
     for (int i = tid; i < nrows; i += NT) colq[i] = T[i * ld + q];
     __syncthreads();
@@ -467,6 +465,66 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
                 }
             }
         }
+    }
+    __syncthreads();
+}
+
+// Same pivot, rows updated four at a time per wave so that several LDS loads
+// are in flight, with two barriers: (A) every thread reads p = T[r][q],
+// copies its column-q entries (rows other than r) to colq and divides its
+// pivot-row entries (columns other than q) — nobody writes T[r][q] or
+// column q here; (B) the other rows are updated, column q from 0 with the
+// factor 1/p (colq[r] is never used), then thread 0 writes T[r][q] = 1/p
+// and runs `between` (the basis bookkeeping, which no pivot step reads).  Per-entry arithmetic is
+// identical to pivot_team.
+template <int NT, typename Real, typename F>
+__device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols, int r, int q,
+                                           Real *colq, F between) {
+    if constexpr (sizeof(Real) == 8) {
+        // FP64 (one workgroup per CU): measured faster with the column copy,
+        // the division and the update in separate barrier phases
+        pivot_rows3<NT, Real>(T, ld, nrows, ncols, r, q, colq);
+        if (threadIdx.x == 0) between();
+        __syncthreads();
+        return;
+    }
+    const int tid = threadIdx.x;
+    Real *tr = T + r * ld;
+    const Real p = tr[q];
+    for (int i = tid; i < nrows; i += NT)
+        if (i != r) colq[i] = T[i * ld + q];
+    for (int j = tid; j < ncols; j += NT)
+        if (j != q) tr[j] = tr[j] / p;
+    __syncthreads();
+    const Real pinv = Real(1) / p;
+    constexpr int NW = NT / MPQP_WAVE;
+    constexpr int U = 4;
+    const int lane = tid & (MPQP_WAVE - 1);
+    const int wave = tid / MPQP_WAVE;
+    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
+        const int j = j0 + lane;
+        if (j < ncols) {
+            const bool jq = j == q;
+            const Real prj = jq ? pinv : tr[j];
+            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
+                Real t[U], f[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
+                    f[u] = i < nrows ? colq[i] : Real(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int i = i0 + u;
+                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, jq ? Real(0) : t[u]);
+                }
+            }
+        }
+    }
+    if (tid == 0) {
+        tr[q] = pinv;
+        between();
     }
     __syncthreads();
 }

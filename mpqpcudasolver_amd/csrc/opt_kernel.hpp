@@ -230,14 +230,11 @@ __device__ __forceinline__ void rows_where(int *rows, int M, OptShared &sh, F f)
 // swap basis[r] <-> nonb[q] after a pivot, keeping where[] current
 template <typename Real>
 __device__ __forceinline__ void opt_swap(OptLds<Real> &L, int r, int q) {
-    if (threadIdx.x == 0) {
-        const int t = L.basis[r];
-        L.basis[r] = L.nonb[q];
-        L.nonb[q] = t;
-        L.where[L.basis[r]] = r;
-        L.where[t] = -(q + 1);
-    }
-    __syncthreads();
+    const int t = L.basis[r];
+    L.basis[r] = L.nonb[q];
+    L.nonb[q] = t;
+    L.where[L.basis[r]] = r;
+    L.where[t] = -(q + 1);
 }
 
 // ---------------------------------------------------------------------
@@ -377,8 +374,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
         if (action == ACT_STOP) { stopped = true; break; }
         if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; return res; }
         if (action == ACT_UNPERT) continue;
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq);
-        opt_swap(L, r, q);
+        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
         OPT_PROF(sh, 2);
         ++res.iters;
         if (stall) { res.status = ST_STALL; return res; }
@@ -414,8 +410,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
         if (action != ACT_PIVOT) __syncthreads();
         if (action == ACT_STOP) break;
         if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; return res; }
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq);
-        opt_swap(L, r, q);
+        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
         ++res.iters;
     }
     if (it >= O.max_iter_p1) { res.status = ST_ITER_CAP; return res; }
@@ -467,8 +462,7 @@ __device__ __forceinline__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, 
             clean = 0;
             continue;
         }
-        pivot_rows<NT, Real>(L.T, ld, M + 1, K + 2, i, best, L.colq);
-        opt_swap(L, i, best);
+        pivot_rows<NT, Real>(L.T, ld, M + 1, K + 2, i, best, L.colq, [&] { opt_swap(L, i, best); });
     }
     return clean;
 }
@@ -550,8 +544,7 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P
             res.status = ST_UNBOUNDED;
             break;
         }
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 1, r, q, L.colq);
-        opt_swap(L, r, q);
+        pivot_rows<NT, Real>(T, ld, M + 1, K + 1, r, q, L.colq, [&] { opt_swap(L, r, q); });
         ++res.iters;
     }
     res.final_set = (res.status == ST_OPTIMAL && -res.z2 > tl.final_tol) ? 1 : 0;
