@@ -194,7 +194,8 @@ def main() -> int:
         "config": {"workload": _workload(args.config), "variant": f"{args.precision}-{args.batching}",
                    "checks_per_step": checks, "problems": len(lps),
                    "precision_used": "mixed" if st["precision_used"] else "fp64",
-                   "fallbacks_fp64": st["fallbacks"], "optimality_lps": st["opt_lps"]},
+                   "fallbacks_fp64": st["fallbacks"], "optimality_lps": st["opt_lps"],
+                   "threads_per_lp": lib.tuning()},
         "roofline": roofline(st, lds_peak, mixed and st["precision_used"] == 1),
         "stats": st,
     }

@@ -99,7 +99,10 @@ typedef struct MpqpOptions {
                                termination to max_iter_phase1 */
     int opt_threads;        /* threads per optimality workgroup; 0 = auto */
     int warm_start;         /* 1: children start from the parent's final basis */
-    int reserved0;
+    int autotune;           /* persistent batching: on the first call for a problem shape,
+                               time the workgroup sizes of both kernels on a sample of the
+                               first level with >= 2048 candidates and keep the fastest
+                               (results do not depend on it); 1 = on (default) */
     double tol_plus;           /* primal ratio test: column entry > tol_plus */
     double tol_minus;          /* dual ratio test: row entry < tol_minus */
     double tol_minus_1;        /* feasibility pricing: stop if min d > tol_minus_1 */
@@ -173,6 +176,11 @@ typedef struct MpqpStats {
 } MpqpStats;
 
 int mpqp_get_stats(MpqpStats *stats);
+
+/* Workgroup sizes (threads per LP) of the feasibility and optimality kernels
+ * used by the last mpqp() call (chosen by the autotuner in persistent mode).
+ * No counterpart in the reference. */
+int mpqp_get_tuning(int *feas_threads, int *opt_threads);
 
 /* Measures the LDS read bandwidth of the device (bytes/s) with a
  * ds_read_b128 streaming kernel over all CUs; <0 on error. */

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -59,6 +60,15 @@ const void *opt_fn(bool fp32, int nt, int S, bool gt) {
     return nt == 256 ? opt_fn_nt<256, false, false>(S) : opt_fn_nt<512, false, false>(S);
 }
 
+// The feasibility kernel at a workgroup size the autotuner may choose.
+void launch_feas(int nt, int grid, size_t lds, hipStream_t st, const FeasArgs &fa) {
+    switch (nt) {
+    case 128: hipLaunchKernelGGL(feas_kernel<128>, dim3(grid), dim3(128), lds, st, fa); break;
+    case 512: hipLaunchKernelGGL(feas_kernel<512>, dim3(grid), dim3(512), lds, st, fa); break;
+    default: hipLaunchKernelGGL(feas_kernel<256>, dim3(grid), dim3(256), lds, st, fa); break;
+    }
+}
+
 __global__ void fill_i32(int *p, int n, int v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -78,6 +88,10 @@ struct Ctx {
     long long lds_limit = 0;  // bytes of LDS one workgroup may allocate
     int n_cu = 0;
     MpqpStats stats{};
+    // autotuner: (M1, K1, M2, K2, n_c, FP32) -> (feasibility, optimality)
+    // threads per workgroup; the sizes used by the last call
+    std::map<std::vector<int>, std::pair<int, int>> tuned;
+    int last_feas_nt = 0, last_opt_nt = 0;
 };
 
 Ctx &ctx() {
@@ -108,6 +122,7 @@ void default_opts(MpqpOptions *o) {
     o->stall_perturb = 20;
     o->stall_exit = 10000;
     o->warm_start = 1;
+    o->autotune = 1;
     o->tol_plus = 1e-9;
     o->tol_minus = -1e-9;
     o->tol_minus_1 = -1e-9;
@@ -511,17 +526,37 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     const bool gt32 = opt_lds_bytes_t<float>(M2, K2, N2, true) > C.lds_limit;
     const long long lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false, gt64);
     const long long lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true, gt32);
-    const int opt_nt = (gt64 || o.opt_threads != 256) ? OPT_NT : 256;
     const int S = red_slots(std::max(M2, N2 - n_eqs));
-    const void *fn64 = opt_fn(false, opt_nt, S, gt64);
-    const void *fn32 = opt_fn(true, opt_nt, S, gt32);
-    auto capacity = [&](const void *fn, long long lds) {
+    auto capacity = [&](const void *fn, int nt, long long lds) {
         int nb = 0;
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, opt_nt, (size_t)lds));
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nt, (size_t)lds));
         return std::max(nb, 1) * std::max(C.n_cu, 1);
     };
-    const int cap64 = capacity(fn64, lds64);
-    const int cap32 = o.precision == MPQP_MIXED ? capacity(fn32, lds32) : 0;
+    // workgroup sizes: fixed, or chosen by the autotuner (persistent mode,
+    // LDS tableaus) on the first call for this problem shape
+    int feas_nt = FEAS_NT;
+    int opt_nt = (gt64 || o.opt_threads != 256) ? OPT_NT : 256;
+    const std::vector<int> tkey{M1, K1, M2, K2, n_c, o.precision == MPQP_MIXED ? 1 : 0};
+    bool tune_pending = false;
+    if (o.autotune && o.batching == MPQP_PERSISTENT && !gt64 && !gt32 && o.opt_threads == 0) {
+        auto it = C.tuned.find(tkey);
+        if (it != C.tuned.end()) {
+            feas_nt = it->second.first;
+            opt_nt = it->second.second;
+        } else {
+            tune_pending = true;
+        }
+    }
+    const void *fn64 = nullptr, *fn32 = nullptr;
+    int cap64 = 0, cap32 = 0;
+    auto select_opt = [&](int nt) {
+        opt_nt = nt;
+        fn64 = opt_fn(false, opt_nt, S, gt64);
+        fn32 = opt_fn(true, opt_nt, S, gt32);
+        cap64 = capacity(fn64, opt_nt, lds64);
+        cap32 = o.precision == MPQP_MIXED ? capacity(fn32, opt_nt, lds32) : 0;
+    };
+    select_opt(opt_nt);
     DBuf gtab64, gtab32;
     if (gt64) gtab64.alloc((size_t)opt_gtab_bytes<double>(M2, K2) * cap64);
     if (gt32 && o.precision == MPQP_MIXED) gtab32.alloc((size_t)opt_gtab_bytes<float>(M2, K2) * cap32);
@@ -655,8 +690,30 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             fa.store = store ? 1 : 0;
             fa.rec = drec.as<DevRecord>();
             fa.fbasis = bases ? dfb.as<int>() : nullptr;
+            // autotuner: on a sample of the first level with >= 2048
+            // candidates (or the last level), time each workgroup size; the
+            // real launches below overwrite what the trials wrote
+            const bool tune_here = tune_pending && (n >= 2048 || lvl == last_lvl);
+            if (tune_here) {
+                FeasArgs ft = fa;
+                ft.n = std::min(n, 8192);
+                float best = INFINITY;
+                for (int nt : {128, 256, 512}) {
+                    HIP_OK(hipEventRecord(e0, st));
+                    launch_feas(nt, ft.n, (size_t)flds, st, ft);
+                    HIP_OK(hipGetLastError());
+                    HIP_OK(hipEventRecord(e1, st));
+                    HIP_OK(hipEventSynchronize(e1));
+                    float tms = 0.f;
+                    HIP_OK(hipEventElapsedTime(&tms, e0, e1));
+                    if (tms < best) {
+                        best = tms;
+                        feas_nt = nt;
+                    }
+                }
+            }
             HIP_OK(hipEventRecord(e0, st));
-            hipLaunchKernelGGL(feas_kernel<FEAS_NT>, dim3(n), dim3(FEAS_NT), (size_t)flds, st, fa);
+            launch_feas(feas_nt, n, (size_t)flds, st, fa);
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(e1, st));
             hrec = pinned_records<DevRecord>((size_t)n);
@@ -701,6 +758,40 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 oa.rec = drec.as<DevRecord>();
                 oa.b1 = bases ? db1.as<int>() : nullptr;
                 oa.b2 = bases ? db2.as<int>() : nullptr;
+                if (tune_here) {
+                    // time the first kernel of the level at each workgroup
+                    // size on a sample of the list (counters reset per trial)
+                    if (mixed) dfbl = pool(PB_FBL).get(sizeof(int) * (size_t)nf);
+                    float best = INFINITY;
+                    int best_nt = opt_nt;
+                    for (int nt : {256, 512}) {
+                        select_opt(nt);
+                        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
+                        HIP_OK(hipGetLastError());
+                        OptArgs ot = oa;
+                        ot.n = std::min(nf, 8192);
+                        ot.counter = dctr.as<int>();
+                        if (mixed) {
+                            ot.fb_list = dfbl.as<int>();
+                            ot.fb_count = dctr.as<int>() + 1;
+                        }
+                        HIP_OK(hipEventRecord(e2, st));
+                        launch(mixed ? fn32 : fn64, std::min(ot.n, mixed ? cap32 : cap64), mixed ? lds32 : lds64, ot);
+                        HIP_OK(hipEventRecord(e3, st));
+                        HIP_OK(hipEventSynchronize(e3));
+                        float tms = 0.f;
+                        HIP_OK(hipEventElapsedTime(&tms, e2, e3));
+                        if (tms < best) {
+                            best = tms;
+                            best_nt = nt;
+                        }
+                    }
+                    select_opt(best_nt);
+                    hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
+                    HIP_OK(hipGetLastError());
+                    C.tuned[tkey] = {feas_nt, opt_nt};
+                    tune_pending = false;
+                }
                 HIP_OK(hipEventRecord(e1, st));
                 if (mixed) {
                     dfbl = pool(PB_FBL).get(sizeof(int) * (size_t)nf);
@@ -835,6 +926,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         mixed = false;
     }
     C.stats.precision_used = mixed ? MPQP_MIXED : MPQP_FP64;
+    C.last_feas_nt = feas_nt;
+    C.last_opt_nt = opt_nt;
     HIP_OK(hipStreamSynchronize(st));
     const double t_gpu = now_s() - t_start;
 
@@ -951,6 +1044,13 @@ int mpqp_get_stats(MpqpStats *stats) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!stats) return -1;
     *stats = ctx().stats;
+    return 0;
+}
+
+int mpqp_get_tuning(int *feas_threads, int *opt_threads) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (feas_threads) *feas_threads = ctx().last_feas_nt;
+    if (opt_threads) *opt_threads = ctx().last_opt_nt;
     return 0;
 }
 

@@ -70,7 +70,7 @@ class MpqpOptions(ctypes.Structure):
         ("stall_exit", ctypes.c_int),
         ("opt_threads", ctypes.c_int),
         ("warm_start", ctypes.c_int),
-        ("reserved0", ctypes.c_int),
+        ("autotune", ctypes.c_int),
         ("tol_plus", ctypes.c_double),
         ("tol_minus", ctypes.c_double),
         ("tol_minus_1", ctypes.c_double),
@@ -247,6 +247,9 @@ class MpqpLib:
             L.mpqp_get_stats.restype = ctypes.c_int
         if hasattr(L, "mpqp_measure_lds_bandwidth"):
             L.mpqp_measure_lds_bandwidth.restype = ctypes.c_double
+        if hasattr(L, "mpqp_get_tuning"):
+            L.mpqp_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+            L.mpqp_get_tuning.restype = ctypes.c_int
 
     @property
     def backend(self) -> str:
@@ -254,6 +257,13 @@ class MpqpLib:
 
     def warm_up(self) -> int:
         return self.lib.warm_up()
+
+    def tuning(self) -> dict:
+        """Threads per workgroup of the feasibility and optimality kernels in
+        the last call (the autotuner's choice in persistent mode)."""
+        f, o = ctypes.c_int(0), ctypes.c_int(0)
+        self.lib.mpqp_get_tuning(ctypes.byref(f), ctypes.byref(o))
+        return {"feasibility": f.value, "optimality": o.value}
 
     def stats(self) -> dict:
         st = MpqpStats()

@@ -133,6 +133,7 @@ void mpqp_default_options(MpqpOptions *o) {
     o->stall_exit = 10000;
     o->opt_threads = 0;
     o->warm_start = 1;
+    o->autotune = 1; /* no effect on the CPU */
     o->tol_plus = 1e-9;
     o->tol_minus = -1e-9;
     o->tol_minus_1 = -1e-9;
@@ -1338,6 +1339,12 @@ int mpqp_get_stats(MpqpStats *st) {
 }
 
 double mpqp_measure_lds_bandwidth(void) { return -1.0; }
+/* No workgroups on the CPU: both sizes reported as 0. */
+int mpqp_get_tuning(int *feas_threads, int *opt_threads) {
+    if (feas_threads) *feas_threads = 0;
+    if (opt_threads) *opt_threads = 0;
+    return 0;
+}
 
 int mpqp_get_records(MpqpRecord *records, int *sets, int *feas_basis, int *p1_basis,
                      int *p2_basis, int max_records) {

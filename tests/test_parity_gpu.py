@@ -356,6 +356,29 @@ def test_persistent_batching_matches_plain(hip):
     assert compare_runs(a, b) == []
 
 
+def test_autotuned_launches_match_fixed(hip):
+    """The autotuner (persistent mode) only picks workgroup sizes: the records
+    of an autotuned mixed run equal those of a run at fixed sizes, and the
+    sizes it reports are ones it may choose."""
+    _log("case: autotuned vs fixed workgroup sizes, mixed persistent, n=30 m=62 p=8, |S|<=3")
+    prog, _ = config_problem(1)
+    lp = assemble_lps(prog)
+    hip.set_options(record=2, precision=1, batching=1, autotune=0)
+    a = hip.run(lp, k_max=3, records=True, bases=True)
+    fixed = hip.tuning()
+    hip.set_options(record=2, precision=1, batching=1, autotune=1)
+    b = hip.run(lp, k_max=3, records=True, bases=True)
+    tuned = hip.tuning()
+    c = hip.run(lp, k_max=3, records=True, bases=True)
+    again = hip.tuning()
+    hip.set_options()
+    assert compare_runs(a, b) == []
+    assert compare_runs(a, c) == []
+    assert fixed == {"feasibility": 256, "optimality": 512}
+    assert tuned["feasibility"] in (128, 256, 512) and tuned["optimality"] in (256, 512)
+    assert again == tuned  # cached per problem shape
+
+
 def test_errors_are_reported(hip):
     _log("case: invalid input")
     prog, k = config_problem(0)
