@@ -282,8 +282,40 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
                 cn[s] = valid ? L.bcnt[L.basis[i]] : 0;
                 neg |= valid && (double)x < tl.dual_stop_tol;
             }
-            tree_argmin_s<S>(v, id, cn, M, lane, DminPred{(n_same % 20) >= 10 ? 1 : 0, tl.dual_stop_tol});
-            const int r = rfl(id[0]);
+            // Leaving row.  In the plain mode (no counter preference) the
+            // reference tree keeps a minimum m that no other value u reaches
+            // within its 1e-10 tie band (m > v_u - 1e-10 false for every u,
+            // the predicate's own expression): such an m wins every combine
+            // on its way to the root, whatever the order.  That case (no NaN)
+            // is taken by a wave minimum; anything else runs the tree.  FP32
+            // only: FP64 values tie exactly often enough (degenerate bases)
+            // that the shortcut measured slower there.
+            const int mode_b = (n_same % 20) >= 10 ? 1 : 0;
+            int r = -1;
+            if (sizeof(Real) == 4 && !mode_b) {
+                double ml = INFINITY;
+                bool nan = false;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const bool valid = lane + MPQP_WAVE * s < M;
+                    nan |= valid && v[s] != v[s];
+                    ml = (valid && v[s] < ml) ? v[s] : ml;
+                }
+                const double m = -wave_max(-ml);
+                int close = 0, at = -1;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const bool valid = lane + MPQP_WAVE * s < M;
+                    close += __popcll(__ballot(valid && (v[s] == m || m > v[s] - 1e-10)));
+                    const unsigned long long hit = __ballot(valid && v[s] == m);
+                    at = (at < 0 && hit) ? (__ffsll((long long)hit) - 1) + MPQP_WAVE * s : at;
+                }
+                if (!__any(nan) && close == 1) r = at;
+            }
+            if (r < 0) {
+                tree_argmin_s<S>(v, id, cn, M, lane, DminPred{mode_b, tl.dual_stop_tol});
+                r = rfl(id[0]);
+            }
             OPT_PROF_DEC(sh, 8);
             if (lane == 0) L.bcnt[L.basis[r]] += 1;
             const bool go_on = __any(neg);
