@@ -51,6 +51,37 @@ def test_tree_counter_tiebreak(orc):
     assert orc.tree_argmin_dmin(v, [0, 3]) == 0
 
 
+def test_unique_minimum_wins_the_leaving_row_tree():
+    """The FP32 leaving-row shortcut of the GPU kernel: in the plain mode, a
+    minimum m with (m > v - 1e-10) false for every other value v is what the
+    reference tree returns, whatever the counters.  Brute force over random
+    values (including near-ties just outside the band, large magnitudes where
+    1e-10 is below one ulp, and exact ties, where the shortcut must not
+    apply) and random counters."""
+    rng = np.random.default_rng(7)
+    pred = pred_dmin(0, -1e-9)
+    applied = 0
+    for trial in range(4000):
+        n = int(rng.integers(2, 200))
+        scale = [1.0, 1e-6, 1e6][trial % 3]
+        v = list((rng.normal(size=n) * scale).astype(float))
+        if trial % 5 == 0:  # a near-tie just outside the band
+            j, k = rng.choice(n, 2, replace=False)
+            v[k] = v[j] + 1.5e-10 * (1 if trial % 2 else -1)
+        if trial % 7 == 0:  # an exact tie at the minimum
+            j = int(np.argmin(v))
+            v[(j + 1) % n] = v[j]
+        cnt = list(rng.integers(0, 30, size=n))
+        m = min(v)
+        close = sum(1 for x in v if x == m or m > x - 1e-10)
+        if close != 1:
+            continue
+        applied += 1
+        idx, val = tree_argmin(pred, v, list(range(n)), cnt)
+        assert val == m and idx == v.index(m)
+    assert applied > 2000
+
+
 def test_tree_sum_order(orc):
     # 1e16 + 1 rounds away the 1; the reference tree adds (0,2) and (1,3)
     # first: (1e16 + -1e16) + (1 + 1) = 2, a left fold would give 1.
