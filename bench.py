@@ -71,14 +71,16 @@ def cpu_baseline(lp, k_sample: int) -> dict:
 
 def timed(lib, lps, k_max, steps, warmup, **opts):
     """A step is one mpqp() call per problem of the workload (one problem
-    except for the 64-problem workload); returns the last step's summed stats
-    and the wall time of the timed steps."""
+    except for the 64-problem workload), through the C ABI; turning the
+    returned active-set array into Python lists is the caller's and is not
+    timed.  Returns the last step's summed stats and the wall time of the
+    timed steps."""
     lib.set_options(**opts)
     for _ in range(warmup):
-        lib.run_many(lps, k_max)
+        lib.run_many(lps, k_max, decode=False)
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, st = lib.run_many(lps, k_max)
+        _, st = lib.run_many(lps, k_max, decode=False)
     dt = time.perf_counter() - t0
     return st, dt
 

@@ -312,7 +312,7 @@ class MpqpLib:
             lp.M2, lp.N2, lp.n_eqs, reserved_memory)
 
     def run(self, lp: LPData, k_max: int, k_min: int = 0, reserved_memory: int = 0,
-            records: bool = False, bases: bool = False) -> MpqpRun:
+            records: bool = False, bases: bool = False, decode: bool = True) -> MpqpRun:
         res = self.mpqp_raw(lp, k_min, k_max, reserved_memory)
         if not res:
             raise RuntimeError(f"mpqp failed: {self.last_error()}")
@@ -321,7 +321,7 @@ class MpqpLib:
             n_batches=r.n_batches, n_splits=r.n_splits, n_feasible_lps=r.n_feasible_lps,
             n_final_lps=r.n_final_lps, time_feas=r.time_feas, time_opti=r.time_opti,
             time_GPU=r.time_GPU,
-            active_sets=decode_active_sets(r.active_sets, r.n_final_lps, lp.n_x),
+            active_sets=decode_active_sets(r.active_sets, r.n_final_lps, lp.n_x) if decode else None,
         )
         if records:
             n = self.lib.mpqp_get_records(None, None, None, None, None, 0)
@@ -342,13 +342,13 @@ class MpqpLib:
         self.lib.mpqp_free_results(res)
         return run
 
-    def run_many(self, lps, k_max: int, records: bool = False) -> tuple[list, dict]:
+    def run_many(self, lps, k_max: int, records: bool = False, decode: bool = True) -> tuple[list, dict]:
         """One mpqp() call per problem, as the reference's host loops over its
         problems (main.py:60-170); returns the runs and the summed stats
         (counts and kernel times added, precision_used of the last call)."""
         runs, tot = [], None
         for lp in lps:
-            runs.append(self.run(lp, k_max=k_max, records=records))
+            runs.append(self.run(lp, k_max=k_max, records=records, decode=decode))
             st = self.stats()
             if tot is None:
                 tot = dict(st)
