@@ -226,6 +226,49 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
 }
 
 // ---------------------------------------------------------------------
+// Level-wise enumeration on the device (persistent batching): the children
+// of the feasible parents, S + {j} for j above max S (in parent order, j
+// ascending, as the host enumeration), and the list of feasible candidates.
+// Counts are turned into offsets by an exclusive scan between the kernels.
+// ---------------------------------------------------------------------
+__global__ void child_count(const DevRecord *prec, const int *psets, int pn, int kw, int level, int n_c, int *cnt) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= pn) return;
+    const int mx = level >= 2 ? psets[(long long)pi * kw + level - 2] : -1;
+    cnt[pi] = prec[pi].feasible ? n_c - 1 - mx : 0;
+}
+
+// p1par[child] = parent whenever the parent's phase 1 ended optimal (its
+// phase-1 state warm-starts the child), else -1
+__global__ void child_write(const DevRecord *prec, const int *psets, const int *off, const int *cnt, int pn, int kw,
+                            int level, int *sets, int *par, int *p1par) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= pn) return;
+    const int c = cnt[pi];
+    if (c == 0) return;
+    const int *ps = psets + (long long)pi * kw;
+    const int mx = level >= 2 ? ps[level - 2] : -1;
+    const int warm = prec[pi].p1_status == ST_OPTIMAL ? pi : -1;
+    const int o = off[pi];
+    for (int t = 0; t < c; ++t) {
+        int *cs = sets + (long long)(o + t) * kw;
+        for (int u = 0; u < kw; ++u) cs[u] = u < level - 1 ? ps[u] : (u == level - 1 ? mx + 1 + t : -1);
+        par[o + t] = pi;
+        p1par[o + t] = warm;
+    }
+}
+
+__global__ void feasible_flag(const DevRecord *rec, int n, int *flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) flag[c] = rec[c].feasible ? 1 : 0;
+}
+
+__global__ void list_write(const int *flag, const int *pos, int n, int *list) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n && flag[c]) list[pos[c]] = c;
+}
+
+// ---------------------------------------------------------------------
 // Optimality LP
 // ---------------------------------------------------------------------
 // slots per lane a reduction over n elements needs on one wave

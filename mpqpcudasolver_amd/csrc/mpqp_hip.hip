@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/mpqp_hip.h"
 #include "kernels.hpp"
 
@@ -221,10 +223,21 @@ struct PoolBuf {
         return BufView{bytes ? p : nullptr, bytes};
     }
 };
-enum { PB_SETS, PB_PAR, PB_REC, PB_LIST, PB_P1, PB_FBL, PB_FB, PB_B1, PB_B2, PB_FS, PB_OS = PB_FS + 2, PB_N = PB_OS + 2 };
+enum {
+    PB_SETS, PB_PAR, PB_REC, PB_LIST, PB_P1, PB_FBL, PB_FB, PB_B1, PB_B2, PB_CNT, PB_OFF, PB_SCANTMP,
+    PB_FS, PB_OS = PB_FS + 2, PB_N = PB_OS + 2
+};
 PoolBuf &pool(int k) {
     static PoolBuf b[PB_N];
     return b[k];
+}
+// per-level buffers of the device enumeration, kept until the end of a
+// call: kind 0 = sets, 1 = parents, 2 = records
+PoolBuf &level_pool(int kind, int lvl) {
+    static std::vector<PoolBuf> b;
+    const size_t i = (size_t)lvl * 3 + (size_t)kind;
+    if (b.size() <= i) b.resize(i + 1);
+    return b[i];
 }
 
 // Pinned host staging for the per-level records (allocated with
@@ -595,6 +608,89 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     int n_batches = 0, n_feas_total = 0;
     double tf = 0.0, to = 0.0;
 
+    // Per-level statistics (work counters from the records) and outputs
+    // (final sets; records and sets for mpqp_get_records)
+    auto level_stats = [&](const DevRecord *hr, int n, int nf, bool mx) {
+        MpqpStats &St = C.stats;
+        St.levels += 1;
+        St.launches_feas += 1;
+        St.candidates += n;
+        St.opt_lps += nf;
+        long long pf = 0, p1 = 0, p2 = 0, fbn = 0;
+        double b_opt = 0.0, f_opt = 0.0;
+        const double Kp2 = (double)(N2 - n_eqs - M2);
+        auto bytes = [&](double Mr, double Kc, double s) { return 2.0 * (Mr + 1) * (Kc + 1) * s + (Mr + Kc + 2) * s; };
+        auto flops = [&](double Mr, double Kc) { return 2.0 * (Mr + 1) * (Kc + 1); };
+        for (int c = 0; c < n; ++c) {
+            pf += hr[c].feas_iters;
+            p1 += hr[c].p1_iters;
+            p2 += hr[c].p2_iters;
+            fbn += hr[c].repivots;
+            // FP32 pivots unless the LP was re-solved in FP64
+            const double s = (mx && !hr[c].repivots) ? 4.0 : 8.0;
+            const double bc = hr[c].p1_iters * bytes(M2, K2, s) + hr[c].p2_iters * bytes(M2, Kp2, s);
+            const double fc = hr[c].p1_iters * flops(M2, K2) + hr[c].p2_iters * flops(M2, Kp2);
+            b_opt += bc;
+            f_opt += fc;
+            if (mx && hr[c].repivots) {
+                St.bytes_fallback += bc;
+                St.flops_fallback += fc;
+            }
+        }
+        St.fallbacks += fbn;
+        St.pivots_feas += pf;
+        St.pivots_p1 += p1;
+        St.pivots_p2 += p2;
+        St.bytes_feas += pf * bytes(M1, K1, 8.0);
+        St.flops_feas += pf * flops(M1, K1);
+        St.bytes_opt += b_opt;
+        St.flops_opt += f_opt;
+    };
+    std::vector<int> prev_recidx;
+    auto level_outputs = [&](const DevRecord *hr, const int *sets, const int *par, int n, int lvl) {
+        const int base = (int)all.size();
+        for (int c = 0; c < n; ++c)
+            if (hr[c].final_set && lvl >= k_min) fin.insert(fin.end(), sets + (size_t)c * kw, sets + (size_t)(c + 1) * kw);
+        if (keep_rec) {
+            all.resize((size_t)base + n);
+            for (int c = 0; c < n; ++c) {
+                MpqpRecord &r = all[(size_t)base + c];
+                std::memcpy(&r, &hr[c], sizeof r);
+                r.level = lvl;
+                r.parent = par[c] >= 0 ? prev_recidx[par[c]] : -1;
+            }
+            allsets.insert(allsets.end(), sets, sets + (size_t)n * kw);
+        }
+        prev_recidx.resize(n);
+        for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
+    };
+
+    // Enumeration on the device in persistent mode (children, feasible
+    // lists and records stay on the device; the host reads two counts per
+    // level and, at the end, the records); on the host in plain mode.
+    const bool devenum = o.batching == MPQP_PERSISTENT || o.precision == MPQP_MIXED;
+    struct LevelOut {
+        int n, nf;
+        BufView sets, par, rec;
+    };
+    std::vector<LevelOut> lv;
+    // exclusive scan of n ints (pooled temporary storage)
+    auto scan = [&](const int *in, int *out, int nn) {
+        size_t tb = 0;
+        HIP_OK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, nn, st));
+        const BufView tmp = pool(PB_SCANTMP).get(std::max<size_t>(tb, 16));
+        HIP_OK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, in, out, nn, st));
+    };
+    // total of a scanned count array: out[nn-1] + in[nn-1]
+    auto scan_total = [&](const int *in, const int *out, int nn) {
+        int t[2] = {0, 0};
+        HIP_OK(hipMemcpyAsync(&t[0], out + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&t[1], in + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        return t[0] + t[1];
+    };
+    auto grid = [](int nn) { return dim3((unsigned)((nn + 255) / 256)); };
+
     // Mixed precision runs levels 0 and 1 with FP32 pivots first; if more
     // than a quarter of their optimality LPs fail certification, the
     // problem is too ill-conditioned for FP32 pivots and the whole run
@@ -608,6 +704,8 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         allfb.clear();
         allb1.clear();
         allb2.clear();
+        lv.clear();
+        prev_recidx.clear();
         n_batches = n_feas_total = 0;
         tf = to = 0.0;
         C.stats = MpqpStats{};
@@ -617,48 +715,81 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         const long long ostride = mixed ? opt_state_bytes<float>(M2, K2) : opt_state_bytes<double>(M2, K2);
 
         std::vector<int> prev_sets, prev_feas, prev_warm;
-        std::vector<int> prev_recidx;
         BufView fstore_prev, ostore_prev;
         int prev_n = 0;
         std::vector<int> cur_sets, cur_par;
         DevRecord *hrec = nullptr;
 
         for (int lvl = 0; lvl <= last_lvl; ++lvl) {
-            cur_sets.clear();
-            cur_par.clear();
-            if (lvl == 0) {
-                cur_sets.assign(kw, -1);
-                cur_par.push_back(-1);
+            int n = 0;
+            BufView dsets, dpar, drec, dp1v;
+            if (devenum) {
+                if (lvl == 0) {
+                    n = 1;
+                    std::vector<int> h(kw + 2, -1);
+                    dsets = level_pool(0, lvl).get(sizeof(int) * kw);
+                    dpar = level_pool(1, lvl).get(sizeof(int));
+                    dp1v = pool(PB_P1).get(sizeof(int));
+                    HIP_OK(hipMemcpyAsync(dsets.p, h.data(), dsets.n, hipMemcpyHostToDevice, st));
+                    HIP_OK(hipMemcpyAsync(dpar.p, h.data(), sizeof(int), hipMemcpyHostToDevice, st));
+                    HIP_OK(hipMemcpyAsync(dp1v.p, h.data(), sizeof(int), hipMemcpyHostToDevice, st));
+                } else {
+                    const LevelOut &pl = lv.back();
+                    const int pn = pl.n;
+                    const BufView cnt = pool(PB_CNT).get(sizeof(int) * (size_t)pn);
+                    const BufView off = pool(PB_OFF).get(sizeof(int) * (size_t)pn);
+                    hipLaunchKernelGGL(child_count, grid(pn), dim3(256), 0, st, pl.rec.as<const DevRecord>(),
+                                       pl.sets.as<const int>(), pn, kw, lvl, n_c, cnt.as<int>());
+                    HIP_OK(hipGetLastError());
+                    scan(cnt.as<const int>(), off.as<int>(), pn);
+                    n = scan_total(cnt.as<const int>(), off.as<const int>(), pn);
+                    if (n == 0) break;
+                    dsets = level_pool(0, lvl).get(sizeof(int) * (size_t)n * kw);
+                    dpar = level_pool(1, lvl).get(sizeof(int) * (size_t)n);
+                    dp1v = pool(PB_P1).get(sizeof(int) * (size_t)n);
+                    hipLaunchKernelGGL(child_write, grid(pn), dim3(256), 0, st, pl.rec.as<const DevRecord>(),
+                                       pl.sets.as<const int>(), off.as<const int>(), cnt.as<const int>(), pn, kw, lvl,
+                                       dsets.as<int>(), dpar.as<int>(), dp1v.as<int>());
+                    HIP_OK(hipGetLastError());
+                }
+                drec = level_pool(2, lvl).get(sizeof(DevRecord) * (size_t)n);
             } else {
-                // children of the feasible parents: S + {j}, j above max(S)
-                size_t nch = 0;
-                for (int pi = 0; pi < prev_n; ++pi)
-                    if (prev_feas[pi]) nch += (size_t)(n_c - 1 - (lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1));
-                cur_sets.resize(nch * kw);
-                cur_par.resize(nch);
-                size_t c = 0;
-                for (int pi = 0; pi < prev_n; ++pi) {
-                    if (!prev_feas[pi]) continue;
-                    const int *ps = prev_sets.data() + (size_t)pi * kw;
-                    const int mx = lvl >= 2 ? ps[lvl - 2] : -1;
-                    for (int j = mx + 1; j < n_c; ++j, ++c) {
-                        int *cs = cur_sets.data() + c * kw;
-                        for (int t = 0; t < kw; ++t) cs[t] = t < lvl - 1 ? ps[t] : (t == lvl - 1 ? j : -1);
-                        cur_par[c] = pi;
+                cur_sets.clear();
+                cur_par.clear();
+                if (lvl == 0) {
+                    cur_sets.assign(kw, -1);
+                    cur_par.push_back(-1);
+                } else {
+                    // children of the feasible parents: S + {j}, j above max(S)
+                    size_t nch = 0;
+                    for (int pi = 0; pi < prev_n; ++pi)
+                        if (prev_feas[pi]) nch += (size_t)(n_c - 1 - (lvl >= 2 ? prev_sets[(size_t)pi * kw + lvl - 2] : -1));
+                    cur_sets.resize(nch * kw);
+                    cur_par.resize(nch);
+                    size_t c = 0;
+                    for (int pi = 0; pi < prev_n; ++pi) {
+                        if (!prev_feas[pi]) continue;
+                        const int *ps = prev_sets.data() + (size_t)pi * kw;
+                        const int mx = lvl >= 2 ? ps[lvl - 2] : -1;
+                        for (int j = mx + 1; j < n_c; ++j, ++c) {
+                            int *cs = cur_sets.data() + c * kw;
+                            for (int t = 0; t < kw; ++t) cs[t] = t < lvl - 1 ? ps[t] : (t == lvl - 1 ? j : -1);
+                            cur_par[c] = pi;
+                        }
                     }
                 }
+                n = (int)cur_par.size();
+                if (n == 0) break;
+                dsets = pool(PB_SETS).get(sizeof(int) * (size_t)n * kw);
+                dpar = pool(PB_PAR).get(sizeof(int) * (size_t)n);
+                drec = pool(PB_REC).get(sizeof(DevRecord) * (size_t)n);
+                HIP_OK(hipMemcpyAsync(dsets.p, cur_sets.data(), dsets.n, hipMemcpyHostToDevice, st));
+                HIP_OK(hipMemcpyAsync(dpar.p, cur_par.data(), dpar.n, hipMemcpyHostToDevice, st));
             }
-            const int n = (int)cur_par.size();
-            if (n == 0) break;
             const bool probe_only = lvl > k_max;
             const bool store = lvl < last_lvl;
             const bool bases = keep_bases && !probe_only;
 
-            const BufView dsets = pool(PB_SETS).get(sizeof(int) * (size_t)n * kw);
-            const BufView dpar = pool(PB_PAR).get(sizeof(int) * (size_t)n);
-            const BufView drec = pool(PB_REC).get(sizeof(DevRecord) * (size_t)n);
-            HIP_OK(hipMemcpyAsync(dsets.p, cur_sets.data(), dsets.n, hipMemcpyHostToDevice, st));
-            HIP_OK(hipMemcpyAsync(dpar.p, cur_par.data(), dpar.n, hipMemcpyHostToDevice, st));
             // stored states ping-pong between two pooled buffers by level parity
             BufView fstore_cur, ostore_cur;
             if (store) {
@@ -716,30 +847,52 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             launch_feas(feas_nt, n, (size_t)flds, st, fa);
             HIP_OK(hipGetLastError());
             HIP_OK(hipEventRecord(e1, st));
-            hrec = pinned_records<DevRecord>((size_t)n);
-            HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            float ms = 0.f;
-            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-            tf += ms * 1e-3;
-            C.stats.ms_feas += ms;
 
             // optimality LPs of the feasible candidates; warm start from the
             // parent's final phase-1 state
-            std::vector<int> list, p1par(n, -1);
-            for (int c = 0; c < n; ++c) {
-                if (hrec[c].feasible) list.push_back(c);
-                const int pi = cur_par[c];
-                if (pi >= 0 && prev_warm[pi]) p1par[c] = pi;
+            int nf = 0;
+            BufView dlist, dp1;
+            std::vector<int> list;
+            if (devenum) {
+                const BufView flag = pool(PB_CNT).get(sizeof(int) * (size_t)n);
+                const BufView pos = pool(PB_OFF).get(sizeof(int) * (size_t)n);
+                hipLaunchKernelGGL(feasible_flag, grid(n), dim3(256), 0, st, drec.as<const DevRecord>(), n, flag.as<int>());
+                HIP_OK(hipGetLastError());
+                scan(flag.as<const int>(), pos.as<int>(), n);
+                nf = scan_total(flag.as<const int>(), pos.as<const int>(), n);
+                if (nf > 0) {
+                    dlist = pool(PB_LIST).get(sizeof(int) * (size_t)nf);
+                    hipLaunchKernelGGL(list_write, grid(n), dim3(256), 0, st, flag.as<const int>(), pos.as<const int>(), n,
+                                       dlist.as<int>());
+                    HIP_OK(hipGetLastError());
+                }
+                dp1 = dp1v;
+            } else {
+                hrec = pinned_records<DevRecord>((size_t)n);
+                HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                std::vector<int> p1par(n, -1);
+                for (int c = 0; c < n; ++c) {
+                    if (hrec[c].feasible) list.push_back(c);
+                    const int pi = cur_par[c];
+                    if (pi >= 0 && prev_warm[pi]) p1par[c] = pi;
+                }
+                nf = (int)list.size();
+                if (nf > 0) {
+                    dlist = pool(PB_LIST).get(sizeof(int) * (size_t)nf);
+                    dp1 = pool(PB_P1).get(sizeof(int) * (size_t)n);
+                    HIP_OK(hipMemcpyAsync(dlist.p, list.data(), dlist.n, hipMemcpyHostToDevice, st));
+                    HIP_OK(hipMemcpyAsync(dp1.p, p1par.data(), dp1.n, hipMemcpyHostToDevice, st));
+                }
             }
-            const int nf = (int)list.size();
+            float ms = 0.f;
+            HIP_OK(hipEventSynchronize(e1));
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            tf += ms * 1e-3;
+            C.stats.ms_feas += ms;
             int n_fb = 0;
             if (nf > 0) {
-                const BufView dlist = pool(PB_LIST).get(sizeof(int) * (size_t)nf);
-                const BufView dp1 = pool(PB_P1).get(sizeof(int) * (size_t)n);
                 BufView dfbl;
-                HIP_OK(hipMemcpyAsync(dlist.p, list.data(), dlist.n, hipMemcpyHostToDevice, st));
-                HIP_OK(hipMemcpyAsync(dp1.p, p1par.data(), dp1.n, hipMemcpyHostToDevice, st));
                 hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
                 HIP_OK(hipGetLastError());
                 OptArgs oa{};
@@ -823,8 +976,12 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     launch(fn64, nf, lds64, oa);
                 }
                 HIP_OK(hipEventRecord(e2, st));
-                HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
-                HIP_OK(hipStreamSynchronize(st));
+                if (devenum) {
+                    HIP_OK(hipEventSynchronize(e2));
+                } else {
+                    HIP_OK(hipMemcpyAsync(hrec, drec.p, drec.n, hipMemcpyDeviceToHost, st));
+                    HIP_OK(hipStreamSynchronize(st));
+                }
                 HIP_OK(hipEventElapsedTime(&ms, e1, e2));
                 to += ms * 1e-3;
                 C.stats.ms_opt += ms;
@@ -834,7 +991,13 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                     C.stats.ms_fallback += ms;
                     C.stats.launches_fallback += 1;
                 }
-                for (int c = 0; c < n; ++c) n_fb += hrec[c].repivots;
+                if (!devenum) {
+                    for (int c = 0; c < n; ++c) n_fb += hrec[c].repivots;
+                } else if (mixed && lvl <= 1) {
+                    // the FP32 launch counted its certification failures
+                    HIP_OK(hipMemcpyAsync(&n_fb, dctr.as<int>() + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+                    HIP_OK(hipStreamSynchronize(st));
+                }
             }
             if (mixed && lvl <= 1) {
                 probe_opt += nf;
@@ -847,60 +1010,20 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             if (probe_only) break;
             n_feas_total += nf;
             ++n_batches;
-            {
-                MpqpStats &St = C.stats;
-                St.levels += 1;
-                St.launches_feas += 1;
-                St.candidates += n;
-                St.opt_lps += nf;
-                St.fallbacks += n_fb;
-                long long pf = 0, p1 = 0, p2 = 0;
-                double b_opt = 0.0, f_opt = 0.0;
-                const double Kp2 = (double)(N2 - n_eqs - M2);
-                auto bytes = [&](double Mr, double Kc, double s) {
-                    return 2.0 * (Mr + 1) * (Kc + 1) * s + (Mr + Kc + 2) * s;
-                };
-                auto flops = [&](double Mr, double Kc) { return 2.0 * (Mr + 1) * (Kc + 1); };
+            if (devenum) {
+                lv.push_back(LevelOut{n, nf, dsets, dpar, drec});
+            } else {
+                level_stats(hrec, n, nf, mixed);
+                level_outputs(hrec, cur_sets.data(), cur_par.data(), n, lvl);
+                prev_sets.swap(cur_sets);
+                prev_feas.assign(n, 0);
+                prev_warm.assign(n, 0);
                 for (int c = 0; c < n; ++c) {
-                    pf += hrec[c].feas_iters;
-                    p1 += hrec[c].p1_iters;
-                    p2 += hrec[c].p2_iters;
-                    // FP32 pivots unless the LP was re-solved in FP64
-                    const double s = (mixed && !hrec[c].repivots) ? 4.0 : 8.0;
-                    const double bc = hrec[c].p1_iters * bytes(M2, K2, s) + hrec[c].p2_iters * bytes(M2, Kp2, s);
-                    const double fc = hrec[c].p1_iters * flops(M2, K2) + hrec[c].p2_iters * flops(M2, Kp2);
-                    b_opt += bc;
-                    f_opt += fc;
-                    if (mixed && hrec[c].repivots) {
-                        St.bytes_fallback += bc;
-                        St.flops_fallback += fc;
-                    }
+                    prev_feas[c] = hrec[c].feasible;
+                    // a phase-1 state warm-starts the children whenever phase 1 ended optimal
+                    prev_warm[c] = hrec[c].p1_status == ST_OPTIMAL;
                 }
-                St.pivots_feas += pf;
-                St.pivots_p1 += p1;
-                St.pivots_p2 += p2;
-                St.bytes_feas += pf * bytes(M1, K1, 8.0);
-                St.flops_feas += pf * flops(M1, K1);
-                St.bytes_opt += b_opt;
-                St.flops_opt += f_opt;
-            }
-            const int base = (int)all.size();
-            // a phase-1 state warm-starts the children whenever phase 1 ended optimal
-            std::vector<int> warm_ok(n);
-            for (int c = 0; c < n; ++c) {
-                warm_ok[c] = hrec[c].p1_status == ST_OPTIMAL;
-                if (hrec[c].final_set && lvl >= k_min)
-                    fin.insert(fin.end(), cur_sets.begin() + (size_t)c * kw, cur_sets.begin() + (size_t)(c + 1) * kw);
-            }
-            if (keep_rec) {
-                all.resize((size_t)base + n);
-                for (int c = 0; c < n; ++c) {
-                    MpqpRecord &r = all[(size_t)base + c];
-                    std::memcpy(&r, &hrec[c], sizeof r);
-                    r.level = lvl;
-                    r.parent = cur_par[c] >= 0 ? prev_recidx[cur_par[c]] : -1;
-                }
-                allsets.insert(allsets.end(), cur_sets.begin(), cur_sets.end());
+                prev_n = n;
             }
             if (bases) {
                 size_t o1 = allfb.size(), o2 = allb1.size();
@@ -912,18 +1035,28 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                 HIP_OK(hipMemcpyAsync(allb2.data() + o2, db2.p, db2.n, hipMemcpyDeviceToHost, st));
                 HIP_OK(hipStreamSynchronize(st));
             }
-            prev_sets.swap(cur_sets);
-            prev_feas.assign(n, 0);
-            for (int c = 0; c < n; ++c) prev_feas[c] = hrec[c].feasible;
-            prev_warm.swap(warm_ok);
-            prev_recidx.resize(n);
-            for (int c = 0; c < n; ++c) prev_recidx[c] = base + c;
-            prev_n = n;
             fstore_prev = fstore_cur;
             ostore_prev = ostore_cur;
         }
-        if (!restart) break;
-        mixed = false;
+        if (restart) {
+            mixed = false;
+            continue;
+        }
+        if (devenum) {
+            // the records and sets of every level, read once at the end
+            for (size_t L = 0; L < lv.size(); ++L) {
+                const int n = lv[L].n;
+                DevRecord *hr = pinned_records<DevRecord>((size_t)n);
+                std::vector<int> hs((size_t)n * kw), hp((size_t)n);
+                HIP_OK(hipMemcpyAsync(hr, lv[L].rec.p, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(hs.data(), lv[L].sets.p, sizeof(int) * hs.size(), hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(hp.data(), lv[L].par.p, sizeof(int) * hp.size(), hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                level_stats(hr, n, lv[L].nf, mixed);
+                level_outputs(hr, hs.data(), hp.data(), n, (int)L);
+            }
+        }
+        break;
     }
     C.stats.precision_used = mixed ? MPQP_MIXED : MPQP_FP64;
     C.last_feas_nt = feas_nt;
