@@ -616,8 +616,10 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 // runs the fma chain of terms t = g, g + G, ... (lane 0 starts from the
 // constant term, the others from +0.0; masked entries and nonbasic
 // variables are skipped in place), and the team adds its partials by lane
-// exchange, xor 1 then xor 2, so every lane ends with (p0+p1)+(p2+p3).
-// Team sizes: rows of A2 4, columns of A2 4, rows of B^-1 2, columns of
+// exchange, xor 1, xor 2 (and for 8 lanes the half-row mirror, lane g with
+// lane 7 - g), so every lane ends with (p0+p1)+(p2+p3) (and that plus
+// (p4+p5)+(p6+p7)).
+// Team sizes: rows of A2 4, columns of A2 8, rows of B^-1 2, columns of
 // B^-1 4 (a team never straddles a wave: G divides 64).  The sparse walks
 // are padded so that their loads need no per-entry guard: the rows of one
 // wave slice share a step count, a multiple of MPQP_CB; columns are padded
@@ -629,7 +631,7 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 #define MPQP_CB 4   // A2 entries in flight per lane
 #define MPQP_CBL 4  // tableau entries in flight per lane
 #define MPQP_GR 4
-#define MPQP_GC 4
+#define MPQP_GC 8
 #define MPQP_GB 2
 #define MPQP_GT 4
 
@@ -645,11 +647,21 @@ __device__ __forceinline__ double lane_xor_d(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// lane g receives lane 7 - g of its group of 8 (DPP row_half_mirror)
+__device__ __forceinline__ double lane_mirror8_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)(b & 0xffffffffLL), (int)(unsigned)(b & 0xffffffffLL),
+                                               0x141, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), 0x141, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <int G>
 __device__ __forceinline__ double team_add(double v) {
-    static_assert(G == 1 || G == 2 || G == 4, "team size");
+    static_assert(G == 1 || G == 2 || G == 4 || G == 8, "team size");
     if constexpr (G >= 2) v = v + lane_xor_d<1>(v);
     if constexpr (G >= 4) v = v + lane_xor_d<2>(v);
+    if constexpr (G >= 8) v = v + lane_mirror8_d(v);
     return v;
 }
 

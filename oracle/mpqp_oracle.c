@@ -550,9 +550,10 @@ void oracle_cert_stats(long long *out, int reset) {
  * partial is a sequential fma chain (partial 0 starts from the constant
  * term, the others from +0.0; masked entries and nonbasic variables are
  * skipped but keep their place t), and the G partials are added by the
- * butterfly p0+p1 (G = 2) or (p0+p1)+(p2+p3) (G = 4).  The GPU computes
- * the partials on G adjacent lanes and adds them by lane exchange (xor 1,
- * then xor 2), which gives the same sums bit for bit.
+ * butterfly p0+p1 (G = 2), (p0+p1)+(p2+p3) (G = 4) or
+ * ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)) (G = 8).  The GPU computes the
+ * partials on G adjacent lanes and adds them by lane exchange (xor 1, xor 2,
+ * then lane 7 - g), which gives the same sums bit for bit.
  *
  * The entries of the w columns in the stationarity rows [M2-n_c-n_x,
  * M2-n_c) survive the S-mask (initialise_A_kernel_1, batched_simplex.cu
@@ -567,7 +568,7 @@ void oracle_cert_stats(long long *out, int reset) {
  *    them % CERT_GC).
  * The one masked entry left in the lists is (row of w_j, w_j), j in S. */
 #define CERT_GR 4 /* rows of A2: primal residual */
-#define CERT_GC 4 /* columns of A2: dual residual, reduced costs */
+#define CERT_GC 8 /* columns of A2: dual residual, reduced costs */
 #define CERT_GB 2 /* rows of B^-1: primal correction */
 #define CERT_GT 4 /* columns of B^-1: dual correction */
 
@@ -590,7 +591,8 @@ static double res_mag(double r) {
 static double team_total(const double *p, int G) {
     if (G == 1) return p[0];
     if (G == 2) return p[0] + p[1];
-    return (p[0] + p[1]) + (p[2] + p[3]);
+    if (G == 4) return (p[0] + p[1]) + (p[2] + p[3]);
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
 /* cost of variable v in the phase being certified: phase 1 = 1 on the
@@ -632,7 +634,7 @@ static double binvT_col(const Prob *P, const OSt_f *O, const int *where, const d
 
 /* c - sum over column v of A2 of a * y[row], in the term order above */
 static double col_resid(const Prob *P, const char *inS, int v, double c, const double *y) {
-    double p[4] = {c, 0.0, 0.0, 0.0};
+    double p[8] = {c, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int j = w_index(P, v), in = j >= 0 && inS[j], jrow = P->M2 - P->n_c + j, t = 0;
     for (int e = P->csc_ptr[v]; e < P->csc_ptr[v + 1]; e++) {
         int rho = P->csc_row[e];
