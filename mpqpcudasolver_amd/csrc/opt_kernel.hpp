@@ -53,6 +53,7 @@ __host__ __device__ inline long long opt_state_bytes(int M, int K) {
 struct OptShared {
     int r, q, action, stall, nrows, item, best;
     double z;
+    double pinf;     // phase 1: sum of the negative x_B (stall detection)
     long long t0;    // phase profile (builds with MPQP_PROFILE only)
     long long *prof;
 };
@@ -130,22 +131,27 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
     const int M = P.M2, K = P.K2, N2 = P.N2;
     const int tid = threadIdx.x;
     constexpr int V = 16 / sizeof(Real);
+    // the first NT entries of basis / nonbasic are loaded together with the
+    // tableau (one memory round trip)
+    const int b0 = tid < M ? sb[tid] : 0;
+    const int n0 = tid < K ? sn[tid] : 0;
     if (sizeof(Src) == sizeof(Real) && src_ld == L.ld) {
         // opt_tw(M, K) elements: at most 3 beyond T's M rows, which land in
         // the cost row (row M, rebuilt before use)
         const int nv = opt_tw(M, K) / V;
         const int4 *s4 = reinterpret_cast<const int4 *>(sT);
         int4 *d4 = reinterpret_cast<int4 *>(L.T);
-        constexpr int UL = 8;  // loads in flight per thread
-        int e0 = tid;
-        for (; e0 + (UL - 1) * NT < nv; e0 += NT * UL) {
+        // loads in flight per thread; indexes past the end are clamped to
+        // the last vector (the same value copied again), so that the copy
+        // has no branch
+        constexpr int UL = 12;
+        for (int e0 = tid; e0 < nv; e0 += NT * UL) {
             int4 vv[UL];
 #pragma unroll
-            for (int u = 0; u < UL; ++u) vv[u] = s4[e0 + u * NT];
+            for (int u = 0; u < UL; ++u) vv[u] = s4[min(e0 + u * NT, nv - 1)];
 #pragma unroll
-            for (int u = 0; u < UL; ++u) d4[e0 + u * NT] = vv[u];
+            for (int u = 0; u < UL; ++u) d4[min(e0 + u * NT, nv - 1)] = vv[u];
         }
-        for (int e = e0; e < nv; e += NT) d4[e] = s4[e];
     } else {
         const int sw = K + 2, ne = M * sw;
         for (int e = tid; e < ne; e += NT) {
@@ -153,12 +159,19 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
             L.T[i * L.ld + j] = (Real)sT[i * src_ld + j];
         }
     }
-    for (int i = tid; i < M; i += NT) L.basis[i] = sb[i];
-    for (int kk = tid; kk < K; kk += NT) L.nonb[kk] = sn[kk];
-    for (int v = tid; v < N2; v += NT) L.where[v] = INT32_MIN;
-    __syncthreads();
-    for (int i = tid; i < M; i += NT) L.where[L.basis[i]] = i;
-    for (int kk = tid; kk < K; kk += NT) L.where[L.nonb[kk]] = -(kk + 1);
+    // basis and nonbasic partition the N2 variables: every where[] entry is
+    // written here
+    (void)N2;
+    for (int i = tid; i < M; i += NT) {
+        const int b = i < NT ? b0 : sb[i];
+        L.basis[i] = b;
+        L.where[b] = i;
+    }
+    for (int kk = tid; kk < K; kk += NT) {
+        const int v = kk < NT ? n0 : sn[kk];
+        L.nonb[kk] = v;
+        L.where[v] = -(kk + 1);
+    }
     __syncthreads();
 }
 
@@ -171,24 +184,36 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
     const int M = P.M2, K = P.K2, N2 = P.N2, ld = L.ld;
     const int r4 = M - P.n_c - P.n_x;
     const int tid = threadIdx.x;
+    const int nt = P.n_x + 1;
+    int *wl = L.rows;  // where[] of the e-variables of the nonzeros (free at the start)
     for (int jj = j0; jj < j1; ++jj) {
         const int j = Sset[jj];
-        if (tid <= P.n_x) {
+        if (tid < nt) {
             const int rho = tid == 0 ? j : r4 + tid - 1;
             L.colq[tid] = (Real)lam_entry(P, j, rho);
+            wl[tid] = L.where[N2 - M + rho];
         }
         __syncthreads();
         for (int i = tid; i < M; i += NT) {
+            // terms in ascending t, zeros skipped; loads of 4 terms in flight
             Real acc = (Real)0;
-            for (int t = 0; t <= P.n_x; ++t) {
-                const Real av = L.colq[t];
-                if (av == (Real)0) continue;
-                const int rho = t == 0 ? j : r4 + t - 1;
-                const int w = L.where[N2 - M + rho];
-                if (w >= 0) {
-                    if (i == w) acc = fma(av, (Real)1, acc);
-                } else {
-                    acc = fma(av, L.T[i * ld + (-w - 1)], acc);
+            for (int t0 = 0; t0 < nt; t0 += 4) {
+                Real av[4], tv[4];
+                int w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u < nt ? t0 + u : nt - 1;
+                    av[u] = t0 + u < nt ? L.colq[t] : (Real)0;
+                    w[u] = wl[t];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) tv[u] = L.T[i * ld + (w[u] >= 0 ? 0 : -w[u] - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const Real f1 = fma(av[u], (Real)1, acc);
+                    const Real f2 = fma(av[u], tv[u], acc);
+                    const Real na = w[u] >= 0 ? (i == w[u] ? f1 : acc) : f2;
+                    acc = av[u] == (Real)0 ? acc : na;
                 }
             }
             L.vcol[i] = acc;
@@ -270,16 +295,31 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             double v[S];
             int id[S], cn[S];
             bool neg = false;
+            // every load is issued unconditionally (index clamped into the
+            // tableau) and masked afterwards: one wait instead of a branch
+            // and a wait per slot
+            Real x0[S], xp[S];
+            int bv[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int i = lane + MPQP_WAVE * s;
+                const int ic = i < M ? i : 0;
+                x0[s] = T[ic * ld + RHS];
+                xp[s] = T[ic * ld + PERT];
+                bv[s] = L.basis[ic];
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) cn[s] = L.bcnt[bv[s]];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
                 const bool valid = i < M;
-                Real x = valid ? T[i * ld + RHS] : (Real)0;
-                if (pert && valid) x = x + T[i * ld + PERT];
+                Real x = (pert && valid) ? x0[s] + xp[s] : x0[s];
+                x = valid ? x : (Real)0;
                 xv[s] = x;
                 v[s] = (double)x;
                 id[s] = i;
-                cn[s] = valid ? L.bcnt[L.basis[i]] : 0;
+                cn[s] = valid ? cn[s] : 0;
                 neg |= valid && (double)x < tl.dual_stop_tol;
             }
             // Leaving row.  In the plain mode (no counter preference) the
@@ -317,89 +357,108 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
                 r = rfl(id[0]);
             }
             OPT_PROF_DEC(sh, 8);
-            if (lane == 0) L.bcnt[L.basis[r]] += 1;
+            const int bvr = L.basis[r];
             const bool go_on = __any(neg);
-            int action = ACT_PIVOT, q = -1, stall = 0;
+            int action = ACT_PIVOT, q = -1;
             if (!go_on) {
                 action = ACT_STOP;
                 if (pert) {
                     bool inf = false;
                     for (int i = lane; i < M; i += MPQP_WAVE)
                         inf |= (double)T[i * ld + RHS] < tl.dual_stop_tol;
-                    if (__any(inf)) {
-                        no_pert = 1;
-                        n_same = 0;
-                        best_p = -INFINITY;
-                        action = ACT_UNPERT;
-                    }
+                    if (__any(inf)) action = ACT_UNPERT;
                 }
             } else {
                 const int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
                 const Real *tr = T + r * ld;
                 const Real *tc = T + M * ld;
+                Real al[S], cl[S];
+                int nc[S];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int kk = lane + MPQP_WAVE * s;
+                    const int kc = kk < K ? kk : 0;
+                    al[s] = tr[kc];
+                    cl[s] = tc[kc];
+                    nc[s] = L.nbcnt[L.nonb[kc]];
+                }
                 Real dlim = (Real)tl.tol_minus;
                 if (tl.pivot_rel > 0.0) {
-                    double am = 0.0;
-                    for (int kk = lane; kk < K; kk += MPQP_WAVE) {
-                        const double a = fabs((double)tr[kk]);
+                    // max |row entry| in Real: the same value as the maximum
+                    // of the entries widened to double
+                    Real am = (Real)0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const Real a = lane + MPQP_WAVE * s < K ? fabs(al[s]) : (Real)0;
                         am = a > am ? a : am;
                     }
-                    const Real amax = (Real)wave_max(am);
+                    const Real amax = (Real)wave_max((double)am);
                     const Real rl = -(Real)tl.pivot_rel * amax;
                     dlim = rl < dlim ? rl : dlim;
                 }
                 OPT_PROF_DEC(sh, 9);
+                // id carries the slot (the predicate only tests its sign)
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int kk = lane + MPQP_WAVE * s;
-                    id[s] = -1;
-                    v[s] = 100000.0;
-                    cn[s] = 0;
-                    if (kk < K) {
-                        const Real al = tr[kk];
-                        if (al < dlim) {
-                            const int var = L.nonb[kk];
-                            id[s] = var;
-                            v[s] = (double)(-tc[kk] / al);
-                            cn[s] = L.nbcnt[var];
-                        }
-                    }
+                    const bool take = kk < K && al[s] < dlim;
+                    const double ratio = (double)(-cl[s] / al[s]);
+                    id[s] = take ? kk : -1;
+                    v[s] = take ? ratio : 100000.0;
+                    cn[s] = take ? nc[s] : 0;
                 }
                 tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
-                const int wv = rfl(id[0]);
+                const int qk = rfl(id[0]);
                 OPT_PROF_DEC(sh, 10);
-                if (wv < 0) {
+                if (qk < 0) {
                     action = ACT_UNBOUNDED;
                 } else {
-                    q = -L.where[wv] - 1;
-                    Real ps[S];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const int i = lane + MPQP_WAVE * s;
-                        ps[s] = (i < M && xv[s] < (Real)0) ? xv[s] : (Real)0;
-                    }
-                    const double pinf = (double)tree_sum_s<S, Real>(ps, M, lane);
-                    OPT_PROF_DEC(sh, 11);
-                    if (pinf > best_p) {
-                        best_p = pinf;
-                        n_same = 0;
-                    } else {
-                        ++n_same;
-                    }
-                    stall = n_same > O.stall_exit ? 1 : 0;
-                    if (lane == 0) L.nbcnt[wv] += 1;
+                    q = qk;
+                    if (lane == 0) L.nbcnt[L.nonb[q]] += 1;
                 }
             }
             if (lane == 0) {
+                L.bcnt[bvr] += 1;
                 sh.r = r;
                 sh.q = q;
                 sh.action = action;
-                sh.stall = stall;
             }
+        } else if (wave == 1) {
+            // meanwhile: the sum of the negative x_B (the same values as the
+            // decision's), used by the stall counter after a pivot
+            const bool pert = !no_pert && n_same >= O.stall_perturb;
+            Real ps[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int i = lane + MPQP_WAVE * s;
+                const int ic = i < M ? i : 0;
+                const Real x0 = T[ic * ld + RHS], xp = T[ic * ld + PERT];
+                const Real x = (pert && i < M) ? x0 + xp : x0;
+                ps[s] = (i < M && x < (Real)0) ? x : (Real)0;
+            }
+            tree_sum_lane0<S, Real>(ps, M, lane);
+            if (lane == 0) sh.pinf = (double)ps[0];
         }
         __syncthreads();
         OPT_PROF(sh, 1);
-        const int action = sh.action, r = sh.r, q = sh.q, stall = sh.stall;
+        const int action = sh.action, r = sh.r, q = sh.q;
+        // stall bookkeeping, kept by every thread: a pivot without progress
+        // in the infeasibility sum counts towards the perturbation / exit
+        int stall = 0;
+        if (action == ACT_PIVOT) {
+            const double pinf = sh.pinf;
+            if (pinf > best_p) {
+                best_p = pinf;
+                n_same = 0;
+            } else {
+                ++n_same;
+            }
+            stall = n_same > O.stall_exit ? 1 : 0;
+        } else if (action == ACT_UNPERT) {
+            no_pert = 1;
+            n_same = 0;
+            best_p = -INFINITY;
+        }
         // every wave has read the decision before wave 0 writes the next one
         // (the pivot below has barriers of its own)
         if (action != ACT_PIVOT) __syncthreads();
@@ -1001,6 +1060,27 @@ __device__ __forceinline__ double cert_z(const OptLds<float> &L, const DevProb &
     return z;
 }
 
+// FP64 tree sum of xb2[rho] * y[rho] (the dual objective of the refined y in
+// L.cx) in the reference order (wave 0), broadcast through red[0]
+template <int NT, int S>
+__device__ __forceinline__ double cert_zdual(const OptLds<float> &L, const DevProb &P, double *red) {
+    const int M = P.M2, lane = threadIdx.x & (MPQP_WAVE - 1);
+    if (threadIdx.x < MPQP_WAVE) {
+        double zs[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = lane + MPQP_WAVE * s;
+            zs[s] = i < M ? P.xb2[i] * L.cx[i] : 0.0;
+        }
+        const double z = tree_sum_s<S, double>(zs, M, lane);
+        if (lane == 0) red[0] = z;
+    }
+    __syncthreads();
+    const double z = red[0];
+    __syncthreads();
+    return z;
+}
+
 // minimum of x over the workgroup
 template <int NT>
 __device__ __forceinline__ double cert_xmin(const OptLds<float> &L, int M, double *red) {
@@ -1111,24 +1191,42 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
         if (!fail) {
             p1 = opt_phase1<Real, NT, S>(L, P, O, tl, sh);
             if constexpr (MIXED) {
-                // FP64 certification of phase 1: x_B >= dual_stop_tol; if an
-                // artificial is basic, nonnegative reduced phase-1 costs
+                // FP64 certification of phase 1.  With an artificial basic:
+                // the dual first (B^T y = c_B refined, nonnegative reduced
+                // phase-1 costs); a dual objective xb2^T y >= dual_feas_tol
+                // bounds the phase-1 optimum from below and certifies the
+                // infeasible verdict alone.  Otherwise (no artificial basic,
+                // or a smaller bound) also the primal: x_B >= dual_stop_tol,
+                // z1 = the artificials' sum of the refined x.
                 const int Nn = P.Nn;
                 auto cb1 = [Nn](int v) { return v >= Nn ? 1.0 : 0.0; };
-                int ok = p1.status == ST_OPTIMAL, ne = 0;
-                if (ok) ok = cert_primal<NT, S>(L, P, inS, red, sh, Sset, k, &ne);
-                if (ok) ok = cert_xmin<NT>(L, M, red) >= O.dual_stop_tol;
+                int ok = p1.status == ST_OPTIMAL, ne = 0, done = 0;
                 if (ok) {
-                    p1.z1 = cert_z<NT, S>(L, P, red, cb1);
-                    p1.kkt = (p1.z1 > -O.dual_feas_tol && p1.z1 < O.dual_feas_tol) ? 1 : 0;
                     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
-                    OPT_PROF(sh, 10);
                     if (sh.nrows > 0) {
+                        ne = elist_build<S>(L, P, sh);
                         ok = cert_dual<NT>(L, P, inS, red, cb1, sh, ne);
                         if (ok)
                             ok = min_red_cost<NT>(L, P, inS, red, [](int) { return true; }, cb1) >= -O.certify_tol;
                         OPT_PROF(sh, 14);
+                        if (ok) {
+                            const double zd = cert_zdual<NT, S>(L, P, red);
+                            if (zd >= O.dual_feas_tol) {
+                                p1.z1 = zd;
+                                p1.kkt = 0;
+                                done = 1;
+                            }
+                        }
                     }
+                }
+                if (ok && !done) {
+                    ok = cert_primal<NT, S>(L, P, inS, red, sh, Sset, k, &ne);
+                    if (ok) ok = cert_xmin<NT>(L, M, red) >= O.dual_stop_tol;
+                    if (ok) {
+                        p1.z1 = cert_z<NT, S>(L, P, red, cb1);
+                        p1.kkt = (p1.z1 > -O.dual_feas_tol && p1.z1 < O.dual_feas_tol) ? 1 : 0;
+                    }
+                    OPT_PROF(sh, 10);
                 }
                 fail = !ok;
             }

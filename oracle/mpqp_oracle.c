@@ -729,21 +729,31 @@ static int certify_phase1(const Prob *P, const MpqpOptions *o, const OSt_f *O, S
     if (out->p1_status != MPQP_ST_OPTIMAL) CFAIL(0);
     build_where(P, O->basis, O->nonb, s->where);
     double *x = s->cx, *r = s->cr, *d = s->cd, *y = s->cy, *cb = s->ccb;
-    if (!cert_primal(P, O, s->where, s->inS, x, r, d)) CFAIL(1);
-    for (int i = 0; i < M; i++)
-        if (x[i] < o->dual_stop_tol) CFAIL(2);
     int any_art = 0;
     for (int i = 0; i < M; i++) {
         cb[i] = O->basis[i] >= P->N_ ? 1.0 : 0.0;
         any_art |= cb[i] != 0.0;
     }
     if (any_art) {
+        /* dual first: a certified dual-feasible y bounds the phase-1
+         * optimum from below by xb2^T y (weak duality); a bound >=
+         * dual_feasible_tol certifies the infeasible verdict alone */
         if (!cert_dual(P, O, s->where, s->inS, 1, y, r, d)) CFAIL(3);
         for (int k = 0; k < P->K2; k++) {
             int v = O->nonb[k];
             if (!(col_resid(P, s->inS, v, cert_cost(P, 1, v), y) >= -o->certify_tol)) CFAIL(4);
         }
+        for (int rho = 0; rho < M; rho++) r[rho] = P->xb2[rho] * y[rho];
+        double zd = tree_sum_d(r, M);
+        if (zd >= o->dual_feasible_tol) {
+            out->z1 = zd;
+            out->kkt_feasible = 0;
+            return 1;
+        }
     }
+    if (!cert_primal(P, O, s->where, s->inS, x, r, d)) CFAIL(1);
+    for (int i = 0; i < M; i++)
+        if (x[i] < o->dual_stop_tol) CFAIL(2);
     for (int i = 0; i < M; i++) r[i] = cb[i] * x[i];
     out->z1 = tree_sum_d(r, M);
     out->kkt_feasible = (out->z1 > -o->dual_feasible_tol && out->z1 < o->dual_feasible_tol);
