@@ -427,10 +427,11 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
 }
 
 // Three-phase form of the same pivot (column copy; division of row r and
-// zeroing of column q; update), four rows in flight per wave.
-template <int NT, typename Real>
+// zeroing of column q, while the last thread runs `between`; update), four
+// rows in flight per wave.
+template <int NT, typename Real, typename F>
 __device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncols, int r, int q,
-                                            Real *colq) {
+                                            Real *colq, F between) {
     const int tid = threadIdx.x;
     for (int i = tid; i < nrows; i += NT) colq[i] = T[i * ld + q];
     __syncthreads();
@@ -441,6 +442,7 @@ __device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncol
     if (tid == 0) tr[q] = Real(1) / p;
     for (int i = tid; i < nrows; i += NT)
         if (i != r) T[i * ld + q] = Real(0);
+    if (tid == NT - 1) between();
     __syncthreads();
     constexpr int NW = NT / MPQP_WAVE;
     constexpr int U = 4;
@@ -474,18 +476,17 @@ __device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncol
 // copies its column-q entries (rows other than r) to colq and divides its
 // pivot-row entries (columns other than q) — nobody writes T[r][q] or
 // column q here; (B) the other rows are updated, column q from 0 with the
-// factor 1/p (colq[r] is never used), then thread 0 writes T[r][q] = 1/p
-// and runs `between` (the basis bookkeeping, which no pivot step reads).  Per-entry arithmetic is
-// identical to pivot_team.
+// factor 1/p (colq[r] is never used), while the first thread of the last
+// wave writes T[r][q] = 1/p (no thread of (B) reads it: column q takes the
+// factor from a register) and runs `between` (the basis bookkeeping, which no
+// pivot step reads).  Per-entry arithmetic is identical to pivot_team.
 template <int NT, typename Real, typename F>
 __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols, int r, int q,
                                            Real *colq, F between) {
     if constexpr (sizeof(Real) == 8) {
         // FP64 (one workgroup per CU): measured faster with the column copy,
         // the division and the update in separate barrier phases
-        pivot_rows3<NT, Real>(T, ld, nrows, ncols, r, q, colq);
-        if (threadIdx.x == 0) between();
-        __syncthreads();
+        pivot_rows3<NT, Real>(T, ld, nrows, ncols, r, q, colq, between);
         return;
     }
     const int tid = threadIdx.x;
@@ -497,6 +498,10 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
         if (j != q) tr[j] = tr[j] / p;
     __syncthreads();
     const Real pinv = Real(1) / p;
+    if (tid == NT - MPQP_WAVE) {
+        tr[q] = pinv;
+        between();
+    }
     constexpr int NW = NT / MPQP_WAVE;
     constexpr int U = 4;
     const int lane = tid & (MPQP_WAVE - 1);
@@ -521,10 +526,6 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
                 }
             }
         }
-    }
-    if (tid == 0) {
-        tr[q] = pinv;
-        between();
     }
     __syncthreads();
 }

@@ -786,6 +786,42 @@ __device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, O
     return ne;
 }
 
+// ord[0, n) := 0 .. n-1 with the indexes whose flag holds first, each group
+// ascending (wave 0 builds it, then the workgroup synchronizes).  The column
+// walks below take their teams in this order so that the long columns share
+// the first round of teams instead of stretching every round.
+// n <= 64 S; the flags are evaluated once, all S of a lane together.
+template <int S, typename F>
+__device__ __forceinline__ void order_flag_first(int *ord, int n, F f) {
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    if (threadIdx.x < MPQP_WAVE) {
+        const unsigned long long lt = (1ull << lane) - 1ull;
+        bool fl[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = lane + MPQP_WAVE * s;
+            fl[s] = f(i < n ? i : 0);
+        }
+        int base = 0;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int i = lane + MPQP_WAVE * s;
+                const bool take = i < n && (fl[s] == (pass == 0));
+                const unsigned long long m = __ballot(take);
+                if (take) ord[base + __popcll(m & lt)] = i;
+                base += __popcll(m);
+            }
+    }
+    __syncthreads();
+}
+
+// a column of more than 4 team steps (both segments counted)
+__device__ __forceinline__ bool long_col(const DevProb &P, int v) {
+    return P.csc_n[v] + P.csc_x[v] > 4 * MPQP_GC;
+}
+
 // One segment of a column walk: steps s < ns of lane g's entries from
 // base (entry base + s * MPQP_GC), row rmask skipped (the masked entry).
 __device__ __forceinline__ double col_walk(const DevProb &P, unsigned base, unsigned ns, int rmask, const double *y,
@@ -960,7 +996,7 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
 // the FP32 cost row (y_rho = c(e_rho) - d(e_rho), d = 0 for a basic e: the
 // e-columns of A2 are the identity); 1 once max|c_B - B^T y| <= 2e-12.
 // ne: the e-list of cert_primal (same basis).
-template <int NT, typename CostF>
+template <int NT, int S, typename CostF>
 __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
                                          CostF cost, OptShared &sh, int ne) {
     const int M = P.M2, ld = L.ld, e0 = P.N2 - M;
@@ -972,12 +1008,13 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
         const double c = cost(e0 + rho);
         y[rho] = w >= 0 ? c : c - (double)L.T[M * ld + (-w - 1)];
     }
-    __syncthreads();
+    int *ord = L.rows;  // positions, long columns first
+    order_flag_first<S>(ord, M, [&](int i) { return long_col(P, L.basis[i]); });
     OPT_PROF(sh, 11);
     for (int it = 0;; ++it) {
         double rn = 0.0;
         for (int q = threadIdx.x; q < M * MPQP_GC; q += NT) {
-            const int i = q / MPQP_GC, g = q % MPQP_GC;
+            const int i = ord[q / MPQP_GC], g = q % MPQP_GC;
             const int bv = L.basis[i];
             double acc = col_resid(P, sb, bv, g, cost(bv), y);
             acc = team_add<MPQP_GC>(acc);
@@ -1022,14 +1059,19 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
 
 // min over the selected nonbasic slots of the FP64 reduced cost c_v - a_v^T y
 // (a NaN reduced cost counts as -inf)
-template <int NT, typename Sel, typename CostF>
-__device__ __forceinline__ double min_red_cost(const OptLds<float> &L, const DevProb &P, const unsigned *inS,
+template <int NT, int S, typename Sel, typename CostF>
+__device__ __forceinline__ double min_red_cost(OptLds<float> &L, const DevProb &P, const unsigned *inS,
                                                double *red, Sel sel, CostF cost) {
     SBits sb;
     sb.load(inS);
+    // nonbasic slots, long columns first (over the e-list, free again;
+    // identity order if K2 exceeds its M2 + 1 entries)
+    int *ord = L.elr;
+    const bool use_ord = P.K2 <= P.M2 + 1 && P.K2 <= MPQP_WAVE * S;
+    if (use_ord) order_flag_first<S>(ord, P.K2, [&](int kk) { return long_col(P, L.nonb[kk]); });
     double mn = INFINITY;
     for (int q = threadIdx.x; q < P.K2 * MPQP_GC; q += NT) {
-        const int kk = q / MPQP_GC, g = q % MPQP_GC;
+        const int kk = use_ord ? ord[q / MPQP_GC] : q / MPQP_GC, g = q % MPQP_GC;
         const int v = L.nonb[kk];
         if (sel(v)) {
             const double d = team_add<MPQP_GC>(col_resid(P, sb, v, g, cost(v), L.cx));
@@ -1205,9 +1247,9 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
                     if (sh.nrows > 0) {
                         ne = elist_build<S>(L, P, sh);
-                        ok = cert_dual<NT>(L, P, inS, red, cb1, sh, ne);
+                        ok = cert_dual<NT, S>(L, P, inS, red, cb1, sh, ne);
                         if (ok)
-                            ok = min_red_cost<NT>(L, P, inS, red, [](int) { return true; }, cb1) >= -O.certify_tol;
+                            ok = min_red_cost<NT, S>(L, P, inS, red, [](int) { return true; }, cb1) >= -O.certify_tol;
                         OPT_PROF(sh, 14);
                         if (ok) {
                             const double zd = cert_zdual<NT, S>(L, P, red);
@@ -1257,10 +1299,10 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
                     if (ok) {
                         p2.z2 = cert_z<NT, S>(L, P, red, cb2);
                         p2.final_set = -p2.z2 > O.final_tol ? 1 : 0;
-                        ok = cert_dual<NT>(L, P, inS, red, cb2, sh, ne);
+                        ok = cert_dual<NT, S>(L, P, inS, red, cb2, sh, ne);
                     }
                     if (ok)
-                        ok = min_red_cost<NT>(L, P, inS, red, [Nn](int v) { return v < Nn; }, cb2) >= -O.certify_tol;
+                        ok = min_red_cost<NT, S>(L, P, inS, red, [Nn](int v) { return v < Nn; }, cb2) >= -O.certify_tol;
                     fail = !ok;
                 }
                 if (!fail && a.b2)
