@@ -98,6 +98,15 @@ __device__ __forceinline__ int half_pow2(int n) {
     return (int)((m + 1u) >> 1);
 }
 
+// The lane index through an empty asm: values derived from it are then
+// recomputed where they are used instead of being hoisted out of the
+// simplex loops and held (or spilled) across every iteration.
+__device__ __forceinline__ int fresh_lane() {
+    int l = (int)(threadIdx.x & (MPQP_WAVE - 1));
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 __device__ __forceinline__ double shfl_down_d(double v, int d) {
     return __shfl_down(v, (unsigned)d, MPQP_WAVE);
 }
@@ -502,27 +511,37 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
         tr[q] = pinv;
         between();
     }
+    // (B) on column pairs: lane c of a wave step holds columns 2c, 2c + 1
+    // as one 8-byte vector (ld even, T 8-byte aligned; a pair that runs past
+    // ncols updates a padding column of the row, which nothing reads)
     constexpr int NW = NT / MPQP_WAVE;
     constexpr int U = 4;
-    const int lane = tid & (MPQP_WAVE - 1);
+    const int lane = fresh_lane();
     const int wave = tid / MPQP_WAVE;
-    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
-        const int j = j0 + lane;
-        if (j < ncols) {
-            const bool jq = j == q;
-            const Real prj = jq ? pinv : tr[j];
+    const int npair = (ncols + 1) >> 1;
+    for (int c0 = 0; c0 < npair; c0 += MPQP_WAVE) {
+        const int c = c0 + lane;
+        if (c < npair) {
+            const int j = 2 * c;
+            const bool q0 = j == q, q1 = j + 1 == q;
+            const float2 pr = *reinterpret_cast<const float2 *>(tr + j);
+            const Real p0 = q0 ? pinv : pr.x, p1 = q1 ? pinv : pr.y;
             for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
-                Real t[U], f[U];
+                float2 t[U];
+                Real f[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
-                    f[u] = i < nrows ? colq[i] : Real(0);
+                    const int i = i0 + u < nrows ? i0 + u : nrows - 1;
+                    t[u] = *reinterpret_cast<const float2 *>(T + i * ld + j);
+                    f[u] = colq[i];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int i = i0 + u;
-                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, jq ? Real(0) : t[u]);
+                    float2 o;
+                    o.x = fma(-f[u], p0, q0 ? Real(0) : t[u].x);
+                    o.y = fma(-f[u], p1, q1 ? Real(0) : t[u].y);
+                    if (i < nrows && i != r) *reinterpret_cast<float2 *>(T + i * ld + j) = o;
                 }
             }
         }

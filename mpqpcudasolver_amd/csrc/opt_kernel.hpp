@@ -16,7 +16,15 @@
 // delta] padded to an odd count, so that a column walk (pivot column,
 // rhs, e-columns) by 64 lanes touches 64 distinct 4-byte banks (FP32) or 32
 // (FP64) instead of a few.
-__host__ __device__ inline int opt_ld(int K) { return (K + 2) | 1; }
+// Row stride of the optimality tableau: odd for FP64 (column walks free of
+// bank conflicts); for FP32 even (the update moves column pairs as 8-byte
+// vectors) and 2 mod 4 (column walks at most two-way conflicted).
+template <typename Real>
+__host__ __device__ inline int opt_ld(int K) {
+    if (sizeof(Real) == 8) return (K + 2) | 1;
+    const int e = (K + 3) & ~1;
+    return (e & 3) ? e : e + 2;
+}
 
 // LDS of one workgroup: T (M+1) x ld (unless the tableau lives in global
 // memory: gt), colq (M+1), vcol (M) in Real; then (8-byte aligned) bcnt,
@@ -25,9 +33,9 @@ __host__ __device__ inline int opt_ld(int K) { return (K + 2) | 1; }
 // (N2 doubles) and colq/vcol the list of nonbasic e-variables.
 template <typename Real>
 __host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, bool gt = false) {
-    const int ld = opt_ld(K);
+    const int ld = opt_ld<Real>(K);
     long long b = (long long)((gt ? 0 : (M + 1) * ld) + (M + 1) + M) * (long long)sizeof(Real);
-    b = ((b + 7) & ~7LL) + (long long)(M + K + 3 * N2 + M + 1) * 4;
+    b = ((b + 7) & ~7LL) + (long long)(M + K + 3 * N2) * 4;
     if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
     return (b + 15) & ~15LL;
 }
@@ -36,17 +44,18 @@ __host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool 
 // (M+1) x ld Real, 256-byte aligned.
 template <typename Real>
 __host__ __device__ inline long long opt_gtab_bytes(int M, int K) {
-    return ((long long)(M + 1) * opt_ld(K) * (long long)sizeof(Real) + 255) & ~255LL;
+    return ((long long)(M + 1) * opt_ld<Real>(K) * (long long)sizeof(Real) + 255) & ~255LL;
 }
 
 // Stored phase-1 state: T (M rows of ld Real, as in LDS, padded to a
 // multiple of 4 elements so that copies move 16-byte vectors) then basis (M
 // ints) then nonbasic (K ints); states are 256-byte aligned.
-__host__ __device__ inline int opt_tw(int M, int K) { return (M * opt_ld(K) + 3) & ~3; }
+template <typename Real>
+__host__ __device__ inline int opt_tw(int M, int K) { return (M * opt_ld<Real>(K) + 3) & ~3; }
 
 template <typename Real>
 __host__ __device__ inline long long opt_state_bytes(int M, int K) {
-    long long b = (long long)opt_tw(M, K) * (long long)sizeof(Real) + (long long)(M + K) * 4;
+    long long b = (long long)opt_tw<Real>(M, K) * (long long)sizeof(Real) + (long long)(M + K) * 4;
     return (b + 255) & ~255LL;
 }
 
@@ -95,7 +104,7 @@ struct OptLds {
     int *elr, *els;   // certification: nonbasic e-list rho / slot (over colq / vcol)
     // gT: the tableau in global memory, or null for the LDS tableau
     __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT) {
-        ld = opt_ld(K);
+        ld = opt_ld<Real>(K);
         T = gT ? gT : reinterpret_cast<Real *>(smem);
         colq = gT ? reinterpret_cast<Real *>(smem) : T + (M + 1) * ld;
         vcol = colq + (M + 1);
@@ -105,12 +114,13 @@ struct OptLds {
         basis = nbcnt + N2;
         nonb = basis + M;
         where = nonb + K;
-        rows = where + N2;
+        // rows (at most M positions) shares vcol's space: no phase needs both
+        rows = reinterpret_cast<int *>(vcol);
         cx = cr = xv = nullptr;
         elr = reinterpret_cast<int *>(colq);
         els = reinterpret_cast<int *>(vcol);
         if (cert) {
-            const size_t off = ((size_t)(reinterpret_cast<unsigned char *>(rows + M + 1) - smem) + 7) & ~(size_t)7;
+            const size_t off = ((size_t)(reinterpret_cast<unsigned char *>(where + N2) - smem) + 7) & ~(size_t)7;
             cx = reinterpret_cast<double *>(smem + off);
             cr = cx + M;
             xv = reinterpret_cast<double *>(bcnt);
@@ -138,7 +148,7 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
     if (sizeof(Src) == sizeof(Real) && src_ld == L.ld) {
         // opt_tw(M, K) elements: at most 3 beyond T's M rows, which land in
         // the cost row (row M, rebuilt before use)
-        const int nv = opt_tw(M, K) / V;
+        const int nv = opt_tw<Real>(M, K) / V;
         const int4 *s4 = reinterpret_cast<const int4 *>(sT);
         int4 *d4 = reinterpret_cast<int4 *>(L.T);
         // loads in flight per thread; indexes past the end are clamped to
@@ -185,7 +195,7 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
     const int r4 = M - P.n_c - P.n_x;
     const int tid = threadIdx.x;
     const int nt = P.n_x + 1;
-    int *wl = L.rows;  // where[] of the e-variables of the nonzeros (free at the start)
+    int *wl = L.bcnt;  // where[] of the e-variables of the nonzeros (free at the start)
     for (int jj = j0; jj < j1; ++jj) {
         const int j = Sset[jj];
         if (tid < nt) {
@@ -290,6 +300,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
     bool stopped = false;
     for (; it < O.max_iter_p1; ++it) {
         if (wave == 0) {
+            const int lane = sizeof(Real) == 4 ? fresh_lane() : (int)(threadIdx.x & (MPQP_WAVE - 1));
             const bool pert = !no_pert && n_same >= O.stall_perturb;
             Real xv[S];
             double v[S];
@@ -426,6 +437,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
         } else if (wave == 1) {
             // meanwhile: the sum of the negative x_B (the same values as the
             // decision's), used by the stall counter after a pivot
+            const int lane = sizeof(Real) == 4 ? fresh_lane() : (int)(threadIdx.x & (MPQP_WAVE - 1));
             const bool pert = !no_pert && n_same >= O.stall_perturb;
             Real ps[S];
 #pragma unroll
@@ -1008,7 +1020,7 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
         const double c = cost(e0 + rho);
         y[rho] = w >= 0 ? c : c - (double)L.T[M * ld + (-w - 1)];
     }
-    int *ord = L.rows;  // positions, long columns first
+    int *ord = L.bcnt;  // positions, long columns first (bcnt is free after phase 1)
     order_flag_first<S>(ord, M, [&](int i) { return long_col(P, L.basis[i]); });
     OPT_PROF(sh, 11);
     for (int it = 0;; ++it) {
@@ -1139,7 +1151,8 @@ __device__ __forceinline__ double cert_xmin(const OptLds<float> &L, int M, doubl
 // level, in the store's element type.
 template <typename Real, typename Dst, int NT>
 __device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, unsigned char *dst) {
-    const int tw = opt_tw(M, K);
+    // the state holds rows of the destination precision's stride
+    const int tw = opt_tw<Dst>(M, K);
     Dst *dT = reinterpret_cast<Dst *>(dst);
     constexpr int V = 16 / sizeof(Real);
     if (sizeof(Dst) == sizeof(Real)) {
@@ -1147,7 +1160,11 @@ __device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, u
         const int4 *s4 = reinterpret_cast<const int4 *>(L.T);
         for (int e = threadIdx.x; e < tw / V; e += NT) d4[e] = s4[e];
     } else {
-        for (int e = threadIdx.x; e < tw; e += NT) dT[e] = (Dst)L.T[e];
+        const int sw = K + 2, ldd = opt_ld<Dst>(K);
+        for (int e = threadIdx.x; e < M * sw; e += NT) {
+            const int i = e / sw, j = e - i * sw;
+            dT[i * ldd + j] = (Dst)L.T[i * L.ld + j];
+        }
     }
     int *db = reinterpret_cast<int *>(dT + (long long)tw);
     for (int i = threadIdx.x; i < M; i += NT) db[i] = L.basis[i];
@@ -1209,7 +1226,7 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
         int warm = 0, started = 0;
         if (!a.force_cold && O.warm && k > 0 && a.p1parent[c] >= 0) {
             const Real *sT = reinterpret_cast<const Real *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride);
-            const int *sb = reinterpret_cast<const int *>(sT + (long long)opt_tw(M, K));
+            const int *sb = reinterpret_cast<const int *>(sT + (long long)opt_tw<Real>(M, K));
             opt_load<Real, NT, Real>(L, P, sT, ld, sb, sb + M);
             started = warm = opt_lambda<Real, NT>(L, P, tl, Sset, k - 1, k);
             __syncthreads();
