@@ -89,8 +89,11 @@
  * for a phase-1 objective bounded below that reduced cost is rounding
  * noise, and the certification checks it in FP64); its final bases are
  * certified in FP64 by iterative refinement (primal x_B, dual y, reduced
- * costs) and the certified z decides the verdicts; an LP that fails
- * certification is solved again in FP64 from the cold start.  Levels 0 and
+ * costs) and the certified z decides the verdicts (phase 1 with an
+ * artificial basic tries the dual alone first: a dual objective xb2^T y >=
+ * dual_feasible_tol of a certified dual-feasible y proves the infeasible
+ * verdict by weak duality); an LP that fails certification is solved again
+ * in FP64 from the cold start.  Levels 0 and
  * 1 decide per problem whether FP32 pivots are used at all (mixed_probe).
  *
  * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
