@@ -349,3 +349,28 @@ def test_mixed_precision_verdicts_equal_fp64(oracle_path, name):
     for f in ("feasible", "kkt_feasible", "final_set"):
         assert np.array_equal(a.records[f], b.records[f]), f
     assert a.active_sets == b.active_sets
+
+
+@pytest.mark.parametrize("name", ["config0", "config1_k2", "small8x24"])
+def test_mixed_dual_bound_equals_fp64_phase1_optimum(oracle_path, name):
+    """A KKT-infeasible verdict of the mixed path rests on weak duality: the
+    certified dual-feasible y of the FP32 phase-1 basis gives the lower
+    bound xb2^T y >= dual_feasible_tol, recorded as z1.  At an optimal basis
+    the bound is the phase-1 optimum, so it equals the FP64 path's z1 up to
+    rounding; a wrong sign, index or cost in the dual objective would not."""
+    prog, k = {
+        "config0": lambda: config_problem(0),
+        "config1_k2": lambda: (config_problem(1)[0], 2),
+        "small8x24": lambda: (synthetic_mpqp(8, 24, 3, 11), 3),
+    }[name]()
+    lp = assemble_lps(prog)
+    o = OracleLib(oracle_path)
+    o.set_options(record=1)
+    a = o.run(lp, k_max=k, records=True).records
+    o.set_options(record=1, precision=1)
+    b = o.run(lp, k_max=k, records=True).records
+    m = (b["kkt_feasible"] == 0) & (b["repivots"] == 0) & (b["p1_status"] == 1)
+    assert m.sum() > 100
+    za, zb = a["z1"][m], b["z1"][m]
+    assert (zb >= 1e-9).all()
+    assert np.max(np.abs(zb - za) / (1.0 + np.abs(za))) < 1e-9
