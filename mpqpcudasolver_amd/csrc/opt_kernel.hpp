@@ -188,9 +188,11 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
 // lambda swaps for S[j0 .. j1): B^-1 lambda_j from the e-columns of the
 // tableau (nonzeros of lambda_j: row j, then the R4 rows), pivot at the
 // position of w_j, or replacement of its nonbasic column.
+// pre: if not NaN, this thread's lambda entry of S[j0] (lam_entry(P, S[j0],
+// row), read ahead by the caller)
 template <typename Real, int NT>
 __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, const DevTol &tl, const int *Sset,
-                                          int j0, int j1) {
+                                          int j0, int j1, double pre = __builtin_nan("")) {
     const int M = P.M2, K = P.K2, N2 = P.N2, ld = L.ld;
     const int r4 = M - P.n_c - P.n_x;
     const int tid = threadIdx.x;
@@ -200,7 +202,7 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
         const int j = Sset[jj];
         if (tid < nt) {
             const int rho = tid == 0 ? j : r4 + tid - 1;
-            L.colq[tid] = (Real)lam_entry(P, j, rho);
+            L.colq[tid] = (Real)(jj == j0 && pre == pre ? pre : lam_entry(P, j, rho));
             wl[tid] = L.where[N2 - M + rho];
         }
         __syncthreads();
@@ -1193,11 +1195,15 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
             GT ? reinterpret_cast<Real *>(a.gtab + (long long)blockIdx.x * a.gtab_stride) : nullptr);
     const int ld = L.ld;
     const DevTol tl = MIXED ? dev_tol32(O) : dev_tol64(O);
+    // persistent mode: the first work item is taken here; each later one is
+    // taken by thread 0 at the start of the previous candidate, so the
+    // atomic's round trip overlaps that candidate's own loads
+    if (a.counter) {
+        if (tid == 0) sh.item = atomicAdd(a.counter, 1);
+    }
     for (int iter = 0;; ++iter) {
         int item;
         if (a.counter) {
-            __syncthreads();
-            if (tid == 0) sh.item = atomicAdd(a.counter, 1);
             __syncthreads();
             item = sh.item;
         } else {
@@ -1214,21 +1220,31 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
         }
 #endif
         if (tid < 8) inS[tid] = 0u;
-        __syncthreads();
-        if (tid == 0)
-            for (int t = 0; t < k; ++t) inS[Sset[t] >> 5] |= 1u << (Sset[t] & 31);
-        if (*P.root_ok == 0) {
+        // the candidate's memory reads are issued together: the S bits (one
+        // thread per member), the parent's state index, the root flag, and
+        // the new constraint's lambda column for a warm start
+        const bool try_warm = !a.force_cold && O.warm && k > 0;
+        const int par = try_warm ? a.p1parent[c] : -1;
+        const int root_ok = *P.root_ok;
+        const int sj = tid < k ? Sset[tid] : 0;
+        const int nt = P.n_x + 1, r4 = M - P.n_c - P.n_x;
+        const int jw = k > 0 ? Sset[k - 1] : 0;
+        const double lam0 = (try_warm && tid < nt) ? lam_entry(P, jw, tid == 0 ? jw : r4 + tid - 1) : 0.0;
+        __syncthreads();  // inS cleared; every thread has read sh.item
+        if (a.counter && tid == 0) sh.item = atomicAdd(a.counter, 1);
+        if (tid < k) atomicOr(&inS[sj >> 5], 1u << (sj & 31));
+        if (root_ok == 0) {
             if (tid == 0) R.p1_status = ST_SINGULAR;
             continue;
         }
         // start: the parent's clean phase-1 state + lambda swap of the new
         // constraint S[k-1]; else the root + lambda swaps of all of S
         int warm = 0, started = 0;
-        if (!a.force_cold && O.warm && k > 0 && a.p1parent[c] >= 0) {
-            const Real *sT = reinterpret_cast<const Real *>(a.ostore_prev + (long long)a.p1parent[c] * a.ostride);
+        if (par >= 0) {
+            const Real *sT = reinterpret_cast<const Real *>(a.ostore_prev + (long long)par * a.ostride);
             const int *sb = reinterpret_cast<const int *>(sT + (long long)opt_tw<Real>(M, K));
             opt_load<Real, NT, Real>(L, P, sT, ld, sb, sb + M);
-            started = warm = opt_lambda<Real, NT>(L, P, tl, Sset, k - 1, k);
+            started = warm = opt_lambda<Real, NT>(L, P, tl, Sset, k - 1, k, lam0);
             __syncthreads();
         }
         if (!started) {
