@@ -169,6 +169,24 @@ __device__ __forceinline__ double wave_max(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+template <int I>
+__device__ __forceinline__ float max_step_f(float v, int lane) {
+    const float o = lane_down_f<I>(v);
+    return ((lane < I) & (o > v)) ? o : v;
+}
+
+// wave_max on float values (one lane shift per step instead of two)
+__device__ __forceinline__ float wave_max_f(float v) {
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    v = max_step_f<32>(v, lane);
+    v = max_step_f<16>(v, lane);
+    v = max_step_f<8>(v, lane);
+    v = max_step_f<4>(v, lane);
+    v = max_step_f<2>(v, lane);
+    v = max_step_f<1>(v, lane);
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // ---------------------------------------------------------------------
 // Reference-tree reductions on one wave.  Element t lives in lane t % 64,
 // slot t / 64.  Result in lane 0, slot 0.

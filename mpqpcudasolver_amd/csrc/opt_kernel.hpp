@@ -311,18 +311,26 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             // every load is issued unconditionally (index clamped into the
             // tableau) and masked afterwards: one wait instead of a branch
             // and a wait per slot
-            Real x0[S], xp[S];
-            int bv[S];
+            // (the ratio test's cost row and nonbasic use counters do not
+            // depend on the leaving row: read here, with the rows')
+            Real x0[S], xp[S], cl[S];
+            int bv[S], nv[S], nc[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
                 const int ic = i < M ? i : 0;
+                const int kc = i < K ? i : 0;
                 x0[s] = T[ic * ld + RHS];
                 xp[s] = T[ic * ld + PERT];
                 bv[s] = L.basis[ic];
+                cl[s] = T[M * ld + kc];
+                nv[s] = L.nonb[kc];
             }
 #pragma unroll
-            for (int s = 0; s < S; ++s) cn[s] = L.bcnt[bv[s]];
+            for (int s = 0; s < S; ++s) {
+                cn[s] = L.bcnt[bv[s]];
+                nc[s] = L.nbcnt[nv[s]];
+            }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
@@ -384,16 +392,11 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             } else {
                 const int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
                 const Real *tr = T + r * ld;
-                const Real *tc = T + M * ld;
-                Real al[S], cl[S];
-                int nc[S];
+                Real al[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int kk = lane + MPQP_WAVE * s;
-                    const int kc = kk < K ? kk : 0;
-                    al[s] = tr[kc];
-                    cl[s] = tc[kc];
-                    nc[s] = L.nbcnt[L.nonb[kc]];
+                    al[s] = tr[kk < K ? kk : 0];
                 }
                 Real dlim = (Real)tl.tol_minus;
                 if (tl.pivot_rel > 0.0) {
@@ -405,7 +408,9 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
                         const Real a = lane + MPQP_WAVE * s < K ? fabs(al[s]) : (Real)0;
                         am = a > am ? a : am;
                     }
-                    const Real amax = (Real)wave_max((double)am);
+                    Real amax;
+                    if constexpr (sizeof(Real) == 4) amax = wave_max_f(am);
+                    else amax = (Real)wave_max((double)am);
                     const Real rl = -(Real)tl.pivot_rel * amax;
                     dlim = rl < dlim ? rl : dlim;
                 }
