@@ -263,6 +263,37 @@ __global__ void feasible_flag(const DevRecord *rec, int n, int *flag) {
     if (c < n) flag[c] = rec[c].feasible ? 1 : 0;
 }
 
+// Record-free end of a call: per-level work sums for the statistics and
+// the final-set flags.  out[0] feasibility pivots; [1] phase-1 / [2] phase-2
+// pivots of the optimality LPs kept in the first precision; [3] / [4] those
+// of the LPs re-solved in FP64; [5] re-solves.  One atomic per block and sum.
+__global__ void level_summary(const DevRecord *rec, int n, unsigned long long *out, int *flag) {
+    __shared__ unsigned long long s[6];
+    if (threadIdx.x < 6) s[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) {
+        const DevRecord &r = rec[c];
+        const int rs = r.repivots ? 1 : 0;
+        atomicAdd(&s[0], (unsigned long long)r.feas_iters);
+        atomicAdd(&s[rs ? 3 : 1], (unsigned long long)r.p1_iters);
+        atomicAdd(&s[rs ? 4 : 2], (unsigned long long)r.p2_iters);
+        if (rs) atomicAdd(&s[5], (unsigned long long)r.repivots);
+        flag[c] = r.final_set ? 1 : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) atomicAdd(&out[threadIdx.x], s[threadIdx.x]);
+}
+
+// rows[t] = sets[idx[t]] (kw ints each) for t < cnt
+__global__ void gather_rows(const int *sets, const int *idx, int cnt, int kw, int *rows) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt * kw) {
+        const int t = e / kw, u = e - t * kw;
+        rows[e] = sets[(long long)idx[t] * kw + u];
+    }
+}
+
 __global__ void list_write(const int *flag, const int *pos, int n, int *list) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < n && flag[c]) list[pos[c]] = c;

@@ -225,7 +225,7 @@ struct PoolBuf {
 };
 enum {
     PB_SETS, PB_PAR, PB_REC, PB_LIST, PB_P1, PB_FBL, PB_FB, PB_B1, PB_B2, PB_CNT, PB_OFF, PB_SCANTMP,
-    PB_FS, PB_OS = PB_FS + 2, PB_N = PB_OS + 2
+    PB_SUM, PB_FS, PB_OS = PB_FS + 2, PB_N = PB_OS + 2
 };
 PoolBuf &pool(int k) {
     static PoolBuf b[PB_N];
@@ -646,6 +646,34 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         St.bytes_opt += b_opt;
         St.flops_opt += f_opt;
     };
+    // level_stats from the device sums of level_summary (same totals: every
+    // term is an integer-valued double)
+    auto level_stats_sums = [&](const unsigned long long *sm, int n, int nf, bool mx) {
+        MpqpStats &St = C.stats;
+        St.levels += 1;
+        St.launches_feas += 1;
+        St.candidates += n;
+        St.opt_lps += nf;
+        const double Kp2 = (double)(N2 - n_eqs - M2);
+        auto bytes = [&](double Mr, double Kc, double s) { return 2.0 * (Mr + 1) * (Kc + 1) * s + (Mr + Kc + 2) * s; };
+        auto flops = [&](double Mr, double Kc) { return 2.0 * (Mr + 1) * (Kc + 1); };
+        const double s1 = mx ? 4.0 : 8.0;
+        const double p1a = (double)sm[1], p2a = (double)sm[2], p1b = (double)sm[3], p2b = (double)sm[4];
+        const double bb = p1b * bytes(M2, K2, 8.0) + p2b * bytes(M2, Kp2, 8.0);
+        const double fb = p1b * flops(M2, K2) + p2b * flops(M2, Kp2);
+        St.bytes_opt += p1a * bytes(M2, K2, s1) + p2a * bytes(M2, Kp2, s1) + bb;
+        St.flops_opt += (p1a + p1b) * flops(M2, K2) + (p2a + p2b) * flops(M2, Kp2);
+        if (mx) {
+            St.bytes_fallback += bb;
+            St.flops_fallback += fb;
+        }
+        St.fallbacks += (long long)sm[5];
+        St.pivots_feas += (long long)sm[0];
+        St.pivots_p1 += (long long)(sm[1] + sm[3]);
+        St.pivots_p2 += (long long)(sm[2] + sm[4]);
+        St.bytes_feas += (double)sm[0] * bytes(M1, K1, 8.0);
+        St.flops_feas += (double)sm[0] * flops(M1, K1);
+    };
     std::vector<int> prev_recidx;
     auto level_outputs = [&](const DevRecord *hr, const int *sets, const int *par, int n, int lvl) {
         const int base = (int)all.size();
@@ -1042,7 +1070,42 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
             mixed = false;
             continue;
         }
-        if (devenum) {
+        if (devenum && !keep_rec) {
+            // no records wanted: per-level sums and the final sets are
+            // reduced and compacted on the device; only they are copied
+            const int nl = (int)lv.size();
+            const BufView dsum = pool(PB_SUM).get(sizeof(unsigned long long) * 8 * (size_t)std::max(nl, 1));
+            hipLaunchKernelGGL(fill_i32, grid(16 * std::max(nl, 1)), dim3(256), 0, st, dsum.as<int>(), 16 * std::max(nl, 1), 0);
+            HIP_OK(hipGetLastError());
+            for (int L = 0; L < nl; ++L) {
+                const int n = lv[L].n;
+                if (n <= 0) continue;
+                const BufView flag = pool(PB_CNT).get(sizeof(int) * (size_t)n);
+                const BufView pos = pool(PB_OFF).get(sizeof(int) * (size_t)n);
+                hipLaunchKernelGGL(level_summary, grid(n), dim3(256), 0, st, lv[L].rec.as<DevRecord>(), n,
+                                   dsum.as<unsigned long long>() + 8 * L, flag.as<int>());
+                HIP_OK(hipGetLastError());
+                if (L < k_min) continue;
+                scan(flag.as<int>(), pos.as<int>(), n);
+                const int cnt = scan_total(flag.as<int>(), pos.as<int>(), n);
+                if (cnt <= 0) continue;
+                const BufView idx = pool(PB_LIST).get(sizeof(int) * (size_t)n);
+                hipLaunchKernelGGL(list_write, grid(n), dim3(256), 0, st, flag.as<int>(), pos.as<int>(), n, idx.as<int>());
+                HIP_OK(hipGetLastError());
+                const BufView rows = pool(PB_P1).get(sizeof(int) * (size_t)cnt * kw);
+                hipLaunchKernelGGL(gather_rows, grid(cnt * kw), dim3(256), 0, st, lv[L].sets.as<int>(), idx.as<int>(), cnt,
+                                   kw, rows.as<int>());
+                HIP_OK(hipGetLastError());
+                const size_t f0 = fin.size();
+                fin.resize(f0 + (size_t)cnt * kw);
+                HIP_OK(hipMemcpyAsync(fin.data() + f0, rows.p, sizeof(int) * (size_t)cnt * kw, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+            }
+            std::vector<unsigned long long> hs(8 * (size_t)std::max(nl, 1));
+            HIP_OK(hipMemcpyAsync(hs.data(), dsum.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            for (int L = 0; L < nl; ++L) level_stats_sums(&hs[8 * (size_t)L], lv[L].n, lv[L].nf, mixed);
+        } else if (devenum) {
             // the records and sets of every level, read once at the end
             for (size_t L = 0; L < lv.size(); ++L) {
                 const int n = lv[L].n;

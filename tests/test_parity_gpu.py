@@ -379,6 +379,29 @@ def test_autotuned_launches_match_fixed(hip):
     assert again == tuned  # cached per problem shape
 
 
+@pytest.mark.parametrize("k_min", [0, 2])
+def test_record_free_call_matches_recorded_call(hip, k_min):
+    """Without records, a persistent call reduces its statistics and compacts
+    its final sets on the device: the returned active sets, counts and every
+    statistic equal those of the same call with records."""
+    _log(f"case: record-free vs recorded call, mixed persistent, n=30 m=62 p=8, |S|<=3, k_min={k_min}")
+    prog, _ = config_problem(1)
+    lp = assemble_lps(prog)
+    hip.set_options(record=1, precision=1, batching=1)
+    a = hip.run(lp, k_max=3, k_min=k_min, records=True)
+    sa = hip.stats()
+    hip.set_options(record=0, precision=1, batching=1)
+    b = hip.run(lp, k_max=3, k_min=k_min)
+    sb = hip.stats()
+    hip.set_options()
+    assert a.active_sets == b.active_sets
+    assert (a.n_feasible_lps, a.n_final_lps) == (b.n_feasible_lps, b.n_final_lps)
+    for key in ("levels", "launches_feas", "candidates", "opt_lps", "pivots_feas", "pivots_p1", "pivots_p2",
+                "fallbacks", "bytes_feas", "bytes_opt", "flops_feas", "flops_opt", "bytes_fallback",
+                "flops_fallback", "precision_used"):
+        assert sa[key] == sb[key], key
+
+
 def test_errors_are_reported(hip):
     _log("case: invalid input")
     prog, k = config_problem(0)
