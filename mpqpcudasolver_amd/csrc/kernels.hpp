@@ -114,11 +114,9 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
 
     for (int w = tid; w < nw; w += NT) inS[w] = 0u;
     __syncthreads();
-    if (tid == 0) {
-        for (int t = 0; t < a.level; ++t) {
-            const int s = a.sets[(long long)c * a.kw + t];
-            inS[s >> 5] |= 1u << (s & 31);
-        }
+    if (tid < a.level) {  // one thread per member of S
+        const int s = a.sets[(long long)c * a.kw + tid];
+        atomicOr(&inS[s >> 5], 1u << (s & 31));
     }
     const int par = a.parent[c];
     if (par >= 0 && O.warm) {

@@ -377,15 +377,33 @@ __device__ __forceinline__ double tree_sum(double (&v)[MPQP_MAXS], int n, int la
 // Lexicographic (value, key) minimum; order-free, result broadcast to all
 // lanes.  key < 0 marks "no candidate".
 __device__ __forceinline__ void lex_min(double &v, int &key, int &aux) {
-    for (int d = 32; d > 0; d >>= 1) {
-        double ov = __shfl_xor(v, d, MPQP_WAVE);
-        int ok = __shfl_xor(key, d, MPQP_WAVE);
-        int oa = __shfl_xor(aux, d, MPQP_WAVE);
-        const bool take = (ok >= 0) & ((key < 0) | (ov < v) | ((ov == v) & (ok < key)));
-        v = take ? ov : v;
-        key = take ? ok : key;
-        aux = take ? oa : aux;
+    // order-free (a total order on (value, key) over valid keys): lane
+    // shifts into lane 0 (DPP / swizzle / permlane, no LDS round trip), then
+    // a broadcast
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+#define MPQP_LEX_STEP(I)                                                                         \
+    {                                                                                            \
+        const double ov = lane_down_d<I>(v);                                                     \
+        const int ok = lane_down_i<I>(key);                                                      \
+        const int oa = lane_down_i<I>(aux);                                                      \
+        const bool take = (lane < I) & (ok >= 0) & ((key < 0) | (ov < v) | ((ov == v) & (ok < key))); \
+        v = take ? ov : v;                                                                       \
+        key = take ? ok : key;                                                                   \
+        aux = take ? oa : aux;                                                                   \
     }
+    MPQP_LEX_STEP(32)
+    MPQP_LEX_STEP(16)
+    MPQP_LEX_STEP(8)
+    MPQP_LEX_STEP(4)
+    MPQP_LEX_STEP(2)
+    MPQP_LEX_STEP(1)
+#undef MPQP_LEX_STEP
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    v = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    key = __builtin_amdgcn_readfirstlane(key);
+    aux = __builtin_amdgcn_readfirstlane(aux);
 }
 
 // primal ratio test on column q of the tableau (wave 0 only): rows with
