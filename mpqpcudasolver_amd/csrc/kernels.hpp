@@ -190,7 +190,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
             break;
         }
         const int r = sh_r, q = sh_q;
-        pivot_rows<NT, double>(T, ld, M + 1, K + 1, r, q, colq, [&] {
+        pivot_rows_narrow<NT>(T, ld, M + 1, K + 1, r, q, colq, [&] {
             const int t = basis[r];
             basis[r] = nonb[q];
             nonb[q] = t;

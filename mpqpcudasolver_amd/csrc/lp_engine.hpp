@@ -516,6 +516,64 @@ __device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncol
     __syncthreads();
 }
 
+// pivot_rows3 for a narrow FP64 tableau (ncols <= 128, ld even, T 16-byte
+// aligned): the update moves column pairs as 16-byte vectors and packs
+// 64 / pairs rows into each wave step, so that few lanes idle when a row is
+// much shorter than a wave.  Per-entry arithmetic is that of pivot_rows3.
+template <int NT, typename F>
+__device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, int ncols, int r, int q,
+                                                  double *colq, F between) {
+    if ((ld & 1) || ncols > 2 * MPQP_WAVE) {
+        pivot_rows3<NT, double>(T, ld, nrows, ncols, r, q, colq, between);
+        return;
+    }
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nrows; i += NT) colq[i] = T[i * ld + q];
+    __syncthreads();
+    const double p = colq[r];
+    double *tr = T + r * ld;
+    for (int j = tid; j < ncols; j += NT)
+        if (j != q) tr[j] = tr[j] / p;
+    if (tid == 0) tr[q] = 1.0 / p;
+    for (int i = tid; i < nrows; i += NT)
+        if (i != r) T[i * ld + q] = 0.0;
+    if (tid == NT - 1) between();
+    __syncthreads();
+    constexpr int NW = NT / MPQP_WAVE;
+    constexpr int U = 2;
+    const int lane = tid & (MPQP_WAVE - 1);
+    const int wave = tid / MPQP_WAVE;
+    const int npair = (ncols + 1) >> 1;
+    const int rpw = MPQP_WAVE / npair;  // rows per wave step
+    const int dr = lane / npair, c = lane - dr * npair;
+    if (dr < rpw) {
+        // a pair that runs past ncols updates the row's padding column
+        const int j = 2 * c;
+        const double2 pr = *reinterpret_cast<const double2 *>(tr + j);
+        const int stride = NW * rpw;
+        for (int i0 = wave * rpw + dr; i0 < nrows; i0 += stride * U) {
+            double2 t[U];
+            double f[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * stride;
+                const int ic = i < nrows ? i : nrows - 1;
+                t[u] = *reinterpret_cast<const double2 *>(T + ic * ld + j);
+                f[u] = colq[ic];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * stride;
+                double2 o;
+                o.x = fma(-f[u], pr.x, t[u].x);
+                o.y = fma(-f[u], pr.y, t[u].y);
+                if (i < nrows && i != r) *reinterpret_cast<double2 *>(T + i * ld + j) = o;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Same pivot, rows updated four at a time per wave so that several LDS loads
 // are in flight, with two barriers: (A) every thread reads p = T[r][q],
 // copies its column-q entries (rows other than r) to colq and divides its
