@@ -1,25 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark: active-set checks/sec of libmpqp_hip.so on one MI355X.
 
-A step is one mpqp() call over the whole synthetic workload: level-wise
-enumeration of every candidate active set with |S| <= k_max, a feasibility LP
-per candidate and an optimality LP (phase 1 + phase 2) per feasible candidate,
-children warm-started from their parent's final states.  One check is one
-candidate that received an LP verdict.
+A step is one pass of the whole synthetic workload through the C ABI (one
+mpqp() call per problem): level-wise enumeration of every candidate active
+set with |S| <= k_max, a feasibility LP per candidate and an optimality LP
+(phase 1 + phase 2) per feasible candidate, children warm-started from their
+parent's final states.  One check is one candidate that received an LP
+verdict.
 
-The headline variant is mixed precision (FP32 pivots, FP64 certification,
-FP64 re-solve of what fails it) with persistent launches.  The plain FP64
-variant (one workgroup per LP, FP64 pivots) runs in the same process on the
-same input; its verdicts (feasible, KKT-feasible, final) are compared with
-the headline's candidate by candidate and its throughput gives the speedup.
+The headline workload is the largest single-GPU configuration: n=60 m=120
+p=12, |S| <= 3 (288,101 candidates), seed 4.  The headline variant is mixed
+precision (FP32 pivots, FP64 certification, FP64 re-solve of what fails it)
+with persistent launches; the plain FP64 variant (one workgroup per LP, FP64
+pivots, one launch per level) runs in the same process on the same input.
+Its verdicts (feasible, KKT-feasible, final) are compared with the
+headline's candidate by candidate, and the largest relative difference of
+the phase-1 / phase-2 objectives is reported.  The other workloads (n=30
+m=62 p=8 |S|<=4; 64 uneven problems |S|<=3; the ill-conditioned n=30 m=62
+p=8) are reported under "configs" at the same step count.
 
-Prints one JSON line.  value is checks/s of the whole job; the problem data
-crosses the C ABI as host arrays (as in the reference) and is uploaded once
-per step (a few hundred KB, inside the timed region).  roofline is the
-dominant kernel's algorithmic LDS traffic rate (the tableau lives in LDS;
-HBM is touched once per LP to load and store a state) against the LDS read
-bandwidth measured on the device; cpu_baseline is the CPU restatement of the
-same algorithm on a bounded sample, timed on this host's cores.
+value is checks/s of the whole job.  The problem data crosses the C ABI as
+host arrays (as in the reference) and is uploaded once per step, inside the
+timed region (a few hundred KB; the rate with inputs resident differs by
+well under 1%).  roofline is the dominant kernel's algorithmic traffic rate:
+pivot counts of the FP64 path (bit-identical to the CPU restatement's) times
+2(M+1)(K+1)s + (M+K+2)s bytes per pivot at the kernel's element size s,
+divided by the kernel's time from HIP events on the library's stream.  Its
+bound is the LDS bandwidth measured on the device when the tableau lives in
+LDS, and HBM (8 TB/s) when it does not fit there and lives in global memory.
+cpu_baseline is the CPU restatement of the same algorithm (FP64, OpenMP over
+the candidates of a level) on a bounded sample of the workload, best of 3.
 """
 from __future__ import annotations
 
@@ -38,6 +48,7 @@ from mpqpcudasolver_amd import assemble_lps, config_problem, load_library  # noq
 from mpqpcudasolver_amd.problem import CONFIGS  # noqa: E402
 
 METRIC = "active-set checks/sec on 1 MI355X: mixed+batched vs FP64 plain, 0 verdict mismatch"
+HBM_PEAK = 8.0e12
 LDS_NOMINAL = 256 * 128 * 2.4e9  # 256 CUs x 128 B/clk x 2.4 GHz
 VERDICTS = ("feasible", "kkt_feasible", "final_set")
 
@@ -50,35 +61,48 @@ def _workload(cfg: int) -> str:
             f"parent-feasible children, warm starts, seed {c['seed']}")
 
 
-def cpu_baseline(lp, k_sample: int) -> dict:
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(lp, k_sample: int, repeats: int = 3) -> dict:
     """The CPU restatement timed on a bounded sample: every level up to
     k_sample of the same problem, FP64, OpenMP over the candidates of a
-    level."""
+    level; best of `repeats`."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleLib  # test infrastructure: the CPU baseline leg
 
     so = os.path.join(ROOT, "oracle", "_build", "libmpqp_oracle.so")
     orc = OracleLib(so)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    orc.set_options(record=1)
-    t0 = time.perf_counter()
-    run = orc.run(lp, k_max=k_sample, records=True)
-    dt = time.perf_counter() - t0
-    n = len(run.records)
-    return {"value": n / dt, "unit": "checks/s", "cores": threads, "kind": "port",
-            "sample": f"levels |S|<={k_sample} of the same problem, FP64: {n} checks in {dt:.2f} s"}
+    orc.set_options(record=0)
+    best, n = None, 0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        orc.run(lp, k_max=k_sample, decode=False)
+        dt = time.perf_counter() - t0
+        n = orc.stats()["candidates"]
+        best = dt if best is None else min(best, dt)
+    return {"value": n / best, "unit": "checks/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"levels |S|<={k_sample} of the same problem, FP64: {n} checks in {best:.2f} s "
+                      f"(best of {repeats})"}
 
 
 def timed(lib, lps, k_max, steps, warmup, **opts):
-    """A step is one mpqp() call per problem of the workload (one problem
-    except for the 64-problem workload), through the C ABI; turning the
-    returned active-set array into Python lists is the caller's and is not
-    timed.  Returns the last step's summed stats and the wall time of the
-    timed steps."""
+    """Warmup steps, then `steps` timed steps; returns the last step's summed
+    stats and the wall time of the timed steps."""
     lib.set_options(**opts)
     for _ in range(warmup):
         lib.run_many(lps, k_max, decode=False)
     t0 = time.perf_counter()
+    st = None
     for _ in range(steps):
         _, st = lib.run_many(lps, k_max, decode=False)
     dt = time.perf_counter() - t0
@@ -104,34 +128,112 @@ def profiled_traffic(kernel: str, cfg: int, variant: str):
     return None
 
 
-def roofline(st: dict, lds_peak: float, fp32: bool) -> dict:
-    """Dominant kernel: the optimality kernel (FP32 pivots in a mixed run,
-    fallback launches excluded) or the feasibility kernel, whichever took
-    longer in the last step.  Times are HIP events on the library's stream."""
+def pivot_bytes(M, K, s):
+    return 2.0 * (M + 1) * (K + 1) * s + (M + K + 2) * s
+
+
+def roofline(st: dict, st64: dict | None, lps, lds_peak: float, fp32: bool) -> dict:
+    """Dominant kernel: the optimality kernel (the FP32 launches of a mixed
+    run, fallback launches excluded) or the feasibility kernel, whichever
+    took longer in the last step.  Algorithmic bytes use the FP64 path's
+    pivot counts (st64, the plain FP64 run of the same workload) when given."""
     ms_opt = st["ms_opt"] - st["ms_fallback"]
+    src = st64 if st64 is not None else st
     if ms_opt >= st["ms_feas"]:
         name = "opt_kernel32 (FP32 pivots + FP64 certification)" if fp32 else "opt_kernel64"
         launches = st["launches_opt"] - st["launches_fallback"]
-        ms, byts, flops = ms_opt, st["bytes_opt"] - st["bytes_fallback"], st["flops_opt"] - st["flops_fallback"]
+        s = 4.0 if fp32 else 8.0
+        # per problem: the FP64 path's phase-1 / phase-2 pivots at this element size
+        byts = flops = 0.0
+        for lp, (p1, p2) in zip(lps, src["per_problem_pivots"]):
+            kp2 = lp.N2 - lp.n_eqs - lp.M2
+            byts += p1 * pivot_bytes(lp.M2, lp.K2, s) + p2 * pivot_bytes(lp.M2, kp2, s)
+            flops += 2.0 * (lp.M2 + 1) * (p1 * (lp.K2 + 1) + p2 * (kp2 + 1))
+        ms = ms_opt
+        in_global = bool(st["tableau_global"] & (2 if fp32 else 1))
     else:
         name = "feas_kernel"
-        launches, ms, byts, flops = st["launches_feas"], st["ms_feas"], st["bytes_feas"], st["flops_feas"]
+        launches, ms = st["launches_feas"], st["ms_feas"]
+        byts = flops = 0.0
+        for lp, pf in zip(lps, src["per_problem_feas_pivots"]):
+            byts += pf * pivot_bytes(lp.M1, lp.K1, 8.0)
+            flops += pf * 2.0 * (lp.M1 + 1) * (lp.K1 + 1)
+        in_global = False
     launches = max(launches, 1)
     avg_s = ms / launches * 1e-3
     achieved = byts / launches / avg_s / 1e9 if avg_s > 0 else 0.0
+    peak = HBM_PEAK if in_global else lds_peak
     return {
-        "bound": "lds",
+        "bound": "hbm" if in_global else "lds",
         "kernel": name,
         "achieved": achieved,
-        "peak": lds_peak / 1e9,
+        "peak": peak / 1e9,
         "unit": "GB/s",
-        "frac": achieved / (lds_peak / 1e9),
+        "frac": achieved / (peak / 1e9),
         "traffic": None,
         "avg_launch_ms": ms / launches,
         "launches": launches,
         "tflops": flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0,
-        "peak_note": "LDS read bandwidth of this device, measured by mpqp_measure_lds_bandwidth",
+        "bytes_per_launch": byts / launches,
+        "pivot_counts": "FP64 path" if st64 is not None else "this variant",
+        "peak_note": ("HBM3E 8 TB/s: the tableau exceeds one workgroup's LDS and lives in global memory"
+                      if in_global else
+                      f"LDS read bandwidth measured on this device by mpqp_measure_lds_bandwidth "
+                      f"(nominal 256 CU x 128 B/clk x 2.4 GHz = {LDS_NOMINAL / 1e12:.1f} TB/s)"),
     }
+
+
+def run_records(lib, lps, k_max, **opts):
+    lib.set_options(record=1, **opts)
+    runs, st = lib.run_many(lps, k_max, records=True)
+    lib.set_options()
+    return runs, st
+
+
+def compare(ra, rb) -> dict:
+    """Verdict mismatches and objective differences of two record runs of
+    the same workload (a: mixed, b: FP64)."""
+    same_sets = all(a.sets.shape == b.sets.shape and bool(np.array_equal(a.sets, b.sets)) for a, b in zip(ra, rb))
+    mism = {f: (sum(int(np.count_nonzero(a.records[f] != b.records[f])) for a, b in zip(ra, rb))
+                if same_sets else -1) for f in VERDICTS}
+    err = {"z1": 0.0, "z2": 0.0}
+    if same_sets:
+        for a, b in zip(ra, rb):
+            x, y = a.records, b.records
+            m1 = (x["p1_status"] == 1) & (y["p1_status"] == 1)
+            m2 = (x["p2_status"] == 1) & (y["p2_status"] == 1)
+            for key, m in (("z1", m1), ("z2", m2)):
+                if m.any():
+                    e = np.abs(x[key][m] - y[key][m]) / np.maximum(1.0, np.abs(y[key][m]))
+                    err[key] = max(err[key], float(e.max()))
+    return {"verdict_mismatches": mism, "same_candidates": same_sets,
+            "same_active_sets": all(a.active_sets == b.active_sets for a, b in zip(ra, rb)),
+            "max_rel_err": err}
+
+
+def per_problem(runs):
+    """FP64-path pivot counts per problem from record runs."""
+    piv = [(int(r.records["p1_iters"].sum()), int(r.records["p2_iters"].sum())) for r in runs]
+    feas = [int(r.records["feas_iters"].sum()) for r in runs]
+    return piv, feas
+
+
+def measure(lib, cfg, steps, warmup, mixed, batching):
+    """Headline variant and FP64-plain comparison on config `cfg`."""
+    prog, k_max = config_problem(cfg)
+    lps = [assemble_lps(p) for p in prog] if isinstance(prog, list) else [assemble_lps(prog)]
+    opts = dict(precision=1 if mixed else 0, batching=1 if batching == "persistent" else 0)
+    st, dt = timed(lib, lps, k_max, steps, warmup, **opts)
+    value = st["candidates"] * steps / dt
+    st64, dt64 = timed(lib, lps, k_max, steps, warmup, precision=0, batching=0)
+    v64 = st64["candidates"] * steps / dt64
+    ra, _ = run_records(lib, lps, k_max, **opts)
+    rb, _ = run_records(lib, lps, k_max, precision=0, batching=0)
+    st64 = dict(st64)
+    st64["per_problem_pivots"], st64["per_problem_feas_pivots"] = per_problem(rb)
+    cmp = compare(ra, rb)
+    return {"lps": lps, "k_max": k_max, "st": st, "dt": dt, "value": value, "st64": st64, "dt64": dt64,
+            "v64": v64, "cmp": cmp, "records_mixed": ra}
 
 
 def main() -> int:
@@ -139,12 +241,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--batching", choices=["plain", "persistent"], default="persistent")
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP64-plain comparison run")
-    ap.add_argument("--cpu-k", type=int, default=3, help="levels of the CPU-baseline sample")
+    ap.add_argument("--cpu-k", type=int, default=2, help="levels of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the other workloads")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 4 times (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -168,65 +270,59 @@ def main() -> int:
     lib = load_library()
     if lib.warm_up() != 0:
         raise RuntimeError(lib.last_error())
-    prog, k_max = config_problem(args.config)
-    lps = [assemble_lps(p) for p in prog] if isinstance(prog, list) else [assemble_lps(prog)]
-    lp = lps[0]
     mixed = args.precision == "mixed"
-    opts = dict(precision=1 if mixed else 0, batching=1 if args.batching == "persistent" else 0)
-
-    st, dt = timed(lib, lps, k_max, args.steps, args.warmup, **opts)
-    checks = st["candidates"]
-    value = checks * args.steps / dt
+    h = measure(lib, args.config, args.steps, args.warmup, mixed, args.batching)
+    st = h["st"]
     lds_peak = lib.measure_lds_bandwidth()
     if not lds_peak > 0:
         lds_peak = LDS_NOMINAL
+    fp32 = mixed and st["precision_used"] == 1
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": h["value"],
         "unit": "checks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": h["dt"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-pivot+f64-certify" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": _workload(args.config), "variant": f"{args.precision}-{args.batching}",
-                   "checks_per_step": checks, "problems": len(lps),
+                   "checks_per_step": st["candidates"], "problems": len(h["lps"]),
                    "precision_used": "mixed" if st["precision_used"] else "fp64",
                    "fallbacks_fp64": st["fallbacks"], "optimality_lps": st["opt_lps"],
                    "threads_per_lp": lib.tuning()},
-        "roofline": roofline(st, lds_peak, mixed and st["precision_used"] == 1),
+        "roofline": roofline(st, h["st64"], h["lps"], lds_peak, fp32),
+        "fp64_plain": {"value": h["v64"], "ms_per_step": h["dt64"] / args.steps * 1e3,
+                       "speedup": h["value"] / h["v64"] if h["v64"] > 0 else None, **h["cmp"]},
         "stats": st,
     }
-    fp32 = mixed and st["precision_used"] == 1
     if out["roofline"]["kernel"].startswith("opt_kernel"):
         kname = "opt_kernel32" if fp32 else "opt_kernel64"
-        out["roofline"]["traffic"] = profiled_traffic(kname, args.config, f"{args.precision}_{args.batching}")
+        variant = f"{args.precision}_{args.batching}"
+        out["roofline"]["traffic"] = profiled_traffic(kname, args.config, variant)
         if out["roofline"]["traffic"] is not None:
             out["roofline"]["traffic_note"] = (f"HBM bytes per launch from profiles/{kname}_config{args.config}_"
-                                               f"{args.precision}_{args.batching}.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    if not args.no_compare and (mixed or args.batching != "plain"):
-        # the FP64 plain variant: throughput and candidate-by-candidate verdicts
-        st64, dt64 = timed(lib, lps, k_max, args.steps, args.warmup, precision=0, batching=0)
-        v64 = st64["candidates"] * args.steps / dt64
-        lib.set_options(record=1, **opts)
-        ra, _ = lib.run_many(lps, k_max, records=True)
-        lib.set_options(record=1, precision=0, batching=0)
-        rb, _ = lib.run_many(lps, k_max, records=True)
-        lib.set_options()
-        same_sets = all(a.sets.shape == b.sets.shape and bool(np.array_equal(a.sets, b.sets))
-                        for a, b in zip(ra, rb))
-        mism = {f: (sum(int(np.count_nonzero(a.records[f] != b.records[f])) for a, b in zip(ra, rb))
-                    if same_sets else -1) for f in VERDICTS}
-        out["fp64_plain"] = {"value": v64, "ms_per_step": dt64 / args.steps * 1e3,
-                             "speedup": value / v64 if v64 > 0 else None,
-                             "verdict_mismatches": mism, "same_candidates": same_sets,
-                             "same_active_sets": all(a.active_sets == b.active_sets for a, b in zip(ra, rb))}
+                                               f"{variant}.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    if not args.no_extras:
+        extras = {}
+        for cfg in (1, 2, 3):
+            if cfg == args.config:
+                continue
+            e = measure(lib, cfg, args.steps, args.warmup, mixed, args.batching)
+            extras[str(cfg)] = {
+                "workload": _workload(cfg), "value": e["value"], "ms_per_step": e["dt"] / args.steps * 1e3,
+                "fp64_plain_value": e["v64"], "speedup": e["value"] / e["v64"] if e["v64"] > 0 else None,
+                "checks_per_step": e["st"]["candidates"], "fallbacks_fp64": e["st"]["fallbacks"],
+                "precision_used": "mixed" if e["st"]["precision_used"] else "fp64", **e["cmp"]}
+            print(f"# config {cfg}: {extras[str(cfg)]['value']:.4g} checks/s, "
+                  f"{extras[str(cfg)]['speedup']:.3f}x FP64 plain", file=sys.stderr, flush=True)
+        out["configs"] = extras
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(lp, min(args.cpu_k, k_max))
+        out["cpu_baseline"] = cpu_baseline(h["lps"][0], min(args.cpu_k, h["k_max"]))
     print(json.dumps(out), flush=True)
     return 0
 

@@ -30,7 +30,8 @@ extern "C" {
  *   active_sets: n_final_lps rows of n_x int32, each row the ascending
  *   constraint indices of one final active set, padded with -1
  *   (decoded by utils.py:109-115).  Owned by the library; valid until the
- *   next mpqp() call on this thread or mpqp_free_results(). */
+ *   next mpqp() call (which frees it unless the caller already did) or
+ *   mpqp_free_results(). */
 typedef struct GPUResults {
     int n_batches;       /* kernel batches launched (levels x chunks) */
     int n_splits;        /* always 1: level-wise frontiers replace splits */
@@ -50,7 +51,8 @@ int warm_up(void);
 /* Replaces mpqp() of mpqp_cuda.so (argtypes main.py:28-49, call main.py:144).
  *   AxT          n_x x n_c, A[:n_c,:n_x]^T column-major (main.py:100,130).
  *   n_c,n_x,n_t  constraint / variable / parameter counts.
- *   k_min,k_max  active-set cardinalities reported / enumerated.
+ *   k_min,k_max  active-set cardinalities reported / enumerated; k_max is
+ *                clamped to min(n_c, n_x) (a larger set is linearly dependent).
  *   extra_batches  accepted, unused (the reference passes 0, main.py:141).
  *   A1,xb1,b1,M1,N1  feasibility LP  [A -F -rowsum I; 0 A_t -rowsum I],
  *                    rhs [b; b_t], slack basis  (main.py:91-98,121).
@@ -139,8 +141,9 @@ void mpqp_default_options(MpqpOptions *opt);
 int mpqp_set_options(const MpqpOptions *opt);   /* 0 ok, <0 invalid */
 int mpqp_get_options(MpqpOptions *opt);
 
-/* Frees a result returned by mpqp() (the reference has no free symbol;
- * Python hosts that never call it leak nothing beyond the last result). */
+/* Frees a result returned by mpqp() (the reference has no free symbol).  A
+ * result not freed by the caller is freed by the next mpqp() call, so a host
+ * that never calls this holds at most one result. */
 void mpqp_free_results(GPUResults *res);
 
 /* Records of the last mpqp() call (requires MpqpOptions.record >= 1).
@@ -171,7 +174,8 @@ typedef struct MpqpStats {
     long long fallbacks;      /* mixed precision: optimality LPs re-solved in FP64 */
     /* the FP64 re-solve launches of a mixed run, included in the _opt totals */
     int launches_fallback;
-    int reserved1;
+    int tableau_global;       /* optimality tableau in global memory (it exceeds one
+                                 workgroup's LDS): bit 0 FP64 kernel, bit 1 FP32 kernel */
     double ms_fallback, bytes_fallback, flops_fallback;
 } MpqpStats;
 

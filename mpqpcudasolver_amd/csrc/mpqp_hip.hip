@@ -94,7 +94,16 @@ struct Ctx {
     // threads per workgroup; the sizes used by the last call
     std::map<std::vector<int>, std::pair<int, int>> tuned;
     int last_feas_nt = 0, last_opt_nt = 0;
+    // the result of the last mpqp() call, freed by the next call unless the
+    // caller freed it (the reference's host never frees its results)
+    GPUResults *last_res = nullptr;
 };
+
+void free_result(GPUResults *res) {
+    if (!res) return;
+    std::free(res->active_sets);
+    std::free(res);
+}
 
 Ctx &ctx() {
     static Ctx c;
@@ -299,7 +308,7 @@ std::string validate(int n_c, int n_x, int n_t, const int *b1, int M1, int N1, c
     if (n_c <= 0 || n_x <= 0 || n_t <= 0 || n_cp < 0 || K1 != n_x + n_t + 1 ||
         M2 != 2 * n_c + 1 + n_cp + n_x || K2 != n_x + n_t + n_c + 2 || n_eqs != n_c + n_x)
         return "mpqp: inconsistent LP dimensions";
-    if (k_max < 0 || k_max > n_c || k_min < 0) return "mpqp: bad k_min/k_max";
+    if (k_max < 0 || k_min < 0) return "mpqp: bad k_min/k_max";
     for (int i = 0; i < M1; i++)
         if (b1[i] != K1 + i) return "mpqp: b1 must be the slack basis N1-M1..N1-1";
     for (int i = 0; i < M1; i++)
@@ -383,6 +392,9 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
                        std::to_string(C.lds_limit) + " bytes)"};
     hipStream_t st = C.stream;
     const int K1 = N1 - M1, K2 = N2 - M2;
+    // a set of more than n_x constraints has linearly dependent rows (LICQ
+    // fails), and a row of active_sets holds n_x entries (utils.py:109-115)
+    k_max = std::min(k_max, std::min(n_c, n_x));
     const int kw = std::max(k_max, 1);
     C.stats = MpqpStats{};
 
@@ -513,19 +525,6 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
     const double t_start = now_s();
     hipLaunchKernelGGL(root_kernel<256>, dim3(1), dim3(256), (size_t)(M2 + 1) * 8 + (size_t)M2 * 4, st, ra);
     HIP_OK(hipGetLastError());
-    if (const char *dbg = std::getenv("MPQP_DEBUG_ROOT")) {
-        // diagnostics: root tableau, basis, nonbasic, ok flag
-        std::vector<double> hr((size_t)M2 * (K2 + 2));
-        std::vector<int> hi((size_t)M2 + K2 + 1);
-        HIP_OK(hipMemcpyAsync(hr.data(), ra.root, hr.size() * 8, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(hi.data(), ra.root_basis, hi.size() * 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
-        if (FILE *f = std::fopen(dbg, "wb")) {
-            std::fwrite(hr.data(), 8, hr.size(), f);
-            std::fwrite(hi.data(), 4, hi.size(), f);
-            std::fclose(f);
-        }
-    }
     P.root = ra.root;
     P.root_basis = ra.root_basis;
     P.root_nonb = ra.root_nonb;
@@ -1122,6 +1121,7 @@ GPUResults *run_mpqp(int n_c, int n_x, int n_t, int k_min, int k_max, const doub
         break;
     }
     C.stats.precision_used = mixed ? MPQP_MIXED : MPQP_FP64;
+    C.stats.tableau_global = (gt64 ? 1 : 0) | (gt32 ? 2 : 0);
     C.last_feas_nt = feas_nt;
     C.last_opt_nt = opt_nt;
     HIP_OK(hipStreamSynchronize(st));
@@ -1181,10 +1181,14 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, i
     (void)extra_batches;
     (void)reserved_memory;
     std::lock_guard<std::mutex> lk(g_mu);
-    ctx().err.clear();
+    Ctx &C = ctx();
+    C.err.clear();
+    free_result(C.last_res);
+    C.last_res = nullptr;
     try {
-        return run_mpqp(n_c, n_x, n_t, k_min, k_max, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2,
-                        N2, n_eqs);
+        C.last_res = run_mpqp(n_c, n_x, n_t, k_min, k_max, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2,
+                              N2, n_eqs);
+        return C.last_res;
     } catch (const HipError &e) {
         ctx().err = e.msg;
         return nullptr;
@@ -1196,8 +1200,9 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, i
 
 void mpqp_free_results(GPUResults *res) {
     if (!res) return;
-    std::free(res->active_sets);
-    std::free(res);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (res == ctx().last_res) ctx().last_res = nullptr;
+    free_result(res);
 }
 
 void mpqp_default_options(MpqpOptions *o) { default_opts(o); }

@@ -200,10 +200,10 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
     int *wl = L.bcnt;  // where[] of the e-variables of the nonzeros (free at the start)
     for (int jj = j0; jj < j1; ++jj) {
         const int j = Sset[jj];
-        if (tid < nt) {
-            const int rho = tid == 0 ? j : r4 + tid - 1;
-            L.colq[tid] = (Real)(jj == j0 && pre == pre ? pre : lam_entry(P, j, rho));
-            wl[tid] = L.where[N2 - M + rho];
+        for (int t = tid; t < nt; t += NT) {
+            const int rho = t == 0 ? j : r4 + t - 1;
+            L.colq[t] = (Real)(jj == j0 && t == tid && pre == pre ? pre : lam_entry(P, j, rho));
+            wl[t] = L.where[N2 - M + rho];
         }
         __syncthreads();
         for (int i = tid; i < M; i += NT) {

@@ -130,7 +130,7 @@ class MpqpStats(ctypes.Structure):
         ("flops_opt", ctypes.c_double),
         ("fallbacks", ctypes.c_longlong),
         ("launches_fallback", ctypes.c_int),
-        ("reserved1", ctypes.c_int),
+        ("tableau_global", ctypes.c_int),
         ("ms_fallback", ctypes.c_double),
         ("bytes_fallback", ctypes.c_double),
         ("flops_fallback", ctypes.c_double),
@@ -325,7 +325,8 @@ class MpqpLib:
         )
         if records:
             n = self.lib.mpqp_get_records(None, None, None, None, None, 0)
-            kw = max(k_max, 1)
+            # record rows are k_max wide after the library's clamp to min(n_c, n_x)
+            kw = max(min(k_max, lp.n_c, lp.n_x), 1)
             recs = np.zeros(n, dtype=RECORD_DTYPE)
             sets = np.full((n, kw), -1, dtype=np.intc)
             fb = p1 = p2 = None
@@ -354,7 +355,9 @@ class MpqpLib:
                 tot = dict(st)
             else:
                 for key, val in st.items():
-                    if key != "precision_used":
+                    if key == "tableau_global":
+                        tot[key] |= val
+                    elif key != "precision_used":
                         tot[key] += val
                 tot["precision_used"] = st["precision_used"]
         return runs, tot or {}

@@ -1159,7 +1159,13 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
     (void)AxT; (void)extra_batches; (void)reserved_memory;
     const MpqpOptions *o = opts();
     g_err[0] = 0;
-    if (k_max < 0 || k_max > n_c || k_min < 0) { set_err("mpqp: bad k_min/k_max"); return NULL; }
+    if (k_max < 0 || k_min < 0) { set_err("mpqp: bad k_min/k_max"); return NULL; }
+    /* the previous result, unless the caller freed it */
+    if (g_last) { GPUResults *old = g_last; g_last = NULL; free(old->active_sets); free(old); }
+    /* a set of more than n_x constraints has linearly dependent rows (LICQ
+     * fails), and a row of active_sets holds n_x entries (utils.py:109-115) */
+    if (k_max > n_c) k_max = n_c;
+    if (k_max > n_x) k_max = n_x;
     Prob P;
     if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
         return NULL;
@@ -1420,6 +1426,65 @@ int oracle_eval_set(const int *S, int k, int n_c, int n_x, int n_t, double *A1, 
     scratch_free(&s);
     prob_free(&P);
     return ret;
+}
+
+/* oracle_eval_set over n candidates (rows of `sets`, kw ints, -1 padded),
+ * the problem set up once and the candidates' prefix chains evaluated in
+ * parallel.  ok[c] = 1 if candidate c was evaluated, 0 if a proper prefix
+ * is infeasible; rec / fb / ob1 / ob2 (rows of 1 / M1 / M2 / M2) may be
+ * NULL.  Returns 0, or -1 on invalid input. */
+int oracle_eval_sets(const int *sets, int kw, int n, int n_c, int n_x, int n_t, double *A1,
+                     double *xb1, int *b1, int M1, int N1, double *A2, double *xb2, int *b2,
+                     int *non_b2, int M2, int N2, int n_eqs, MpqpRecord *rec, int *fb, int *ob1,
+                     int *ob2, int *ok) {
+    const MpqpOptions *o = opts();
+    Prob P;
+    if (prob_init(&P, n_c, n_x, n_t, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs, o))
+        return -1;
+    MpqpOptions o_run = *o;
+    if (o->precision == MPQP_MIXED && P.root_ok && !mixed_probe(&P, o)) o_run.precision = MPQP_FP64;
+    o = &o_run;
+    const int mixed = o->precision == MPQP_MIXED;
+    #pragma omp parallel
+    {
+        Scratch s;
+        scratch_init(&s, &P);
+        FState Fa, Fb;
+        OPack Wa, Wb, W2;
+        fstate_alloc(&Fa, &P); fstate_alloc(&Fb, &P);
+        opack_alloc(&Wa, &P, 3); opack_alloc(&Wb, &P, 3); opack_alloc(&W2, &P, 3);
+        #pragma omp for schedule(dynamic, 1)
+        for (int c = 0; c < n; c++) {
+            const int *S = sets + (size_t)c * kw;
+            int k = 0;
+            while (k < kw && S[k] >= 0) k++;
+            FState *fpar = NULL, *fcur = &Fa, *fspare = &Fb;
+            OPack *opar = NULL, *ocur = &Wa, *ospare = &Wb;
+            int opar_ok = 0, good = 1;
+            MpqpRecord r;
+            for (int t = 0; t <= k; t++) {
+                BasisOut bo = {0, NULL, NULL, NULL};
+                if (t == k) {
+                    bo.fb = fb ? fb + (size_t)c * M1 : NULL;
+                    bo.b1 = ob1 ? ob1 + (size_t)c * M2 : NULL;
+                    bo.b2 = ob2 ? ob2 + (size_t)c * M2 : NULL;
+                }
+                eval_candidate(&P, o, S, t, fpar, opar ? opack_ptr(opar, mixed) : NULL, opar_ok, fcur,
+                               &ocur->d, &W2.d, &ocur->f, &W2.f, &s, &r, &bo);
+                if (t < k && !r.feasible) { good = 0; break; }
+                fpar = fcur; { FState *tmp = fspare; fspare = fcur; fcur = tmp; }
+                opar = ocur; { OPack *tmp = ospare; ospare = ocur; ocur = tmp; }
+                opar_ok = WARM_OK(r);
+            }
+            ok[c] = good;
+            if (good && rec) { rec[c] = r; rec[c].parent = -1; }
+        }
+        fstate_free(&Fa); fstate_free(&Fb);
+        opack_free(&Wa); opack_free(&Wb); opack_free(&W2);
+        scratch_free(&s);
+    }
+    prob_free(&P);
+    return 0;
 }
 
 /* Exposes the root tableau (S = {} KKT basis) for tests: fills root

@@ -20,6 +20,11 @@ class OracleLib(MpqpLib):
                                       _dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
                                       _dp, _dp, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(MpqpRecord), _ip, _ip, _ip]
+        L.oracle_eval_sets.restype = ctypes.c_int
+        L.oracle_eval_sets.argtypes = [_ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(MpqpRecord), _ip, _ip, _ip, _ip]
         L.oracle_root.restype = ctypes.c_int
         L.oracle_root.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   _dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
@@ -50,6 +55,31 @@ class OracleLib(MpqpLib):
             return None
         arr = np.frombuffer(bytes(rec), dtype=RECORD_DTYPE)[0]
         return arr, fb, b1, b2
+
+    def eval_sets(self, lp, sets, bases=True):
+        """eval_set over many candidates at once (problem set up once, the
+        prefix chains in parallel).  Returns (records, ok, fb, b1, b2): ok[c]
+        is 0 where a proper prefix of candidate c is infeasible."""
+        n = len(sets)
+        kw = max(1, max((len(S) for S in sets), default=1))
+        arr = np.full((n, kw), -1, np.intc)
+        for c, S in enumerate(sets):
+            arr[c, :len(S)] = sorted(S)
+        recs = np.zeros(n, dtype=RECORD_DTYPE)
+        ok = np.zeros(n, np.intc)
+        fb = np.full((n, lp.M1), -1, np.intc) if bases else None
+        b1 = np.full((n, lp.M2), -1, np.intc) if bases else None
+        b2 = np.full((n, lp.M2), -1, np.intc) if bases else None
+        ptr = lambda a: a.ctypes.data_as(_ip) if a is not None else None
+        rc = self.lib.oracle_eval_sets(
+            arr.ctypes.data_as(_ip), kw, n, lp.n_c, lp.n_x, lp.n_t,
+            lp.A1.ctypes.data_as(_dp), lp.xb1.ctypes.data_as(_dp), lp.b1.ctypes.data_as(_ip), lp.M1, lp.N1,
+            lp.A2.ctypes.data_as(_dp), lp.xb2.ctypes.data_as(_dp), lp.b2.ctypes.data_as(_ip),
+            lp.non_b2.ctypes.data_as(_ip), lp.M2, lp.N2, lp.n_eqs,
+            recs.ctypes.data_as(ctypes.POINTER(MpqpRecord)), ptr(fb), ptr(b1), ptr(b2), ok.ctypes.data_as(_ip))
+        if rc < 0:
+            raise RuntimeError(self.last_error())
+        return recs, ok.astype(bool), fb, b1, b2
 
     def root(self, lp):
         root = np.zeros((lp.M2, lp.K2 + 2))

@@ -43,3 +43,32 @@ def compare_runs(a, b, bases=True):
         if getattr(a, f) != getattr(b, f):
             out.append(f"{f}: {getattr(a, f)} vs {getattr(b, f)}")
     return out
+
+
+GOLDEN_DIR = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "golden")
+GOLDEN = ("config0", "small5x16k3", "stress10x24k2")
+
+
+class Golden:
+    """A golden fixture (tests/golden/make_golden.py): the problem, k_max and
+    the expected per-candidate outputs, shaped like an MpqpRun."""
+
+    def __init__(self, name):
+        import os
+
+        from mpqpcudasolver_amd import MPQPProb, assemble_lps
+
+        d = np.load(os.path.join(GOLDEN_DIR, f"{name}.npz"), allow_pickle=False)
+        self.name = name
+        self.prog = MPQPProb(d["prob_A"], d["prob_b"], d["prob_c"], d["prob_H"], d["prob_Q"],
+                             d["prob_A_t"], d["prob_b_t"], d["prob_F"])
+        self.lp = assemble_lps(self.prog)
+        self.k_max = int(d["k_max"])
+        self.records = d["records"]
+        self.sets = d["sets"]
+        self.feas_basis, self.p1_basis, self.p2_basis = d["feas_basis"], d["p1_basis"], d["p2_basis"]
+        self.n_final_lps = int(d["n_final"])
+        self.n_feasible_lps = int(self.records["feasible"].sum())
+        self.highs_feasible, self.highs_kkt, self.highs_eps = d["highs_feasible"], d["highs_kkt"], d["highs_eps"]
+        fin = self.records["final_set"] == 1
+        self.active_sets = [[int(v) for v in row if v >= 0] for row in self.sets[fin]]

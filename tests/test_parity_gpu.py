@@ -1,9 +1,12 @@
 """GPU parity: libmpqp_hip.so against the CPU oracle, through the C ABI.
 
 Verdicts, statuses, iteration counts, candidate sets and final bases must be
-identical; objectives (z_feas, z1, z2) within 1e-9 relative.  Large runs are
-checked on a random sample of candidates (oracle_eval_set walks each
-sample's prefix chain) plus size-independent properties."""
+identical; objectives (z_feas, z1, z2) within 1e-9 relative.  The n=30 m=62
+p=8 problem and its ill-conditioned variant are compared record for record
+over all 597,619 candidates (FP64, and mixed precision for the first), the
+n=60 m=120 p=12 problem over |S| <= 2 and on 2000 sampled candidates of the
+full run; the golden fixtures, numpy optimality checks of the returned
+bases and option-forced status paths complete the picture."""
 import ctypes
 import os
 import sys
@@ -12,10 +15,11 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from mpqpcudasolver_amd import (assemble_lps, config_problem, load_library, precision_stress_mpqp,
+from basis_check import check_run
+from mpqpcudasolver_amd import (MPQPProb, assemble_lps, config_problem, load_library, precision_stress_mpqp,
                                 synthetic_mpqp)
 from oracle_lib import OracleLib
-from parity import FLOAT_FIELDS, INT_FIELDS, REL_TOL, compare_runs
+from parity import FLOAT_FIELDS, GOLDEN, INT_FIELDS, REL_TOL, Golden, compare_runs
 from reftree import pred_dmin, pred_dratio, pred_p2, tree_argmin, tree_sum
 
 pytestmark = pytest.mark.gpu
@@ -145,34 +149,16 @@ def test_parity_cold_start(hip, orc):
 
 
 @pytest.mark.parametrize("cfg", [1, 3])
-def test_parity_full_config_sampled(hip, orc, cfg):
-    """Full |S| <= 4 run; 150 random candidates re-evaluated by the oracle
-    along their prefix chains, plus enumeration properties of all."""
-    _log(f"case: config {cfg} full run, sampled parity")
+def test_parity_full_config(hip, orc, cfg):
+    """Full |S| <= 4 run (597,619 candidates), record for record: statuses,
+    iteration counts, verdicts, objectives and all three final bases."""
+    _log(f"case: config {cfg} full run, every record and basis vs the oracle")
     prog, k = config_problem(cfg)
-    lp = assemble_lps(prog)
-    hip.set_options(record=1)
-    g = hip.run(lp, k_max=k, records=True)
-    r = g.records
-    sets = [[int(v) for v in row if v >= 0] for row in g.sets]
-    # enumeration properties
-    assert r["feasible"].sum() == g.n_feasible_lps
-    assert (r["final_set"] == 1).sum() == g.n_final_lps
-    par = r["parent"]
-    assert np.all(r["feasible"][par[par >= 0]] == 1)
-    assert np.all(r["level"] == [len(s) for s in sets])
-    assert np.all((r["kkt_feasible"] == 0) | (r["feasible"] == 1))
-    assert np.all((r["final_set"] == 0) | (r["kkt_feasible"] == 1))
-    rng = np.random.default_rng(cfg)
-    sample = rng.choice(len(sets), 150, replace=False)
-    for c in sample:
-        rec = orc.eval_set(lp, sets[c])[0]
-        for f in INT_FIELDS:
-            if f == "parent":
-                continue
-            assert rec[f] == r[f][c], (sets[c], f, rec[f], r[f][c])
-        for f in FLOAT_FIELDS:
-            assert abs(rec[f] - r[f][c]) <= REL_TOL * max(1.0, abs(rec[f])), (sets[c], f)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), k)
+    hip.set_options()
+    orc.set_options()
+    assert len(g.records) == 597619
+    assert compare_runs(g, o) == []
 
 
 # ------------------------------------------------------- mixed precision
@@ -240,28 +226,19 @@ def test_mixed_verdicts_match_fp64(hip, cfg):
         assert st["fallbacks"] <= 0.05 * st["opt_lps"]
 
 
-def test_mixed_full_config_sampled(hip, orc):
-    """Config 1 full run in mixed precision; 60 random candidates re-evaluated
-    by the oracle's mixed path along their prefix chains."""
-    _log("case: config 1 full mixed run, sampled parity")
+def test_mixed_full_config(hip, orc):
+    """Config 1 full |S| <= 4 run in mixed precision, record for record
+    against the oracle's mixed path (FP32 pivots, FP64 certification, FP64
+    re-solve of what fails it)."""
+    _log("case: config 1 full mixed run, every record and basis vs the oracle")
     prog, k = config_problem(1)
-    lp = assemble_lps(prog)
-    hip.set_options(record=1, precision=1)
-    orc.set_options(precision=1)
-    g = hip.run(lp, k_max=k, records=True)
-    r = g.records
-    sets = [[int(v) for v in row if v >= 0] for row in g.sets]
-    rng = np.random.default_rng(7)
-    for c in rng.choice(len(sets), 60, replace=False):
-        rec = orc.eval_set(lp, sets[c])[0]
-        for f in INT_FIELDS:
-            if f == "parent":
-                continue
-            assert rec[f] == r[f][c], (sets[c], f, rec[f], r[f][c])
-        for f in FLOAT_FIELDS:
-            assert abs(rec[f] - r[f][c]) <= REL_TOL * max(1.0, abs(rec[f])), (sets[c], f)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), k, precision=1)
+    st_g, st_o = hip.stats(), orc.stats()
     hip.set_options()
     orc.set_options()
+    assert compare_runs(g, o) == []
+    assert st_g["precision_used"] == st_o["precision_used"] == 1
+    assert st_g["fallbacks"] == st_o["fallbacks"]
 
 
 # ---------------------------------------- tableau larger than LDS (n=60 m=120 p=12)
@@ -278,25 +255,27 @@ def test_parity_config4_first_levels(hip, orc, precision):
     orc.set_options()
 
 
-def test_config4_full_sampled_and_mixed_verdicts(hip, orc):
-    """Full |S| <= 3 run: 40 sampled candidates against the oracle along their
-    prefix chains, and the mixed path's verdicts equal the FP64 path's."""
-    _log("case: config 4 full run, sampled parity and mixed verdicts")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_parity_config4_two_levels(hip, orc, precision):
+    """|S| <= 2 (7,261 candidates), record for record with bases."""
+    _log(f"case: config 4 |S| <= 2, precision {precision}, every record and basis vs the oracle")
+    prog, _ = config_problem(4)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 2, precision=precision)
+    hip.set_options()
+    orc.set_options()
+    assert len(g.records) == 7261
+    assert compare_runs(g, o) == []
+
+
+def test_config4_full_mixed_verdicts_and_sample(hip, orc):
+    """Full |S| <= 3 run (288,101 candidates): the mixed path's verdicts equal
+    the FP64 path's on every candidate, and 2000 sampled candidates of the
+    FP64 run equal the oracle's records along their prefix chains."""
+    _log("case: config 4 full run, mixed vs FP64 verdicts, 2000 sampled vs the oracle")
     prog, k = config_problem(4)
     lp = assemble_lps(prog)
-    hip.set_options(record=1)
-    f = hip.run(lp, k_max=k, records=True)
-    r = f.records
-    sets = [[int(v) for v in row if v >= 0] for row in f.sets]
-    rng = np.random.default_rng(4)
-    for c in rng.choice(len(sets), 40, replace=False):
-        rec = orc.eval_set(lp, sets[c])[0]
-        for fld in INT_FIELDS:
-            if fld == "parent":
-                continue
-            assert rec[fld] == r[fld][c], (sets[c], fld, rec[fld], r[fld][c])
-        for fld in FLOAT_FIELDS:
-            assert abs(rec[fld] - r[fld][c]) <= REL_TOL * max(1.0, abs(rec[fld])), (sets[c], fld)
+    hip.set_options(record=2)
+    f = hip.run(lp, k_max=k, records=True, bases=True)
     hip.set_options(record=1, precision=1)
     m = hip.run(lp, k_max=k, records=True)
     hip.set_options()
@@ -305,6 +284,20 @@ def test_config4_full_sampled_and_mixed_verdicts(hip, orc):
         d = np.flatnonzero(f.records[fld] != m.records[fld])
         assert len(d) == 0, (fld, len(d), d[:5])
     assert f.active_sets == m.active_sets
+    rng = np.random.default_rng(4)
+    idx = rng.choice(len(f.records), 2000, replace=False)
+    sets = [[int(v) for v in f.sets[c] if v >= 0] for c in idx]
+    orc.set_options(record=2)
+    recs, ok, fb, b1, b2 = orc.eval_sets(lp, sets)
+    assert ok.all()
+    for t, c in enumerate(idx):
+        for fld in INT_FIELDS:
+            if fld != "parent":
+                assert recs[t][fld] == f.records[fld][c], (sets[t], fld)
+        for fld in FLOAT_FIELDS:
+            assert abs(recs[t][fld] - f.records[fld][c]) <= REL_TOL * max(1.0, abs(recs[t][fld])), (sets[t], fld)
+        assert np.array_equal(fb[t], f.feas_basis[c]) and np.array_equal(b1[t], f.p1_basis[c])
+        assert np.array_equal(b2[t], f.p2_basis[c])
 
 
 # --------------------------------------------------- 64 uneven problems
@@ -409,3 +402,112 @@ def test_errors_are_reported(hip):
     lp.b2 = lp.b2[::-1].copy()
     assert not hip.mpqp_raw(lp, 0, k)
     assert "b2" in hip.last_error()
+
+
+# ------------------------------------------------------ golden fixtures
+
+@pytest.mark.parametrize("name", GOLDEN)
+@pytest.mark.parametrize("batching", [0, 1])
+def test_golden_fp64(hip, name, batching):
+    """The HiGHS-checked golden fixtures (tests/golden), record for record
+    with bases, plain and persistent launches."""
+    _log(f"case: golden {name}, FP64, batching {batching}")
+    g = Golden(name)
+    hip.set_options(record=2, batching=batching)
+    run = hip.run(g.lp, k_max=g.k_max, records=True, bases=True)
+    hip.set_options()
+    assert compare_runs(run, g) == []
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_mixed_verdicts(hip, name):
+    """Mixed precision: the fixture's verdicts and final active sets."""
+    _log(f"case: golden {name}, mixed precision verdicts")
+    g = Golden(name)
+    hip.set_options(record=1, precision=1)
+    run = hip.run(g.lp, k_max=g.k_max, records=True)
+    hip.set_options()
+    assert np.array_equal(run.sets, g.sets)
+    for fld in ("feasible", "kkt_feasible", "final_set"):
+        assert np.array_equal(run.records[fld], g.records[fld]), fld
+    assert run.active_sets == g.active_sets
+
+
+# ------------------------------------ optimality conditions of GPU bases
+
+@pytest.mark.parametrize("name", ["config0", "small8x24", "small5x16k4", "stress10x24"])
+def test_gpu_bases_satisfy_optimality_conditions(hip, name):
+    """Every basis the GPU returns re-solved with numpy: nonsingular, primal
+    and dual feasible, reproducing its recorded objective."""
+    _log(f"case: numpy optimality checks of GPU bases, {name}")
+    prog, k = {
+        "config0": lambda: config_problem(0),
+        "small8x24": lambda: (synthetic_mpqp(8, 24, 3, 11), 3),
+        "small5x16k4": lambda: (synthetic_mpqp(5, 16, 2, 13), 4),
+        "stress10x24": lambda: (precision_stress_mpqp(10, 24, 3, 13, n_dup=4), 3),
+    }[name]()
+    lp = assemble_lps(prog)
+    hip.set_options(record=2, batching=1)
+    run = hip.run(lp, k_max=k, records=True, bases=True)
+    hip.set_options()
+    assert check_run(lp, run) == {}
+
+
+@pytest.mark.parametrize("cfg", [1, 4])
+def test_gpu_bases_sampled_optimality_conditions(hip, cfg):
+    _log(f"case: numpy optimality checks of 300 sampled GPU bases, config {cfg}")
+    prog, k = config_problem(cfg)
+    lp = assemble_lps(prog)
+    hip.set_options(record=2, batching=1, precision=1)
+    run = hip.run(lp, k_max=min(k, 3), records=True, bases=True)
+    hip.set_options()
+    rng = np.random.default_rng(50 + cfg)
+    idx = rng.choice(len(run.records), 300, replace=False)
+    bad = check_run(lp, run, idx)
+    assert bad == {}, list(bad.items())[:3]
+
+
+# --------------------------------------------------- forced status paths
+
+FORCED = [
+    ("feas_iter_cap", dict(max_iter_feas=1), "feas_status", 3),
+    ("feas_unbounded", dict(tol_plus=10.0), "feas_status", 2),
+    ("p1_iter_cap", dict(max_iter_phase1=3), "p1_status", 3),
+    ("p1_unbounded", dict(tol_minus=-10.0), "p1_status", 2),
+    ("p1_stall", dict(stall_exit=2), "p1_status", 4),
+    ("p1_singular", dict(licq_tol=0.3), "p1_status", 5),
+    ("p2_iter_cap", dict(max_iter_phase2=1), "p2_status", 3),
+    ("no_perturb_delay", dict(stall_perturb=0), "p1_status", 1),
+]
+
+
+@pytest.mark.parametrize("name,opts,field,status", FORCED, ids=[f[0] for f in FORCED])
+@pytest.mark.parametrize("batching", [0, 1])
+def test_forced_status_paths(hip, orc, name, opts, field, status, batching):
+    """Caps, stall exit, empty ratio tests and singular warm starts forced by
+    the options: every record and basis equal the oracle's, and the forced
+    status occurs."""
+    _log(f"case: forced {name} {opts}, batching {batching}, n=8 m=24 p=3 |S|<=3")
+    lp = assemble_lps(synthetic_mpqp(8, 24, 3, 11))
+    hip.set_options(record=2, batching=batching, **opts)
+    orc.set_options(record=2, **opts)
+    g = hip.run(lp, k_max=3, records=True, bases=True)
+    o = orc.run(lp, k_max=3, records=True, bases=True)
+    hip.set_options()
+    orc.set_options()
+    assert compare_runs(g, o) == []
+    assert (g.records[field] == status).sum() > 0
+
+
+def test_duplicated_rows(hip, orc):
+    """Two identical constraint rows (an exactly dependent pair)."""
+    _log("case: exactly duplicated constraint rows, n=8 m=24 p=3 |S|<=3")
+    prog = synthetic_mpqp(8, 24, 3, 11)
+    A, b, F = prog.A.copy(), prog.b.copy(), prog.F.copy()
+    A[5], b[5], F[5] = A[2], b[2], F[2]
+    dup = MPQPProb(A, b, prog.c, prog.H, prog.Q, prog.A_t, prog.b_t, F)
+    for prec in (0, 1):
+        g, o = _run_pair(hip, orc, assemble_lps(dup), 3, precision=prec)
+        assert compare_runs(g, o) == [], prec
+    hip.set_options()
+    orc.set_options()
