@@ -120,6 +120,14 @@ typedef struct MpqpOptions {
     double tol32;              /* mixed precision: pivot / stop tolerance of the FP32 pass (1e-5) */
     double licq32;             /* mixed precision: FP32 LICQ screen; smaller pivots go to FP64 (1e-6) */
     double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass (1e-3) */
+    double licq_screen;        /* LICQ screen of each candidate before its LPs: S fails when a
+                                  pivot d_t of the Cholesky factorisation of (A Q^-1 A^T)_SS (rows
+                                  of S in ascending order) has d_t <= licq_screen * G_tt, i.e. row
+                                  t is within that squared sine of the span of the rows before it
+                                  in the Q^-1 metric; such an S gets feas_status SINGULAR, no LP
+                                  and no children.  1e-14 (0: off) */
+    long long max_chunk;       /* most candidates of one level processed per launch; 0: derived
+                                  from the free device memory minus reserved_memory */
 } MpqpOptions;
 
 /* One record per candidate active set of the last mpqp() call, in

@@ -30,6 +30,7 @@ struct FeasArgs {
     int store;
     DevRecord *rec;     // n
     int *fbasis;        // n x M1 or null
+    const int *licq_fail;  // n: 1 = S failed the LICQ screen (no LP), or null
 };
 
 struct OptArgs {
@@ -112,6 +113,24 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     const int nw = (P.n_c + 31) / 32;
     const DevTol tl = dev_tol64(O);
 
+    if (a.licq_fail && a.licq_fail[c]) {
+        // rows of S linearly dependent (LICQ screen): no LP, no children
+        if (tid == 0) {
+            DevRecord &R = a.rec[c];
+            R.level = a.level;
+            R.feas_status = ST_SINGULAR;
+            R.feas_iters = R.feasible = 0;
+            R.z_feas = 0.0;
+            R.p1_status = R.p1_iters = R.kkt_feasible = R.p1_warm = 0;
+            R.p2_status = R.p2_iters = R.final_set = 0;
+            R.z1 = R.z2 = 0.0;
+            R.repivots = 0;
+            R.p1_clean = 0;
+        }
+        if (a.fbasis)
+            for (int i = tid; i < M; i += NT) a.fbasis[(long long)c * M + i] = -1;
+        return;
+    }
     for (int w = tid; w < nw; w += NT) inS[w] = 0u;
     __syncthreads();
     if (tid < a.level) {  // one thread per member of S
@@ -254,6 +273,111 @@ __global__ void child_write(const DevRecord *prec, const int *psets, const int *
         par[o + t] = pi;
         p1par[o + t] = warm;
     }
+}
+
+// child_write over parents [p0, p0 + pn) of a chunk: off[t] is parent
+// p0 + t's first child relative to the range; par holds chunk-wide parent
+// indices (their rows in the parent chunk's state stores)
+__global__ void child_write_range(const DevRecord *prec, const int *psets, const int *off, int p0, int pn, int kw,
+                                  int level, int n_c, int *sets, int *par, int *p1par) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= pn) return;
+    const int pi = p0 + t;
+    if (!prec[pi].feasible) return;
+    const int *ps = psets + (long long)pi * kw;
+    const int mx = level >= 2 ? ps[level - 2] : -1;
+    const int warm = prec[pi].p1_status == ST_OPTIMAL ? pi : -1;
+    const int o = off[t];
+    for (int j = mx + 1; j < n_c; ++j) {
+        const int ch = o + (j - mx - 1);
+        int *cs = sets + (long long)ch * kw;
+        for (int u = 0; u < kw; ++u) cs[u] = u < level - 1 ? ps[u] : (u == level - 1 ? j : -1);
+        par[ch] = pi;
+        p1par[ch] = warm;
+    }
+}
+
+// rows[t] = sets[idx[t]] widened to nx entries, -1 padded (t < cnt)
+__global__ void gather_rows_nx(const int *sets, const int *idx, int cnt, int kw, int nx, int *rows) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt * nx) {
+        const int t = e / nx, u = e - t * nx;
+        rows[e] = u < kw ? sets[(long long)idx[t] * kw + u] : -1;
+    }
+}
+
+// ---------------------------------------------------------------------
+// LICQ screen.  G = A Q^-1 A^T (n_c x n_c) is built once per problem:
+// gram_solve turns W = [Q | A^T] (n_x x (n_x + n_c), row-major) into
+// [I | Q^-1 A^T] by Gauss-Jordan with diagonal pivots (Q is symmetric
+// positive definite), gram_mfma multiplies A by Q^-1 A^T on the FP64
+// matrix cores.  licq_screen then factors G_SS (rows of S ascending) per
+// candidate.
+// ---------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void gram_solve(double *W, int n_x, int ld, int ncols, double *colq) {
+    for (int i = 0; i < n_x; ++i) pivot_team<NT, double>(W, ld, n_x, ncols, i, i, colq);
+}
+
+// G (n_c x n_c, row-major) = A (n_c x n_x, row-major) x Y (n_x x n_c, rows
+// of stride ldy): one wave per 16 x 16 tile of G, v_mfma_f64_16x16x4f64
+// over n_x in steps of 4 (operands outside the matrices read as 0).
+// Lane l supplies A[i0 + l % 16][k + l / 16] and Y[k + l / 16][j0 + l % 16]
+// and receives G[i0 + 4 r + l / 16][j0 + l % 16] in result register r = 0..3
+// (the result layout of this instruction, measured on gfx950).
+__global__ __launch_bounds__(64) void gram_mfma(const double *A, const double *Y, int n_c, int n_x, int ldy,
+                                                 double *G) {
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const int tiles = (n_c + 15) / 16;
+    const int ti = blockIdx.x / tiles, tj = blockIdx.x % tiles;
+    const int l = threadIdx.x, lr = l % 16, lk = l / 16;
+    const int i = ti * 16 + lr, j = tj * 16 + lr;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n_x; k0 += 4) {
+        const int k = k0 + lk;
+        const double a = (i < n_c && k < n_x) ? A[(long long)i * n_x + k] : 0.0;
+        const double b = (j < n_c && k < n_x) ? Y[(long long)k * ldy + j] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int gi = ti * 16 + 4 * r + lk, gj = tj * 16 + lr;
+        if (gi < n_c && gj < n_c) G[(long long)gi * n_c + gj] = acc[r];
+    }
+}
+
+// One group of GS lanes per candidate (GS >= level, a power of two <= 64):
+// lane r of the group holds row r of the Cholesky factor of G_SS in LDS and
+// reads the other rows' column entries by lane exchange.  fail[c] = 1 when a
+// pivot d_t <= tol * G_tt (G_tt the original diagonal entry); d_t is the
+// squared distance of row t of A (Q^-1 metric) to the span of the rows
+// before it, relative to its squared length.
+template <int GS>
+__global__ __launch_bounds__(64) void licq_screen(const double *G, int n_c, const int *sets, int n, int kw, int level,
+                                                   double tol, int *fail) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int PER = 64 / GS;
+    double *Lrow = reinterpret_cast<double *>(smem) + (long long)threadIdx.x * level;
+    const int g = threadIdx.x / GS, r = threadIdx.x % GS;
+    const int c = blockIdx.x * PER + g;
+    const bool live = c < n;
+    const int *S = sets + (long long)(live ? c : 0) * kw;
+    const int sr = (live && r < level) ? S[r] : 0;
+    for (int q = 0; q < level; ++q) Lrow[q] = (live && r < level) ? G[(long long)sr * n_c + S[q]] : 0.0;
+    bool bad = false;
+    for (int t = 0; t < level; ++t) {
+        const double gtt = __shfl(Lrow[t], t, GS);  // row t's current pivot entry
+        const double dtt = G[(long long)(live ? S[t] : 0) * n_c + (live ? S[t] : 0)];
+        bad |= !(gtt > tol * dtt);
+        const double sq = sqrt(gtt > 0.0 ? gtt : 1.0);
+        const double lt = (r > t && r < level) ? Lrow[t] / sq : 0.0;
+        if (r > t && r < level) Lrow[t] = lt;
+        for (int q = t + 1; q < level; ++q) {
+            const double lq = __shfl(lt, q, GS);
+            if (r >= q && r < level) Lrow[q] = fma(-lt, lq, Lrow[q]);
+        }
+    }
+    if (live && r == 0) fail[c] = bad ? 1 : 0;
 }
 
 __global__ void feasible_flag(const DevRecord *rec, int n, int *flag) {

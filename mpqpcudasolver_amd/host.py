@@ -86,6 +86,8 @@ class MpqpOptions(ctypes.Structure):
         ("tol32", ctypes.c_double),
         ("licq32", ctypes.c_double),
         ("pivot_rel32", ctypes.c_double),
+        ("licq_screen", ctypes.c_double),
+        ("max_chunk", ctypes.c_longlong),
     ]
 
 

@@ -152,6 +152,8 @@ void mpqp_default_options(MpqpOptions *o) {
     o->tol32 = 1e-5;
     o->licq32 = 1e-6;
     o->pivot_rel32 = 1e-3;
+    o->licq_screen = 1e-14;
+    o->max_chunk = 0; /* no effect on the CPU */
 }
 
 static const MpqpOptions *opts(void) {
@@ -243,11 +245,15 @@ typedef struct {
     double *root;
     int *root_basis, *root_nonb;
     int root_ok;
+    /* G = A Q^-1 A^T (n_c x n_c, row-major) for the LICQ screen, or NULL */
+    double *G;
     /* structural nonzeros of A2 by column (rows ascending) and by row
      * (variables ascending), for the certification sums */
     int *csc_ptr, *csc_row, *csr_ptr, *csr_var;
     double *csc_val, *csr_val;
 } Prob;
+
+static int licq_ok(const Prob *P, const int *S, int k, double tol);
 
 /* var ids of the optimality LP */
 #define W_VAR(P, j) ((P)->n_x + (P)->n_t + (j))
@@ -929,6 +935,11 @@ static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, in
     if (bo && bo->b1) for (int i = 0; i < M2; i++) bo->b1[i] = -1;
     if (bo && bo->b2) for (int i = 0; i < M2; i++) bo->b2[i] = -1;
 
+    /* LICQ screen: linearly dependent rows of S get no LP and no children */
+    if (k >= 1 && o->licq_screen > 0.0 && !licq_ok(P, S, k, o->licq_screen)) {
+        rec->feas_status = MPQP_ST_SINGULAR;
+        return;
+    }
     /* feasibility LP (FP64 in both precisions) */
     if (fpar && o->warm_start) {
         memcpy(F->T, fpar->T, sizeof(double) * (size_t)(M1 + 1) * P->ld1);
@@ -959,6 +970,63 @@ static void eval_candidate(const Prob *P, const MpqpOptions *o, const int *S, in
 /* ------------------------------------------------------------------ */
 /* Problem setup                                                       */
 /* ------------------------------------------------------------------ */
+
+/* LICQ screen.  G = A Q^-1 A^T with A = A1[:n_c, :n_x] (the rows of AxT)
+ * and Q = the stationarity rows of A2 (columns x'), each restored to the
+ * sign of its positive diagonal entry (main.py:108-116 negates rows with a
+ * negative right-hand side).  Q^-1 A^T: Gauss-Jordan of [Q | A^T] with
+ * diagonal pivots (Q symmetric positive definite); G by dot products. */
+static double *build_gram(const Prob *P) {
+    int n_c = P->n_c, n_x = P->n_x, M2 = P->M2, M1 = P->M1;
+    int r4 = M2 - n_c - n_x, ldw = n_x + n_c;
+    double *W = (double *)malloc(sizeof(double) * (size_t)n_x * ldw);
+    double *G = (double *)malloc(sizeof(double) * (size_t)n_c * n_c);
+    for (int i = 0; i < n_x; i++) {
+        double sgn = P->A2[(size_t)i * M2 + r4 + i] < 0.0 ? -1.0 : 1.0;
+        for (int c = 0; c < n_x; c++) W[(size_t)i * ldw + c] = sgn * P->A2[(size_t)c * M2 + r4 + i];
+        for (int j = 0; j < n_c; j++) W[(size_t)i * ldw + n_x + j] = P->A1[(size_t)i * M1 + j];
+    }
+    for (int p = 0; p < n_x; p++) {
+        double piv = W[(size_t)p * ldw + p];
+        for (int j = 0; j < ldw; j++)
+            if (j != p) W[(size_t)p * ldw + j] /= piv;
+        for (int i = 0; i < n_x; i++) {
+            if (i == p) continue;
+            double f = W[(size_t)i * ldw + p];
+            for (int j = 0; j < ldw; j++)
+                if (j != p) W[(size_t)i * ldw + j] = fma(-f, W[(size_t)p * ldw + j], W[(size_t)i * ldw + j]);
+        }
+    }
+    for (int i = 0; i < n_c; i++)
+        for (int j = 0; j < n_c; j++) {
+            double acc = 0.0;
+            for (int k = 0; k < n_x; k++) acc = fma(P->A1[(size_t)k * M1 + i], W[(size_t)k * ldw + n_x + j], acc);
+            G[(size_t)i * n_c + j] = acc;
+        }
+    free(W);
+    return G;
+}
+
+/* 1 if the rows of S (ascending) pass the screen: right-looking Cholesky of
+ * G_SS, every pivot d_t > tol * G_tt (the original diagonal entry).  Sets
+ * of more than 64 constraints are not screened. */
+static int licq_ok(const Prob *P, const int *S, int k, double tol) {
+    if (!P->G || k < 1 || k > 64) return 1;
+    int n_c = P->n_c, ok = 1;
+    double *L = (double *)malloc(sizeof(double) * (size_t)k * k);
+    for (int r = 0; r < k; r++)
+        for (int q = 0; q < k; q++) L[r * k + q] = P->G[(size_t)S[r] * n_c + S[q]];
+    for (int t = 0; t < k; t++) {
+        double gtt = L[t * k + t];
+        if (!(gtt > tol * P->G[(size_t)S[t] * n_c + S[t]])) ok = 0;
+        double sq = sqrt(gtt > 0.0 ? gtt : 1.0);
+        for (int r = t + 1; r < k; r++) L[r * k + t] /= sq;
+        for (int r = t + 1; r < k; r++)
+            for (int q = t + 1; q <= r; q++) L[r * k + q] = fma(-L[r * k + t], L[q * k + t], L[r * k + q]);
+    }
+    free(L);
+    return ok;
+}
 
 static int prob_init(Prob *P, int n_c, int n_x, int n_t, const double *A1, const double *xb1,
                      const int *b1, int M1, int N1, const double *A2, const double *xb2,
@@ -1029,10 +1097,12 @@ static int prob_init(Prob *P, int n_c, int n_x, int n_t, const double *A1, const
     P->root_basis = (int *)malloc(sizeof(int) * (size_t)M2);
     P->root_nonb = (int *)malloc(sizeof(int) * (size_t)P->K2);
     build_root(P, o);
+    P->G = o->licq_screen > 0.0 ? build_gram(P) : NULL;
     return 0;
 }
 
 static void prob_free(Prob *P) {
+    free(P->G);
     free(P->root); free(P->root_basis); free(P->root_nonb);
     free(P->csc_ptr); free(P->csc_row); free(P->csc_val);
     free(P->csr_ptr); free(P->csr_var); free(P->csr_val);

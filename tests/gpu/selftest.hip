@@ -136,4 +136,45 @@ int selftest_arith(const double *a, const double *b, const double *c, double *q,
     hipFree(da); hipFree(db); hipFree(dc); hipFree(dq); hipFree(dr);
     return e == hipSuccess ? 0 : -3;
 }
+
+// G = A Y on the FP64 matrix cores (gram_mfma): A n_c x n_x, Y n_x x n_c,
+// both row-major; checks the operand / result lane layout
+int selftest_gram(const double *A, const double *Y, int n_c, int n_x, double *G) {
+    if (n_c < 1 || n_x < 1) return -1;
+    double *dA, *dY, *dG;
+    if (hipMalloc(&dA, 8 * (size_t)n_c * n_x) || hipMalloc(&dY, 8 * (size_t)n_c * n_x) ||
+        hipMalloc(&dG, 8 * (size_t)n_c * n_c))
+        return -2;
+    hipMemcpy(dA, A, 8 * (size_t)n_c * n_x, hipMemcpyHostToDevice);
+    hipMemcpy(dY, Y, 8 * (size_t)n_c * n_x, hipMemcpyHostToDevice);
+    const int tiles = (n_c + 15) / 16;
+    hipLaunchKernelGGL(gram_mfma, dim3(tiles * tiles), dim3(64), 0, 0, dA, dY, n_c, n_x, n_c, dG);
+    hipError_t e = hipMemcpy(G, dG, 8 * (size_t)n_c * n_c, hipMemcpyDeviceToHost);
+    hipFree(dA); hipFree(dY); hipFree(dG);
+    return e == hipSuccess ? 0 : -3;
+}
+
+// licq_screen over n candidate sets (rows of kw ints) of size `level`
+int selftest_licq(const double *G, int n_c, const int *sets, int n, int kw, int level, double tol, int *fail) {
+    if (n < 1 || level < 1 || level > 64 || kw < level) return -1;
+    double *dG; int *ds, *df;
+    if (hipMalloc(&dG, 8 * (size_t)n_c * n_c) || hipMalloc(&ds, 4 * (size_t)n * kw) || hipMalloc(&df, 4 * (size_t)n))
+        return -2;
+    hipMemcpy(dG, G, 8 * (size_t)n_c * n_c, hipMemcpyHostToDevice);
+    hipMemcpy(ds, sets, 4 * (size_t)n * kw, hipMemcpyHostToDevice);
+    const int gs = level <= 4 ? 4 : (level <= 8 ? 8 : (level <= 16 ? 16 : (level <= 32 ? 32 : 64)));
+    const int per = 64 / gs;
+    const dim3 grid((n + per - 1) / per);
+    const size_t lds = (size_t)64 * level * 8;
+    switch (gs) {
+    case 4: hipLaunchKernelGGL(licq_screen<4>, grid, dim3(64), lds, 0, dG, n_c, ds, n, kw, level, tol, df); break;
+    case 8: hipLaunchKernelGGL(licq_screen<8>, grid, dim3(64), lds, 0, dG, n_c, ds, n, kw, level, tol, df); break;
+    case 16: hipLaunchKernelGGL(licq_screen<16>, grid, dim3(64), lds, 0, dG, n_c, ds, n, kw, level, tol, df); break;
+    case 32: hipLaunchKernelGGL(licq_screen<32>, grid, dim3(64), lds, 0, dG, n_c, ds, n, kw, level, tol, df); break;
+    default: hipLaunchKernelGGL(licq_screen<64>, grid, dim3(64), lds, 0, dG, n_c, ds, n, kw, level, tol, df); break;
+    }
+    hipError_t e = hipMemcpy(fail, df, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    hipFree(dG); hipFree(ds); hipFree(df);
+    return e == hipSuccess ? 0 : -3;
+}
 }

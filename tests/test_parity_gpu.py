@@ -511,3 +511,104 @@ def test_duplicated_rows(hip, orc):
         assert compare_runs(g, o) == [], prec
     hip.set_options()
     orc.set_options()
+
+
+# ------------------------------------------------ LICQ screen (A Q^-1 A^T)
+
+def test_gram_mfma_layout(st):
+    """G = A Y on the FP64 matrix cores against numpy (operand and result
+    lane layout of v_mfma_f64_16x16x4f64, ragged edges)."""
+    _log("case: gram_mfma n_c=37 n_x=11 and n_c=16 n_x=4 vs numpy")
+    st.selftest_gram.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+    rng = np.random.default_rng(3)
+    for n_c, n_x in ((37, 11), (16, 4), (5, 9)):
+        A = np.ascontiguousarray(rng.standard_normal((n_c, n_x)))
+        Y = np.ascontiguousarray(rng.standard_normal((n_x, n_c)))
+        G = np.zeros((n_c, n_c))
+        assert st.selftest_gram(A.ctypes.data_as(_dp), Y.ctypes.data_as(_dp), n_c, n_x, G.ctypes.data_as(_dp)) == 0
+        assert np.allclose(G, A @ Y, rtol=1e-13, atol=1e-13)
+
+
+def test_licq_screen_kernel_matches_rank(st):
+    """The per-candidate Cholesky screen on G = A A^T of rows with exact
+    dependencies: flagged exactly where numpy.linalg.matrix_rank(A_S) < |S|,
+    for set sizes 1..9 (lane groups of 4, 8 and 16)."""
+    _log("case: licq_screen kernel vs matrix_rank, |S| = 1..9")
+    st.selftest_licq.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, _ip]
+    rng = np.random.default_rng(5)
+    n_c, n_x = 30, 12
+    A = rng.standard_normal((n_c, n_x))
+    A[7] = A[3]
+    A[11] = A[1] - 2.0 * A[2]
+    A[20] = 0.0
+    G = np.ascontiguousarray(A @ A.T)
+    for level in range(1, 10):
+        sets = np.array([np.sort(rng.choice(n_c, level, replace=False)) for _ in range(300)], dtype=np.intc)
+        if level >= 2:
+            sets[:40, :2] = [3, 7]
+            sets[:40] = np.sort(np.where(np.arange(level) < 2, sets[:40], (sets[:40] + 8) % n_c), axis=1)
+        sets = np.ascontiguousarray(sets)
+        fail = np.zeros(len(sets), np.intc)
+        assert st.selftest_licq(G.ctypes.data_as(_dp), n_c, sets.ctypes.data_as(_ip), len(sets), level, level,
+                                1e-14, fail.ctypes.data_as(_ip)) == 0
+        for t, S in enumerate(sets):
+            dep = len(set(S.tolist())) < level or np.linalg.matrix_rank(A[S]) < level
+            assert bool(fail[t]) == dep, (level, S.tolist())
+
+
+def test_licq_dependent_rows_parity(hip, orc):
+    """A duplicated row and a dependent triple: screened sets, their absent
+    children and every other record equal the oracle's (both precisions)."""
+    _log("case: dependent rows (duplicate + triple), n=8 m=24 p=3 |S|<=3, vs the oracle")
+    from test_licq_cpu import dependent_problem
+
+    lp = assemble_lps(dependent_problem())
+    for prec in (0, 1):
+        g, o = _run_pair(hip, orc, lp, 3, precision=prec)
+        assert compare_runs(g, o) == [], prec
+        assert (g.records["feas_status"] == 5).sum() > 0
+    hip.set_options()
+    orc.set_options()
+
+
+# ----------------------------------------- chunked levels, reference call
+
+@pytest.mark.parametrize("precision,batching", [(0, 0), (0, 1), (1, 1)])
+def test_chunked_run_matches_unchunked(hip, precision, batching):
+    """Levels split into chunks of at most 700 candidates (depth-first):
+    records, bases and final sets equal those of the unsplit run."""
+    _log(f"case: chunked (max_chunk=700) vs unchunked, n=30 m=62 p=8 |S|<=3, precision {precision}, "
+         f"batching {batching}")
+    lp = assemble_lps(config_problem(1)[0])
+    hip.set_options(record=2, precision=precision, batching=batching)
+    a = hip.run(lp, k_max=3, records=True, bases=True)
+    na = a.n_batches
+    hip.set_options(record=2, precision=precision, batching=batching, max_chunk=700)
+    b = hip.run(lp, k_max=3, records=True, bases=True)
+    hip.set_options()
+    assert compare_runs(a, b) == []
+    assert na == 4 and b.n_batches > 20
+
+
+def test_reference_call_kmax_nx(hip, orc):
+    """The reference host's call (main.py:140-144): k_min = 0, k_max = n_x,
+    AxT passed, reserved_memory = 2000 (main.py:68-69), through the raw
+    ctypes binding; n=6 m=40 p=3 enumerates |S| <= 6 (648,664 candidates),
+    record for record against the oracle, and again in chunks."""
+    _log("case: reference call k_max = n_x = 6, n=6 m=40 p=3, vs the oracle, and chunked")
+    lp = assemble_lps(synthetic_mpqp(6, 40, 3, 8))
+    hip.set_options(record=1)
+    res = hip.mpqp_raw(lp, 0, lp.n_x, reserved_memory=2000)
+    assert res, hip.last_error()
+    g = hip.run(lp, k_max=lp.n_x, reserved_memory=2000, records=True)
+    orc.set_options(record=1)
+    o = orc.run(lp, k_max=lp.n_x, records=True)
+    assert len(g.records) == 648664
+    assert compare_runs(g, o, bases=False) == []
+    hip.set_options(record=1, max_chunk=20000)
+    c = hip.run(lp, k_max=lp.n_x, records=True)
+    hip.set_options()
+    orc.set_options()
+    assert compare_runs(g, c, bases=False) == []
+    assert c.n_batches > g.n_batches
