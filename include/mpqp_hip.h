@@ -149,6 +149,25 @@ void mpqp_default_options(MpqpOptions *opt);
 int mpqp_set_options(const MpqpOptions *opt);   /* 0 ok, <0 invalid */
 int mpqp_get_options(MpqpOptions *opt);
 
+/* mpqp() for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Same inputs as mpqp() (extra_batches and
+ * reserved_memory dropped); the caller passes the result struct and the
+ * active-set rows (max_sets rows of n_x int32), and out->active_sets is set
+ * to that buffer.  All device memory of the call comes from one static
+ * 256 MiB array per GPU, held until the process ends: the call makes no
+ * hipMalloc / hipFree / hipHostMalloc and no library call that allocates
+ * device or pinned memory (host bookkeeping uses the C++ heap).  Records
+ * are not kept (MpqpOptions.record is ignored); the statistics are.
+ * Requires warm_up().  Returns 0; -1 on error (including a problem whose
+ * buffers exceed the 256 MiB: see mpqp_last_error); -2 if more than
+ * max_sets final sets were found (out->n_final_lps holds their count).
+ * Calls that overlap on one GPU share its array and race. */
+int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
+                 const double *A1, const double *xb1, const int *b1, int M1, int N1,
+                 const double *A2, const double *xb2, const int *b2, const int *non_b2,
+                 int M2, int N2, int n_eqs, GPUResults *out, int *active_sets, int max_sets);
+
 /* Frees a result returned by mpqp() (the reference has no free symbol).  A
  * result not freed by the caller is freed by the next mpqp() call, so a host
  * that never calls this holds at most one result. */

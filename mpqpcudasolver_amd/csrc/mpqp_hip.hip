@@ -19,7 +19,9 @@
 // split / root-search / compaction machinery of batched_simplex.cu:
 // 1141-1215,1449-1540 (see kernels.hpp for the kernels).
 //
-// Device memory comes from an arena of grow-only pools kept across calls.
+// Device memory comes from an arena: grow-only pools kept across calls for
+// mpqp(), or the fixed `scratch` array for mpqp_noalloc(), which allocates
+// no device memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +42,13 @@
 
 static_assert(sizeof(DevRecord) == sizeof(MpqpRecord), "record layout");
 static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
+
+// Device memory of mpqp_noalloc(): every call on a GPU uses that GPU's copy
+// of this array (it holds the last call's contents; overlapping calls on
+// one GPU race).  Host code addresses it only through the pointer
+// hipGetSymbolAddress returns (Ctx::scratch_ptr), never through its name.
+#define MPQP_SCRATCH_BYTES (256ull << 20)
+static __device__ __align__(256) unsigned char scratch[MPQP_SCRATCH_BYTES];
 
 namespace {
 
@@ -173,6 +182,56 @@ struct PoolArena : Arena {
     }
 };
 
+// The fixed scratch array (mpqp_noalloc()): per-call buffers bump-allocated
+// from its start, then one equal region per level, reset when a new chunk
+// of that level starts.
+struct ScratchArena : Arena {
+    unsigned char *base = nullptr;
+    size_t cap = 0, used = 0, lvl_size = 0;
+    std::vector<size_t> lvl_used;
+    std::map<std::pair<int, int>, BufView> named;
+    [[noreturn]] void overflow() const {
+        throw HipError{"mpqp_noalloc: the call needs more than the " + std::to_string(cap >> 20) +
+                       " MiB of the library's scratch memory"};
+    }
+    BufView get(int kind, int level, size_t bytes) override {
+        auto it = named.find({kind, level});
+        if (it != named.end() && it->second.n >= bytes) return BufView{bytes ? it->second.p : nullptr, bytes};
+        BufView v;
+        if (level < 0) {
+            if (!lvl_used.empty()) throw HipError{"mpqp_noalloc: per-call buffer requested after the level split"};
+            const size_t off = (used + 255) & ~(size_t)255;
+            if (off + bytes > cap) overflow();
+            v = BufView{base + off, bytes};
+            used = off + bytes;
+        } else {
+            if ((size_t)level >= lvl_used.size()) overflow();
+            const size_t off = (lvl_used[level] + 255) & ~(size_t)255;
+            if (off + bytes > lvl_size) overflow();
+            v = BufView{base + used + (size_t)level * lvl_size + off, bytes};
+            lvl_used[level] = off + bytes;
+        }
+        named[{kind, level}] = v;
+        return v;
+    }
+    size_t budget(size_t) override { return cap - std::min(cap, (used + 255) & ~(size_t)255); }
+    void begin_call() override {
+        used = 0;
+        lvl_size = 0;
+        lvl_used.clear();
+        named.clear();
+    }
+    void set_levels(int nl) override {
+        used = (used + 255) & ~(size_t)255;
+        lvl_size = ((cap - std::min(cap, used)) / (size_t)std::max(nl, 1)) & ~(size_t)255;
+        lvl_used.assign(nl, 0);
+    }
+    void begin_level(int level) override {
+        if ((size_t)level < lvl_used.size()) lvl_used[level] = 0;
+        for (auto it = named.begin(); it != named.end();) it = it->first.second == level ? named.erase(it) : ++it;
+    }
+};
+
 struct Ctx {
     bool ready = false;
     int device = 0;
@@ -196,6 +255,9 @@ struct Ctx {
     // caller freed it (the reference's host never frees its results)
     GPUResults *last_res = nullptr;
     PoolArena pools;
+    // device address of `scratch` on each device (set up by warm_up)
+    std::map<int, unsigned char *> scratch_ptr;
+    ScratchArena scratch_arena;
 };
 
 Ctx &ctx() {
@@ -265,6 +327,12 @@ void ensure_ready() {
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, c.device));
     c.n_cu = prop.multiProcessorCount;
+    // the scratch array's address on this device (host code uses only this)
+    void *sp = nullptr;
+    HIP_OK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    c.scratch_ptr[c.device] = static_cast<unsigned char *>(sp);
+    c.scratch_arena.base = static_cast<unsigned char *>(sp);
+    c.scratch_arena.cap = MPQP_SCRATCH_BYTES;
     c.ready = true;
 }
 
@@ -1174,8 +1242,8 @@ struct Run {
     }
 };
 
-// validation and the traversal of one call
-Output run_call(const Input &in, Arena &A, bool record_free) {
+// shared by mpqp() and mpqp_noalloc(): validation and the traversal
+Output run_call(const Input &in, Arena &A, bool noalloc) {
     Ctx &C = ctx();
     const std::string v = validate(in.n_c, in.n_x, in.n_t, in.b1, in.M1, in.N1, in.xb1, in.b2, in.non_b2, in.M2,
                                    in.N2, in.n_eqs, in.k_min, in.k_max);
@@ -1188,9 +1256,10 @@ Output run_call(const Input &in, Arena &A, bool record_free) {
         throw HipError{"mpqp: optimality LP bookkeeping exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
     MpqpOptions o = opts();
-    if (record_free) o.record = 0;
+    // record-free, device-side enumeration without pinned staging
+    if (noalloc) o.record = 0;
     A.begin_call();
-    Run r(C, o, A, in, record_free);
+    Run r(C, o, A, in, noalloc);
     r.run();
     return r.out;
 }
@@ -1245,6 +1314,50 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, i
     } catch (const std::exception &e) {
         C.err = e.what();
         return nullptr;
+    }
+}
+
+int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, const double *A1,
+                 const double *xb1, const int *b1, int M1, int N1, const double *A2, const double *xb2,
+                 const int *b2, const int *non_b2, int M2, int N2, int n_eqs, GPUResults *out, int *active_sets,
+                 int max_sets) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx &C = ctx();
+    C.err.clear();
+    if (!out || (max_sets > 0 && !active_sets) || max_sets < 0) {
+        C.err = "mpqp_noalloc: null output or negative max_sets";
+        return -1;
+    }
+    try {
+        if (!C.ready) throw HipError{"mpqp_noalloc: call warm_up() first"};
+        const Input in{AxT, n_c, n_x, n_t, k_min, k_max, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs,
+                       0};
+        const Output o = run_call(in, C.scratch_arena, true);
+        const int kw = o.kw;
+        const int nfin = (int)(o.fin.size() / (size_t)kw);
+        std::memset(out, 0, sizeof *out);
+        out->n_batches = o.n_batches;
+        out->n_splits = 1;
+        out->n_feasible_lps = o.n_feasible;
+        out->n_final_lps = nfin;
+        out->time_feas = o.tf;
+        out->time_opti = o.to;
+        out->time_GPU = o.t_gpu;
+        out->active_sets = active_sets;
+        if (nfin > max_sets) {
+            C.err = "mpqp_noalloc: " + std::to_string(nfin) + " final sets exceed max_sets " + std::to_string(max_sets);
+            return -2;
+        }
+        for (int row = 0; row < nfin; ++row)
+            for (int t = 0; t < n_x; ++t)
+                active_sets[(size_t)row * n_x + t] = t < kw ? o.fin[(size_t)row * kw + t] : -1;
+        return 0;
+    } catch (const HipError &e) {
+        C.err = e.msg;
+        return -1;
+    } catch (const std::exception &e) {
+        C.err = e.what();
+        return -1;
     }
 }
 

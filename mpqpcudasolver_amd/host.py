@@ -243,6 +243,12 @@ class MpqpLib:
         L.mpqp_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.mpqp_last_error.restype = ctypes.c_int
         L.mpqp_backend.restype = ctypes.c_char_p
+        if hasattr(L, "mpqp_noalloc"):
+            L.mpqp_noalloc.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, _ip, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, _ip, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(GPUResults), _ip, ctypes.c_int]
+            L.mpqp_noalloc.restype = ctypes.c_int
         self.has_stats = hasattr(L, "mpqp_get_stats")
         if self.has_stats:
             L.mpqp_get_stats.argtypes = [ctypes.POINTER(MpqpStats)]
@@ -344,6 +350,23 @@ class MpqpLib:
             run.feas_basis, run.p1_basis, run.p2_basis = fb, p1, p2
         self.lib.mpqp_free_results(res)
         return run
+
+    def run_noalloc(self, lp: LPData, k_max: int, k_min: int = 0, max_sets: int = 100000) -> MpqpRun:
+        """mpqp_noalloc(): the same call with caller-owned outputs and no
+        device allocation (raises on error or if max_sets is too small)."""
+        out = GPUResults()
+        rows = np.full((max(max_sets, 1), lp.n_x), -1, dtype=np.intc)
+        rc = self.lib.mpqp_noalloc(
+            _ptr(lp.A0, _dp), lp.n_c, lp.n_x, lp.n_t, k_min, k_max,
+            _ptr(lp.A1, _dp), _ptr(lp.xb1, _dp), _ptr(lp.b1, _ip), lp.M1, lp.N1,
+            _ptr(lp.A2, _dp), _ptr(lp.xb2, _dp), _ptr(lp.b2, _ip), _ptr(lp.non_b2, _ip),
+            lp.M2, lp.N2, lp.n_eqs, ctypes.byref(out), _ptr(rows, _ip), max_sets)
+        if rc != 0:
+            raise RuntimeError(f"mpqp_noalloc failed ({rc}): {self.last_error()}")
+        return MpqpRun(
+            n_batches=out.n_batches, n_splits=out.n_splits, n_feasible_lps=out.n_feasible_lps,
+            n_final_lps=out.n_final_lps, time_feas=out.time_feas, time_opti=out.time_opti,
+            time_GPU=out.time_GPU, active_sets=decode_active_sets(out.active_sets, out.n_final_lps, lp.n_x))
 
     def run_many(self, lps, k_max: int, records: bool = False, decode: bool = True) -> tuple[list, dict]:
         """One mpqp() call per problem, as the reference's host loops over its

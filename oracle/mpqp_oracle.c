@@ -1423,6 +1423,37 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
     return res;
 }
 
+/* The allocation-free entry point's contract on the CPU: mpqp() without
+ * records, results copied into the caller's struct and rows. */
+int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, const double *A1,
+                 const double *xb1, const int *b1, int M1, int N1, const double *A2, const double *xb2,
+                 const int *b2, const int *non_b2, int M2, int N2, int n_eqs, GPUResults *out, int *active_sets,
+                 int max_sets) {
+    if (!out || (max_sets > 0 && !active_sets) || max_sets < 0) {
+        set_err("mpqp_noalloc: null output or negative max_sets");
+        return -1;
+    }
+    MpqpOptions saved = *opts();
+    MpqpOptions o = saved;
+    o.record = 0;
+    g_opt = o;
+    GPUResults *res = mpqp((double *)AxT, n_c, n_x, n_t, k_min, k_max, 0, (double *)A1, (double *)xb1, (int *)b1,
+                           M1, N1, (double *)A2, (double *)xb2, (int *)b2, (int *)non_b2, M2, N2, n_eqs, 0);
+    g_opt = saved;
+    if (!res) return -1;
+    *out = *res;
+    out->active_sets = active_sets;
+    int rc = 0;
+    if (res->n_final_lps > max_sets) {
+        set_err("mpqp_noalloc: final sets exceed max_sets");
+        rc = -2;
+    } else {
+        memcpy(active_sets, res->active_sets, sizeof(int) * (size_t)res->n_final_lps * n_x);
+    }
+    mpqp_free_results(res);
+    return rc;
+}
+
 int mpqp_get_stats(MpqpStats *st) {
     if (!st) return -1;
     *st = g_stats;
