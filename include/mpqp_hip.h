@@ -99,7 +99,8 @@ typedef struct MpqpOptions {
                                batch iterations in which no LP finished; per LP, 300
                                drops LPs that converge later, so the default leaves
                                termination to max_iter_phase1 */
-    int opt_threads;        /* threads per optimality workgroup; 0 = auto */
+    int opt_threads;        /* threads per optimality workgroup (256, 512; 1024 for tableaus
+                               split between LDS and global memory); 0 = auto */
     int warm_start;         /* 1: children start from the parent's final basis */
     int autotune;           /* persistent batching: on the first call for a problem shape,
                                time the workgroup sizes of both kernels on a sample of the
@@ -128,6 +129,10 @@ typedef struct MpqpOptions {
                                   and no children.  1e-14 (0: off) */
     long long max_chunk;       /* most candidates of one level processed per launch; 0: derived
                                   from the free device memory minus reserved_memory */
+    int split_wgs;             /* an optimality tableau larger than LDS keeps as many rows in LDS
+                                  as fit when split_wgs workgroups share a CU (1 or 2), the rest
+                                  in global memory; 0: chosen per precision */
+    int reserved2;
 } MpqpOptions;
 
 /* One record per candidate active set of the last mpqp() call, in

@@ -54,8 +54,9 @@ struct OptArgs {
     int *fb_list;         // mixed: candidates whose FP32 result failed
     int *fb_count;        //        certification (appended), and their count
     long long *prof;      // phase cycle counts, 16 per workgroup (MPQP_PROFILE builds), or null
-    unsigned char *gtab;  // global-memory tableaus (one per workgroup) when LDS is too small, or null
-    long long gtab_stride;
+    unsigned char *gtab;  // split tableaus: rows [split_rows, M2) of each workgroup's in global
+    long long gtab_stride;  // memory (the tableau does not fit in LDS), or null
+    int split_rows;       // rows of a split tableau held in LDS (even), with the cost row
 };
 
 // Byte layout of one stored state: T (rows x cols doubles, contiguous) then
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
                 done = 1 + ST_OPTIMAL;
             } else {
                 // ratio test, lowest row on ties (pivot_kernel_1_2 :901-942)
-                r = primal_ratio_wave<double>(T, ld, M, aux, K, tl);
+                r = primal_ratio_wave(flat_tab(T, ld), M, aux, K, tl);
                 if (r < 0) done = 1 + ST_UNBOUNDED;
             }
             if (lane == 0) {

@@ -15,7 +15,11 @@
 // Arithmetic contract (bit-identical to the CPU restatement used as the
 // correctness check): one IEEE division per pivot-row entry, one fma per
 // other entry, sums only through the tree above; compiled with
-// -ffp-contract=off so no other fusion happens.
+// -ffp-contract=off so no other fusion happens.  A column whose (divided)
+// pivot-row entry is zero is left as it is: the fma would change at most
+// the sign of a zero entry, and 41-50% of pivot-row entries are zero in
+// these LPs, so the update skips those columns (the restatement skips the
+// same ones).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -86,6 +90,28 @@ struct DevRecord {
 };
 
 enum { ST_NOT_RUN = 0, ST_OPTIMAL = 1, ST_UNBOUNDED = 2, ST_ITER_CAP = 3, ST_STALL = 4, ST_SINGULAR = 5 };
+
+// Row access of a row-major tableau with leading dimension ld.  SPLIT: the
+// tableau does not fit in LDS; rows [0, R) and the cost row M are at lo (LDS),
+// rows [R, M) at hi (global memory).  Otherwise every row is at lo.
+template <typename Real, bool SPLIT>
+struct TabRef {
+    typedef Real real_t;
+    Real *lo, *hi;
+    int R, M, ld;
+    __device__ __forceinline__ Real *row(int i) const {
+        if constexpr (!SPLIT) {
+            return lo + i * ld;
+        } else {
+            return i < R ? lo + i * ld : (i == M ? lo + R * ld : hi + (i - R) * ld);
+        }
+    }
+};
+
+template <typename Real>
+__device__ __forceinline__ TabRef<Real, false> flat_tab(Real *T, int ld) {
+    return TabRef<Real, false>{T, nullptr, 0, 0, ld};
+}
 
 __device__ __forceinline__ int half_pow2(int n) {
     if (n <= 1) return 0;
@@ -409,14 +435,15 @@ __device__ __forceinline__ void lex_min(double &v, int &key, int &aux) {
 // primal ratio test on column q of the tableau (wave 0 only): rows with
 // T[i][q] > max(tol_plus, pivot_rel * max_i |T[i][q]|), minimum
 // T[i][RHS] / T[i][q], lowest row on ties.  Result broadcast to the wave.
-template <typename Real>
-__device__ __forceinline__ int primal_ratio_wave(const Real *T, int ld, int M, int q, int rhs, const DevTol &tl) {
+template <typename TB>
+__device__ __forceinline__ int primal_ratio_wave(const TB &T, int M, int q, int rhs, const DevTol &tl) {
+    typedef typename TB::real_t Real;
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     Real lim = (Real)tl.tol_plus;
     if (tl.pivot_rel > 0.0) {
         double cm = 0.0;
         for (int i = lane; i < M; i += MPQP_WAVE) {
-            const double a = fabs((double)T[i * ld + q]);
+            const double a = fabs((double)T.row(i)[q]);
             cm = a > cm ? a : cm;
         }
         const Real cmax = (Real)wave_max(cm);
@@ -426,9 +453,10 @@ __device__ __forceinline__ int primal_ratio_wave(const Real *T, int ld, int M, i
     double rv = 0.0;
     int rk = -1, dummy = 0;
     for (int i = lane; i < M; i += MPQP_WAVE) {
-        const Real al = T[i * ld + q];
+        const Real *ti = T.row(i);
+        const Real al = ti[q];
         if (al > lim) {
-            const Real rt = T[i * ld + rhs] / al;
+            const Real rt = ti[rhs] / al;
             if (rk < 0 || (double)rt < rv) { rv = (double)rt; rk = i; }
         }
     }
@@ -461,11 +489,12 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
         const int j = j0 + lane;
         if (j < ncols) {
             const Real prj = tr[j];
-            for (int i = wave; i < nrows; i += NW) {
-                if (i == r) continue;
-                Real *ti = T + i * ld;
-                ti[j] = fma(-colq[i], prj, ti[j]);
-            }
+            if (prj != Real(0))
+                for (int i = wave; i < nrows; i += NW) {
+                    if (i == r) continue;
+                    Real *ti = T + i * ld;
+                    ti[j] = fma(-colq[i], prj, ti[j]);
+                }
         }
     }
     __syncthreads();
@@ -474,19 +503,20 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
 // Three-phase form of the same pivot (column copy; division of row r and
 // zeroing of column q, while the last thread runs `between`; update), four
 // rows in flight per wave.
-template <int NT, typename Real, typename F>
-__device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncols, int r, int q,
-                                            Real *colq, F between) {
+template <int NT, typename TB, typename F>
+__device__ __forceinline__ void pivot_rows3(const TB &T, int nrows, int ncols, int r, int q,
+                                            typename TB::real_t *colq, F between) {
+    typedef typename TB::real_t Real;
     const int tid = threadIdx.x;
-    for (int i = tid; i < nrows; i += NT) colq[i] = T[i * ld + q];
+    for (int i = tid; i < nrows; i += NT) colq[i] = T.row(i)[q];
     __syncthreads();
     const Real p = colq[r];
-    Real *tr = T + r * ld;
+    Real *tr = T.row(r);
     for (int j = tid; j < ncols; j += NT)
         if (j != q) tr[j] = tr[j] / p;
     if (tid == 0) tr[q] = Real(1) / p;
     for (int i = tid; i < nrows; i += NT)
-        if (i != r) T[i * ld + q] = Real(0);
+        if (i != r) T.row(i)[q] = Real(0);
     if (tid == NT - 1) between();
     __syncthreads();
     constexpr int NW = NT / MPQP_WAVE;
@@ -497,20 +527,21 @@ __device__ __forceinline__ void pivot_rows3(Real *T, int ld, int nrows, int ncol
         const int j = j0 + lane;
         if (j < ncols) {
             const Real prj = tr[j];
-            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
-                Real t[U], f[U];
+            if (prj != Real(0))
+                for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
+                    Real t[U], f[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
-                    f[u] = i < nrows ? colq[i] : Real(0);
-                }
+                    for (int u = 0; u < U; ++u) {
+                        const int i = i0 + u;
+                        t[u] = i < nrows ? T.row(i)[j] : Real(0);
+                        f[u] = i < nrows ? colq[i] : Real(0);
+                    }
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, t[u]);
+                    for (int u = 0; u < U; ++u) {
+                        const int i = i0 + u;
+                        if (i < nrows && i != r) T.row(i)[j] = fma(-f[u], prj, t[u]);
+                    }
                 }
-            }
         }
     }
     __syncthreads();
@@ -524,7 +555,7 @@ template <int NT, typename F>
 __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, int ncols, int r, int q,
                                                   double *colq, F between) {
     if ((ld & 1) || ncols > 2 * MPQP_WAVE) {
-        pivot_rows3<NT, double>(T, ld, nrows, ncols, r, q, colq, between);
+        pivot_rows3<NT>(flat_tab(T, ld), nrows, ncols, r, q, colq, between);
         return;
     }
     const int tid = threadIdx.x;
@@ -546,10 +577,11 @@ __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, 
     const int npair = (ncols + 1) >> 1;
     const int rpw = MPQP_WAVE / npair;  // rows per wave step
     const int dr = lane / npair, c = lane - dr * npair;
-    if (dr < rpw) {
-        // a pair that runs past ncols updates the row's padding column
-        const int j = 2 * c;
-        const double2 pr = *reinterpret_cast<const double2 *>(tr + j);
+    const int j = 2 * c;
+    const double2 pr = *reinterpret_cast<const double2 *>(tr + (dr < rpw ? j : 0));
+    // a pair that runs past ncols updates the row's padding column; a pair
+    // of zero pivot-row entries is unchanged
+    if (dr < rpw && (pr.x != 0.0 || pr.y != 0.0)) {
         const int stride = NW * rpw;
         for (int i0 = wave * rpw + dr; i0 < nrows; i0 += stride * U) {
             double2 t[U];
@@ -565,8 +597,8 @@ __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, 
             for (int u = 0; u < U; ++u) {
                 const int i = i0 + u * stride;
                 double2 o;
-                o.x = fma(-f[u], pr.x, t[u].x);
-                o.y = fma(-f[u], pr.y, t[u].y);
+                o.x = pr.x != 0.0 ? fma(-f[u], pr.x, t[u].x) : t[u].x;
+                o.y = pr.y != 0.0 ? fma(-f[u], pr.y, t[u].y) : t[u].y;
                 if (i < nrows && i != r) *reinterpret_cast<double2 *>(T + i * ld + j) = o;
             }
         }
@@ -583,20 +615,21 @@ __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, 
 // wave writes T[r][q] = 1/p (no thread of (B) reads it: column q takes the
 // factor from a register) and runs `between` (the basis bookkeeping, which no
 // pivot step reads).  Per-entry arithmetic is identical to pivot_team.
-template <int NT, typename Real, typename F>
-__device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols, int r, int q,
-                                           Real *colq, F between) {
+template <int NT, typename TB, typename F>
+__device__ __forceinline__ void pivot_rows(const TB &T, int nrows, int ncols, int r, int q,
+                                           typename TB::real_t *colq, F between) {
+    typedef typename TB::real_t Real;
     if constexpr (sizeof(Real) == 8) {
         // FP64 (one workgroup per CU): measured faster with the column copy,
         // the division and the update in separate barrier phases
-        pivot_rows3<NT, Real>(T, ld, nrows, ncols, r, q, colq, between);
+        pivot_rows3<NT>(T, nrows, ncols, r, q, colq, between);
         return;
     }
     const int tid = threadIdx.x;
-    Real *tr = T + r * ld;
+    Real *tr = T.row(r);
     const Real p = tr[q];
     for (int i = tid; i < nrows; i += NT)
-        if (i != r) colq[i] = T[i * ld + q];
+        if (i != r) colq[i] = T.row(i)[q];
     for (int j = tid; j < ncols; j += NT)
         if (j != q) tr[j] = tr[j] / p;
     __syncthreads();
@@ -615,27 +648,28 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
     const int npair = (ncols + 1) >> 1;
     for (int c0 = 0; c0 < npair; c0 += MPQP_WAVE) {
         const int c = c0 + lane;
-        if (c < npair) {
-            const int j = 2 * c;
-            const bool q0 = j == q, q1 = j + 1 == q;
-            const float2 pr = *reinterpret_cast<const float2 *>(tr + j);
-            const Real p0 = q0 ? pinv : pr.x, p1 = q1 ? pinv : pr.y;
+        const int j = 2 * c;
+        const bool q0 = j == q, q1 = j + 1 == q;
+        const float2 pr = *reinterpret_cast<const float2 *>(tr + (c < npair ? j : 0));
+        const Real p0 = q0 ? pinv : pr.x, p1 = q1 ? pinv : pr.y;
+        // a pair of zero pivot-row entries is unchanged (column q never is)
+        if (c < npair && (p0 != Real(0) || p1 != Real(0))) {
             for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
                 float2 t[U];
                 Real f[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int i = i0 + u < nrows ? i0 + u : nrows - 1;
-                    t[u] = *reinterpret_cast<const float2 *>(T + i * ld + j);
+                    t[u] = *reinterpret_cast<const float2 *>(T.row(i) + j);
                     f[u] = colq[i];
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int i = i0 + u;
                     float2 o;
-                    o.x = fma(-f[u], p0, q0 ? Real(0) : t[u].x);
-                    o.y = fma(-f[u], p1, q1 ? Real(0) : t[u].y);
-                    if (i < nrows && i != r) *reinterpret_cast<float2 *>(T + i * ld + j) = o;
+                    o.x = p0 != Real(0) ? fma(-f[u], p0, q0 ? Real(0) : t[u].x) : t[u].x;
+                    o.y = p1 != Real(0) ? fma(-f[u], p1, q1 ? Real(0) : t[u].y) : t[u].y;
+                    if (i < nrows && i != r) *reinterpret_cast<float2 *>(T.row(i) + j) = o;
                 }
             }
         }
@@ -646,11 +680,13 @@ __device__ __forceinline__ void pivot_rows(Real *T, int ld, int nrows, int ncols
 // Pivot on row r with an entering column v held outside the tableau (rows
 // [0, nrows) of v); the leaving column is discarded.  Retained entries get
 // exactly the arithmetic of pivot_team.
-template <int NT, typename Real>
-__device__ __forceinline__ void pivot_ext(Real *T, int ld, int nrows, int ncols, int r, const Real *v) {
+template <int NT, typename TB>
+__device__ __forceinline__ void pivot_ext(const TB &T, int nrows, int ncols, int r,
+                                          const typename TB::real_t *v) {
+    typedef typename TB::real_t Real;
     const int tid = threadIdx.x;
     const Real p = v[r];
-    Real *tr = T + r * ld;
+    Real *tr = T.row(r);
     for (int j = tid; j < ncols; j += NT) tr[j] = tr[j] / p;
     __syncthreads();
     constexpr int NW = NT / MPQP_WAVE;
@@ -659,20 +695,20 @@ __device__ __forceinline__ void pivot_ext(Real *T, int ld, int nrows, int ncols,
     const int wave = tid / MPQP_WAVE;
     for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
         const int j = j0 + lane;
-        if (j < ncols) {
-            const Real prj = tr[j];
+        const Real prj = j < ncols ? tr[j] : Real(0);
+        if (prj != Real(0)) {
             for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
                 Real t[U], f[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int i = i0 + u;
-                    t[u] = i < nrows ? T[i * ld + j] : Real(0);
+                    t[u] = i < nrows ? T.row(i)[j] : Real(0);
                     f[u] = i < nrows ? v[i] : Real(0);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int i = i0 + u;
-                    if (i < nrows && i != r) T[i * ld + j] = fma(-f[u], prj, t[u]);
+                    if (i < nrows && i != r) T.row(i)[j] = fma(-f[u], prj, t[u]);
                 }
             }
         }
@@ -681,16 +717,17 @@ __device__ __forceinline__ void pivot_ext(Real *T, int ld, int nrows, int ncols,
 }
 
 // Reduced-cost row from a compact ascending list of the rows with cb != 0.
-template <int NT, typename Real, typename CbF, typename CnF>
-__device__ __forceinline__ void cost_row_list(Real *T, int ld, int M, int ncols, const int *rows, int nrows_cb,
+template <int NT, typename TB, typename CbF, typename CnF>
+__device__ __forceinline__ void cost_row_list(const TB &T, int M, int ncols, const int *rows, int nrows_cb,
                                               CbF cb, CnF cN) {
+    typedef typename TB::real_t Real;
     for (int j = threadIdx.x; j < ncols; j += NT) {
         Real acc = cN(j);
         for (int t = 0; t < nrows_cb; ++t) {
             const int i = rows[t];
-            acc = fma(-cb(i), T[i * ld + j], acc);
+            acc = fma(-cb(i), T.row(i)[j], acc);
         }
-        T[M * ld + j] = acc;
+        T.row(M)[j] = acc;
     }
     __syncthreads();
 }

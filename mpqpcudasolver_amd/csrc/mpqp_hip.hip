@@ -73,8 +73,10 @@ const void *opt_fn_nt(int S) {
 
 // The optimality kernel for a precision (FP32 pivots with certification, or
 // FP64), workgroup size, reduction slot count and tableau placement (LDS,
-// or global memory when it does not fit: 512 threads only).
+// or split between LDS and global memory when it does not fit: 512 or 1024
+// threads).
 const void *opt_fn(bool fp32, int nt, int S, bool gt) {
+    if (gt && nt == 1024) return fp32 ? opt_fn_nt<1024, true, true>(S) : opt_fn_nt<1024, false, true>(S);
     if (gt) return fp32 ? opt_fn_nt<512, true, true>(S) : opt_fn_nt<512, false, true>(S);
     if (fp32) return nt == 256 ? opt_fn_nt<256, true, false>(S) : opt_fn_nt<512, true, false>(S);
     return nt == 256 ? opt_fn_nt<256, false, false>(S) : opt_fn_nt<512, false, false>(S);
@@ -112,6 +114,19 @@ __global__ void fill_i32(int *p, int n, int v) {
 struct HipError {
     std::string msg;
 };
+
+// Rows of an optimality tableau too large for LDS that stay in LDS (even,
+// so that a state copy splits on a whole 16-byte vector), next to its cost
+// row and the bookkeeping, within `budget` bytes; the other rows live in
+// global memory.
+template <typename Real>
+int split_rows_for(int M, int K, int N2, bool cert, long long budget) {
+    const long long row = (long long)opt_ld<Real>(K) * (long long)sizeof(Real);
+    long long R = (budget - opt_lds_bytes_t<Real>(M, K, N2, cert, 1)) / row;
+    R = std::max(0LL, std::min<long long>(R, M)) & ~1LL;
+    while (R > 0 && opt_lds_bytes_t<Real>(M, K, N2, cert, (int)R + 1) > budget) R -= 2;
+    return (int)R;
+}
 
 #define HIP_OK(expr)                                                                   \
     do {                                                                               \
@@ -247,9 +262,10 @@ struct Ctx {
     long long lds_limit = 0;  // bytes of LDS one workgroup may allocate
     int n_cu = 0;
     MpqpStats stats{};
-    // autotuner: (M1, K1, M2, K2, n_c, FP32) -> (feasibility, optimality)
-    // threads per workgroup; the sizes used by the last call
-    std::map<std::vector<int>, std::pair<int, int>> tuned;
+    // autotuner: (M1, K1, M2, K2, n_c, FP32) -> feasibility and optimality
+    // threads per workgroup and, for a split tableau, the workgroups per CU
+    // its LDS share is sized for (FP32, FP64); the sizes of the last call
+    std::map<std::vector<int>, std::vector<int>> tuned;
     int last_feas_nt = 0, last_opt_nt = 0;
     // the result of the last mpqp() call, freed by the next call unless the
     // caller freed it (the reference's host never frees its results)
@@ -301,6 +317,7 @@ void default_opts(MpqpOptions *o) {
     o->pivot_rel32 = 1e-3;
     o->licq_screen = 1e-14;
     o->max_chunk = 0;
+    o->split_wgs = 0;
 }
 
 const MpqpOptions &opts() {
@@ -474,7 +491,10 @@ struct Run {
     bool mixed = false, keep_rec = false, keep_bases = false, screen = false;
     long long fstride = 0, flds = 0, ostride = 0, lds64 = 0, lds32 = 0;
     bool gt64 = false, gt32 = false;
+    int R64 = 0, R32 = 0;  // rows of a split tableau kept in LDS
     int S = 1, feas_nt = FEAS_NT, opt_nt = OPT_NT, cap64 = 0, cap32 = 0;
+    int w32 = 2, w64 = 1;                // workgroups per CU a split tableau is sized for
+    size_t gtab64_max = 0, gtab32_max = 0;  // global rows of the largest variant
     const void *fn64 = nullptr, *fn32 = nullptr;
     bool tune_pending = false;
     std::vector<int> tkey;
@@ -675,25 +695,69 @@ struct Run {
         const int n_c = in.n_c, M1 = in.M1, M2 = in.M2, N2 = in.N2;
         fstride = state_bytes(M1, K1 + 1, M1, K1);
         flds = feas_lds_bytes(M1, K1, n_c);
-        gt64 = opt_lds_bytes_t<double>(M2, K2, N2, false) > C.lds_limit;
-        gt32 = opt_lds_bytes_t<float>(M2, K2, N2, true) > C.lds_limit;
-        lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false, gt64);
-        lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true, gt32);
+        // static shared memory of the kernels (decision, S bits, reductions)
+        const long long budget = C.lds_limit - 512;
+        gt64 = opt_lds_bytes_t<double>(M2, K2, N2, false, M2 + 1) > budget;
+        gt32 = opt_lds_bytes_t<float>(M2, K2, N2, true, M2 + 1) > budget;
         S = red_slots(std::max(M2, N2 - in.n_eqs));
         feas_nt = FEAS_NT;
-        opt_nt = (gt64 || o.opt_threads != 256) ? OPT_NT : 256;
+        // 1024 threads only where every optimality kernel of the call splits its tableau
+        const bool all_split = gt64 && (gt32 || o.precision != MPQP_MIXED);
+        const int nt0 = o.opt_threads == 1024 ? (all_split ? 1024 : OPT_NT)
+                                              : ((gt64 || o.opt_threads != 256) ? OPT_NT : 256);
+        // a split tableau keeps in LDS what fits when w workgroups share a
+        // CU: by default FP64 one (its tableau is twice as large), FP32 two
+        // (one at 1024 threads, where one workgroup fills the CU)
+        const int sw64 = o.split_wgs > 0 ? o.split_wgs : 1;
+        const int sw32 = o.split_wgs > 0 ? o.split_wgs : (nt0 == 1024 ? 1 : 2);
+        // the global rows of every variant the autotuner may try fit the
+        // buffers sized here
+        gtab64_max = gtab32_max = 0;
+        for (const auto &v : split_variants()) {
+            select_variant(v[0], v[1], v[2]);
+            gtab64_max = std::max(gtab64_max, gt64 ? (size_t)opt_gtab_bytes<double>(M2, K2, R64) * cap64 : (size_t)0);
+            gtab32_max = std::max(gtab32_max, gt32 ? (size_t)opt_gtab_bytes<float>(M2, K2, R32) * cap32 : (size_t)0);
+        }
+        select_variant(nt0, sw32, sw64);
+        gtab64_max = std::max(gtab64_max, gt64 ? (size_t)opt_gtab_bytes<double>(M2, K2, R64) * cap64 : (size_t)0);
+        gtab32_max = std::max(gtab32_max, gt32 ? (size_t)opt_gtab_bytes<float>(M2, K2, R32) * cap32 : (size_t)0);
         tkey = {M1, K1, M2, K2, n_c, o.precision == MPQP_MIXED ? 1 : 0};
         tune_pending = false;
-        if (o.autotune && o.batching == MPQP_PERSISTENT && !gt64 && !gt32 && o.opt_threads == 0) {
+        const bool tuned_split = o.precision == MPQP_MIXED ? gt32 : gt64;
+        if (o.autotune && o.batching == MPQP_PERSISTENT && o.opt_threads == 0 && (!tuned_split || o.split_wgs == 0)) {
             auto it = C.tuned.find(tkey);
             if (it != C.tuned.end()) {
-                feas_nt = it->second.first;
-                opt_nt = it->second.second;
+                feas_nt = it->second[0];
+                select_variant(it->second[1], it->second[2], it->second[3]);
             } else {
                 tune_pending = true;
             }
         }
-        select_opt(opt_nt);
+    }
+
+    // (threads, FP32 workgroups per CU, FP64 workgroups per CU) the autotuner
+    // tries for the optimality kernel it times: workgroup sizes for an LDS
+    // tableau; for a split one also the LDS share of its rows
+    std::vector<std::vector<int>> split_variants() const {
+        const bool tuned_split = o.precision == MPQP_MIXED ? gt32 : gt64;
+        if (!tuned_split) {
+            // an LDS tableau; the FP64 re-solve of a mixed run may split its own
+            if (o.precision == MPQP_MIXED && gt64) return {{512, w32, w64}};
+            return {{256, w32, w64}, {512, w32, w64}};
+        }
+        if (o.precision == MPQP_MIXED) return {{512, 2, 1}, {512, 1, 1}, {1024, 1, 1}};
+        return {{512, 2, 2}, {512, 1, 1}, {1024, 1, 1}};
+    }
+
+    void select_variant(int nt, int ws32, int ws64) {
+        const int M2 = in.M2, N2 = in.N2;
+        w32 = ws32;
+        w64 = ws64;
+        R64 = gt64 ? split_rows_for<double>(M2, K2, N2, false, C.lds_limit / w64 - 512) : M2;
+        R32 = gt32 ? split_rows_for<float>(M2, K2, N2, true, C.lds_limit / w32 - 512) : M2;
+        lds64 = opt_lds_bytes_t<double>(M2, K2, N2, false, gt64 ? R64 + 1 : M2 + 1);
+        lds32 = opt_lds_bytes_t<float>(M2, K2, N2, true, gt32 ? R32 + 1 : M2 + 1);
+        select_opt(nt);
     }
 
     int capacity(const void *fn, int nt, long long lds) {
@@ -703,6 +767,12 @@ struct Run {
     }
 
     void select_opt(int nt) {
+        // one workgroup size serves both kernels of a call: a split kernel
+        // exists for 512 and 1024 threads, an LDS one for 256 and 512
+        const bool mx = o.precision == MPQP_MIXED;
+        const bool any_split = gt64 || (mx && gt32), all_split = gt64 && (!mx || gt32);
+        if (nt == 256 && any_split) nt = 512;
+        if (nt == 1024 && !all_split) nt = 512;
         opt_nt = nt;
         fn64 = opt_fn(false, opt_nt, S, gt64);
         fn32 = opt_fn(true, opt_nt, S, gt32);
@@ -728,8 +798,8 @@ struct Run {
 
     void plan_chunks() {
         // the per-call buffers first, so the budget sees what is left
-        if (gt64) A.get(K_GT64, -1, (size_t)opt_gtab_bytes<double>(in.M2, K2) * cap64);
-        if (gt32 && mixed) A.get(K_GT32, -1, (size_t)opt_gtab_bytes<float>(in.M2, K2) * cap32);
+        if (gt64) A.get(K_GT64, -1, gtab64_max);
+        if (gt32 && mixed) A.get(K_GT32, -1, gtab32_max);
         A.get(K_CTR, -1, sizeof(int) * 4);
         A.get(K_SUM, -1, sizeof(unsigned long long) * 8 * (size_t)(last_lvl + 1));
         const size_t budget = A.budget((size_t)std::max(in.reserved_memory, 0) << 20);
@@ -943,16 +1013,15 @@ struct Run {
             oa.b2 = bases ? db2.as<int>() : nullptr;
             BufView dfbl;
             if (mixed) dfbl = lvl_buf(L_FBL, level, sizeof(int) * (size_t)nf);
-            BufView gtab64 = gt64 ? A.get(K_GT64, -1, (size_t)opt_gtab_bytes<double>(in.M2, K2) * cap64) : BufView{};
-            BufView gtab32 = (gt32 && mixed) ? A.get(K_GT32, -1, (size_t)opt_gtab_bytes<float>(in.M2, K2) * cap32)
-                                             : BufView{};
+            BufView gtab64 = gt64 ? A.get(K_GT64, -1, gtab64_max) : BufView{};
+            BufView gtab32 = (gt32 && mixed) ? A.get(K_GT32, -1, gtab32_max) : BufView{};
             if (tune_here) {
-                // time the first kernel of the chunk at each workgroup size
-                // on a sample of the list (counters reset per trial)
+                // time the first kernel of the chunk in each variant on a
+                // sample of the list (counters reset per trial)
                 float best = INFINITY;
-                int best_nt = opt_nt;
-                for (int nt : {256, 512}) {
-                    select_opt(nt);
+                std::vector<int> best_v{opt_nt, w32, w64};
+                for (const auto &v : split_variants()) {
+                    select_variant(v[0], v[1], v[2]);
                     hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
                     HIP_OK(hipGetLastError());
                     OptArgs ot = oa;
@@ -961,6 +1030,13 @@ struct Run {
                     if (mixed) {
                         ot.fb_list = dfbl.as<int>();
                         ot.fb_count = dctr.as<int>() + 1;
+                        ot.gtab = gtab32.as<unsigned char>();
+                        ot.gtab_stride = opt_gtab_bytes<float>(in.M2, K2, R32);
+                        ot.split_rows = R32;
+                    } else {
+                        ot.gtab = gtab64.as<unsigned char>();
+                        ot.gtab_stride = opt_gtab_bytes<double>(in.M2, K2, R64);
+                        ot.split_rows = R64;
                     }
                     HIP_OK(hipEventRecord(e2, st));
                     launch_opt(mixed ? fn32 : fn64, std::min(ot.n, mixed ? cap32 : cap64), mixed ? lds32 : lds64, ot);
@@ -970,13 +1046,13 @@ struct Run {
                     HIP_OK(hipEventElapsedTime(&tms, e2, e3));
                     if (tms < best) {
                         best = tms;
-                        best_nt = nt;
+                        best_v = v;
                     }
                 }
-                select_opt(best_nt);
+                select_variant(best_v[0], best_v[1], best_v[2]);
                 hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, st, dctr.as<int>(), 4, 0);
                 HIP_OK(hipGetLastError());
-                C.tuned[tkey] = {feas_nt, opt_nt};
+                C.tuned[tkey] = {feas_nt, opt_nt, w32, w64};
                 tune_pending = false;
             }
             HIP_OK(hipEventRecord(e1, st));
@@ -985,7 +1061,8 @@ struct Run {
                 oa.fb_list = dfbl.as<int>();
                 oa.fb_count = dctr.as<int>() + 1;
                 oa.gtab = gtab32.as<unsigned char>();
-                oa.gtab_stride = opt_gtab_bytes<float>(in.M2, K2);
+                oa.gtab_stride = opt_gtab_bytes<float>(in.M2, K2, R32);
+                oa.split_rows = R32;
                 launch_opt(fn32, std::min(nf, cap32), lds32, oa);
                 // FP64 re-solve from the cold start of what failed certification
                 OptArgs ob = oa;
@@ -997,13 +1074,15 @@ struct Run {
                 ob.fb_list = nullptr;
                 ob.fb_count = nullptr;
                 ob.gtab = gtab64.as<unsigned char>();
-                ob.gtab_stride = opt_gtab_bytes<double>(in.M2, K2);
+                ob.gtab_stride = opt_gtab_bytes<double>(in.M2, K2, R64);
+                ob.split_rows = R64;
                 HIP_OK(hipEventRecord(e3, st));
                 launch_opt(fn64, std::min(nf, cap64), lds64, ob);
             } else if (o.batching == MPQP_PERSISTENT || gt64) {
                 oa.counter = dctr.as<int>();
                 oa.gtab = gtab64.as<unsigned char>();
-                oa.gtab_stride = opt_gtab_bytes<double>(in.M2, K2);
+                oa.gtab_stride = opt_gtab_bytes<double>(in.M2, K2, R64);
+                oa.split_rows = R64;
                 launch_opt(fn64, std::min(nf, cap64), lds64, oa);
             } else {
                 launch_opt(fn64, nf, lds64, oa);
@@ -1252,7 +1331,7 @@ Output run_call(const Input &in, Arena &A, bool noalloc) {
     if (feas_lds_bytes(in.M1, in.N1 - in.M1, in.n_c) > C.lds_limit)
         throw HipError{"mpqp: feasibility tableau exceeds the LDS of one workgroup (" + std::to_string(C.lds_limit) +
                        " bytes)"};
-    if (opt_lds_bytes_t<double>(in.M2, in.N2 - in.M2, in.N2, false, true) > C.lds_limit)
+    if (opt_lds_bytes_t<double>(in.M2, in.N2 - in.M2, in.N2, false, 1) > C.lds_limit - 512)
         throw HipError{"mpqp: optimality LP bookkeeping exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
     MpqpOptions o = opts();
@@ -1374,7 +1453,9 @@ int mpqp_set_options(const MpqpOptions *o) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!o || o->max_iter_feas < 0 || o->max_iter_phase1 < 0 || o->max_iter_phase2 < 0 ||
         !(o->tol_plus >= 0.0) || !(o->tol_minus <= 0.0) || !(o->licq_tol >= 0.0) || !(o->licq_screen >= 0.0) ||
-        o->max_chunk < 0 || (o->precision != MPQP_FP64 && o->precision != MPQP_MIXED) ||
+        o->max_chunk < 0 || o->split_wgs < 0 || o->split_wgs > 2 ||
+        (o->opt_threads != 0 && o->opt_threads != 256 && o->opt_threads != 512 && o->opt_threads != 1024) ||
+        (o->precision != MPQP_FP64 && o->precision != MPQP_MIXED) ||
         (o->batching != MPQP_PLAIN && o->batching != MPQP_PERSISTENT)) {
         ctx().err = "mpqp_set_options: invalid option values";
         return -1;

@@ -26,25 +26,27 @@ __host__ __device__ inline int opt_ld(int K) {
     return (e & 3) ? e : e + 2;
 }
 
-// LDS of one workgroup: T (M+1) x ld (unless the tableau lives in global
-// memory: gt), colq (M+1), vcol (M) in Real; then (8-byte aligned) bcnt,
-// nbcnt, basis, nonb, where, rows in int; for certification (FP32 pivots)
-// x and r in double.  During certification bcnt/nbcnt hold x by variable
-// (N2 doubles) and colq/vcol the list of nonbasic e-variables.
+// LDS of one workgroup: trows rows of T (M + 1 for the whole tableau; R + 1
+// when it is split: rows [0, R) and the cost row), colq (M+1), vcol (M) in
+// Real; then (8-byte aligned) bcnt, nbcnt, basis, nonb, where, rows in int;
+// for certification (FP32 pivots) x and r in double.  During certification
+// bcnt/nbcnt hold x by variable (N2 doubles) and colq/vcol the list of
+// nonbasic e-variables.
 template <typename Real>
-__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, bool gt = false) {
+__host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, int trows) {
     const int ld = opt_ld<Real>(K);
-    long long b = (long long)((gt ? 0 : (M + 1) * ld) + (M + 1) + M) * (long long)sizeof(Real);
+    long long b = (long long)((long long)trows * ld + (M + 1) + M) * (long long)sizeof(Real);
     b = ((b + 7) & ~7LL) + (long long)(M + K + 3 * N2) * 4;
     if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
     return (b + 15) & ~15LL;
 }
 
-// Global-memory tableau of one workgroup (when the LDS one does not fit):
-// (M+1) x ld Real, 256-byte aligned.
+// Global-memory rows [R, M) of a split tableau, one workgroup's share:
+// (M - R) x ld Real plus the up to 3 padding elements a state copy moves
+// past row M - 1 (opt_tw), 256-byte aligned.
 template <typename Real>
-__host__ __device__ inline long long opt_gtab_bytes(int M, int K) {
-    return ((long long)(M + 1) * opt_ld<Real>(K) * (long long)sizeof(Real) + 255) & ~255LL;
+__host__ __device__ inline long long opt_gtab_bytes(int M, int K, int R) {
+    return ((long long)(M - R) * opt_ld<Real>(K) * (long long)sizeof(Real) + 64 + 255) & ~255LL;
 }
 
 // Stored phase-1 state: T (M rows of ld Real, as in LDS, padded to a
@@ -94,19 +96,22 @@ struct OptShared {
 
 enum { ACT_PIVOT = 0, ACT_STOP = 1, ACT_UNPERT = 2, ACT_UNBOUNDED = 3 };
 
-template <typename Real>
+template <typename Real, bool SPLIT>
 struct OptLds {
     int ld;
-    Real *T, *colq, *vcol;
+    TabRef<Real, SPLIT> T;
+    Real *colq, *vcol;
     int *basis, *nonb, *where, *bcnt, *nbcnt, *rows;
     double *cx, *cr;  // certification vectors (FP32 pivots only)
     double *xv;       // certification: x by variable (over bcnt / nbcnt)
     int *elr, *els;   // certification: nonbasic e-list rho / slot (over colq / vcol)
-    // gT: the tableau in global memory, or null for the LDS tableau
-    __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT) {
+    // SPLIT: rows [R, M) of the tableau at gT (global memory), the rest in LDS
+    __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT, int R) {
         ld = opt_ld<Real>(K);
-        T = gT ? gT : reinterpret_cast<Real *>(smem);
-        colq = gT ? reinterpret_cast<Real *>(smem) : T + (M + 1) * ld;
+        Real *lo = reinterpret_cast<Real *>(smem);
+        const int trows = SPLIT ? R + 1 : M + 1;
+        T = TabRef<Real, SPLIT>{lo, gT, R, M, ld};
+        colq = lo + trows * ld;
         vcol = colq + (M + 1);
         const size_t ib = ((size_t)(reinterpret_cast<unsigned char *>(vcol + M) - smem) + 7) & ~(size_t)7;
         bcnt = reinterpret_cast<int *>(smem + ib);
@@ -135,8 +140,8 @@ struct OptLds {
 // Loads T from a stored state (same element type, rows of L.ld: a flat
 // 16-byte-vector copy) or from the FP64 root (rows of K+2, rounded to
 // Real), then basis, nonbasic and the position map where[].
-template <typename Real, int NT, typename Src>
-__device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, const Src *sT, int src_ld,
+template <typename Real, int NT, typename Src, bool SPLIT>
+__device__ __forceinline__ void opt_load(OptLds<Real, SPLIT> &L, const DevProb &P, const Src *sT, int src_ld,
                                          const int *sb, const int *sn) {
     const int M = P.M2, K = P.K2, N2 = P.N2;
     const int tid = threadIdx.x;
@@ -150,7 +155,11 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
         // the cost row (row M, rebuilt before use)
         const int nv = opt_tw<Real>(M, K) / V;
         const int4 *s4 = reinterpret_cast<const int4 *>(sT);
-        int4 *d4 = reinterpret_cast<int4 *>(L.T);
+        // a split tableau: vectors [0, nlo) to LDS, the rest (rows >= R, R
+        // even so that R * ld is a whole number of vectors) to global memory
+        const int nlo = SPLIT ? L.T.R * L.ld / V : nv;
+        int4 *dlo = reinterpret_cast<int4 *>(L.T.lo);
+        int4 *dhi = reinterpret_cast<int4 *>(L.T.hi) - nlo;
         // loads in flight per thread; indexes past the end are clamped to
         // the last vector (the same value copied again), so that the copy
         // has no branch
@@ -160,13 +169,17 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
 #pragma unroll
             for (int u = 0; u < UL; ++u) vv[u] = s4[min(e0 + u * NT, nv - 1)];
 #pragma unroll
-            for (int u = 0; u < UL; ++u) d4[min(e0 + u * NT, nv - 1)] = vv[u];
+            for (int u = 0; u < UL; ++u) {
+                const int e = min(e0 + u * NT, nv - 1);
+                if (SPLIT && e >= nlo) dhi[e] = vv[u];
+                else dlo[e] = vv[u];
+            }
         }
     } else {
         const int sw = K + 2, ne = M * sw;
         for (int e = tid; e < ne; e += NT) {
             const int i = e / sw, j = e - i * sw;
-            L.T[i * L.ld + j] = (Real)sT[i * src_ld + j];
+            L.T.row(i)[j] = (Real)sT[i * src_ld + j];
         }
     }
     // basis and nonbasic partition the N2 variables: every where[] entry is
@@ -190,8 +203,8 @@ __device__ __forceinline__ void opt_load(OptLds<Real> &L, const DevProb &P, cons
 // position of w_j, or replacement of its nonbasic column.
 // pre: if not NaN, this thread's lambda entry of S[j0] (lam_entry(P, S[j0],
 // row), read ahead by the caller)
-template <typename Real, int NT>
-__device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, const DevTol &tl, const int *Sset,
+template <typename Real, int NT, bool SPLIT>
+__device__ __forceinline__ int opt_lambda(OptLds<Real, SPLIT> &L, const DevProb &P, const DevTol &tl, const int *Sset,
                                           int j0, int j1, double pre = __builtin_nan("")) {
     const int M = P.M2, K = P.K2, N2 = P.N2, ld = L.ld;
     const int r4 = M - P.n_c - P.n_x;
@@ -219,7 +232,7 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
                     w[u] = wl[t];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) tv[u] = L.T[i * ld + (w[u] >= 0 ? 0 : -w[u] - 1)];
+                for (int u = 0; u < 4; ++u) tv[u] = L.T.row(i)[w[u] >= 0 ? 0 : -w[u] - 1];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const Real f1 = fma(av[u], (Real)1, acc);
@@ -235,10 +248,10 @@ __device__ __forceinline__ int opt_lambda(OptLds<Real> &L, const DevProb &P, con
         if (p >= 0) {
             const Real pv = L.vcol[p];
             if (!(fabs((double)pv) > tl.licq)) return 0;
-            pivot_ext<NT, Real>(L.T, ld, M, K + 2, p, L.vcol);
+            pivot_ext<NT>(L.T, M, K + 2, p, L.vcol);
         } else {
             const int kk = -p - 1;
-            for (int i = tid; i < M; i += NT) L.T[i * ld + kk] = L.vcol[i];
+            for (int i = tid; i < M; i += NT) L.T.row(i)[kk] = L.vcol[i];
             __syncthreads();
         }
     }
@@ -265,8 +278,8 @@ __device__ __forceinline__ void rows_where(int *rows, int M, OptShared &sh, F f)
 }
 
 // swap basis[r] <-> nonb[q] after a pivot, keeping where[] current
-template <typename Real>
-__device__ __forceinline__ void opt_swap(OptLds<Real> &L, int r, int q) {
+template <typename Real, bool SPLIT>
+__device__ __forceinline__ void opt_swap(OptLds<Real, SPLIT> &L, int r, int q) {
     const int t = L.basis[r];
     L.basis[r] = L.nonb[q];
     L.nonb[q] = t;
@@ -282,15 +295,15 @@ struct P1Result {
     double z1;
 };
 
-template <typename Real, int NT, int S>
-__device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = L.ld;
+template <typename Real, int NT, int S, bool SPLIT>
+__device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn;
     const int RHS = K, PERT = K + 1;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
-    Real *T = L.T;
+    const TabRef<Real, SPLIT> &T = L.T;
     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
-    cost_row_list<NT, Real>(
-        T, ld, M, K + 2, L.rows, sh.nrows, [&](int) { return (Real)1; },
+    cost_row_list<NT>(
+        T, M, K + 2, L.rows, sh.nrows, [&](int) { return (Real)1; },
         [&](int j) { return j < K ? (L.nonb[j] >= Nn ? (Real)1 : (Real)0) : (Real)0; });
     for (int v = tid; v < N2; v += NT) { L.bcnt[v] = 0; L.nbcnt[v] = 0; }
     __syncthreads();
@@ -320,10 +333,11 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
                 const int i = lane + MPQP_WAVE * s;
                 const int ic = i < M ? i : 0;
                 const int kc = i < K ? i : 0;
-                x0[s] = T[ic * ld + RHS];
-                xp[s] = T[ic * ld + PERT];
+                const Real *tic = T.row(ic);
+                x0[s] = tic[RHS];
+                xp[s] = tic[PERT];
                 bv[s] = L.basis[ic];
-                cl[s] = T[M * ld + kc];
+                cl[s] = T.row(M)[kc];
                 nv[s] = L.nonb[kc];
             }
 #pragma unroll
@@ -386,12 +400,12 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
                 if (pert) {
                     bool inf = false;
                     for (int i = lane; i < M; i += MPQP_WAVE)
-                        inf |= (double)T[i * ld + RHS] < tl.dual_stop_tol;
+                        inf |= (double)T.row(i)[RHS] < tl.dual_stop_tol;
                     if (__any(inf)) action = ACT_UNPERT;
                 }
             } else {
                 const int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
-                const Real *tr = T + r * ld;
+                const Real *tr = T.row(r);
                 Real al[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
@@ -451,7 +465,8 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
                 const int ic = i < M ? i : 0;
-                const Real x0 = T[ic * ld + RHS], xp = T[ic * ld + PERT];
+                const Real *tic = T.row(ic);
+                const Real x0 = tic[RHS], xp = tic[PERT];
                 const Real x = (pert && i < M) ? x0 + xp : x0;
                 ps[s] = (i < M && x < (Real)0) ? x : (Real)0;
             }
@@ -484,7 +499,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
         if (action == ACT_STOP) { stopped = true; break; }
         if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; return res; }
         if (action == ACT_UNPERT) continue;
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
+        pivot_rows<NT>(T, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
         OPT_PROF(sh, 2);
         ++res.iters;
         if (stall) { res.status = ST_STALL; return res; }
@@ -495,7 +510,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
     // on the phase-1 cost, Dantzig pricing with lowest variable on ties.
     for (; it < O.max_iter_p1; ++it) {
         if (wave == 0) {
-            const Real *tc = T + M * ld;
+            const Real *tc = T.row(M);
             double dv = 0.0;
             int key = -1, aux = -1;
             for (int kk = lane; kk < K; kk += MPQP_WAVE) {
@@ -506,7 +521,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
             lex_min(dv, key, aux);
             int action = ACT_STOP, r = -1;
             if (!(dv > tl.tol_minus)) {
-                r = primal_ratio_wave<Real>(T, ld, M, aux, RHS, tl);
+                r = primal_ratio_wave(T, M, aux, RHS, tl);
                 action = r >= 0 ? ACT_PIVOT : (tl.cleanup_accept ? ACT_STOP : ACT_UNBOUNDED);
             }
             if (lane == 0) {
@@ -520,7 +535,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
         if (action != ACT_PIVOT) __syncthreads();
         if (action == ACT_STOP) break;
         if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; return res; }
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
+        pivot_rows<NT>(T, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
         ++res.iters;
     }
     if (it >= O.max_iter_p1) { res.status = ST_ITER_CAP; return res; }
@@ -529,7 +544,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int i = lane + MPQP_WAVE * s;
-            zs[s] = (i < M) ? (L.basis[i] >= Nn ? (Real)1 : (Real)0) * T[i * ld + RHS] : (Real)0;
+            zs[s] = (i < M) ? (L.basis[i] >= Nn ? (Real)1 : (Real)0) * T.row(i)[RHS] : (Real)0;
         }
         const double zz = (double)tree_sum_s<S, Real>(zs, M, lane);
         if (lane == 0) sh.z = zz;
@@ -545,9 +560,9 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real> &L, const DevProb &P
 // After a feasible phase 1: pivot artificials still basic out of the basis
 // (ascending positions; entering slot = largest |T[i][k]| > tol_plus among
 // non-artificial slots, lowest slot on ties).  Returns 1 if none stays.
-template <typename Real, int NT, int S>
-__device__ __forceinline__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, Nn = P.Nn, ld = L.ld;
+template <typename Real, int NT, int S, bool SPLIT>
+__device__ __forceinline__ int opt_drive_out(OptLds<Real, SPLIT> &L, const DevProb &P, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, Nn = P.Nn;
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] >= Nn; });
     const int n = sh.nrows;
@@ -559,7 +574,7 @@ __device__ __forceinline__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, 
             int key = -1, aux = 0;
             for (int kk = lane; kk < K; kk += MPQP_WAVE) {
                 if (L.nonb[kk] >= Nn) continue;
-                const double a = fabs((double)L.T[i * ld + kk]);
+                const double a = fabs((double)L.T.row(i)[kk]);
                 if (a > tl.tol_plus && (key < 0 || -a < v)) { v = -a; key = kk; }
             }
             lex_min(v, key, aux);
@@ -572,7 +587,7 @@ __device__ __forceinline__ int opt_drive_out(OptLds<Real> &L, const DevProb &P, 
             clean = 0;
             continue;
         }
-        pivot_rows<NT, Real>(L.T, ld, M + 1, K + 2, i, best, L.colq, [&] { opt_swap(L, i, best); });
+        pivot_rows<NT>(L.T, M + 1, K + 2, i, best, L.colq, [&] { opt_swap(L, i, best); });
     }
     return clean;
 }
@@ -585,15 +600,15 @@ struct P2Result {
     double z2;
 };
 
-template <typename Real, int NT, int S>
-__device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
-    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, ld = L.ld, RHS = K;
+template <typename Real, int NT, int S, bool SPLIT>
+__device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const DevProb &P, const DevOpts &O, const DevTol &tl, OptShared &sh) {
+    const int M = P.M2, K = P.K2, N2 = P.N2, Nn = P.Nn, RHS = K;
     const int EPS = P.n_x + P.n_t + P.n_c;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1), wave = tid / MPQP_WAVE;
-    Real *T = L.T;
+    const TabRef<Real, SPLIT> &T = L.T;
     rows_where<S>(L.rows, M, sh, [&](int i) { return L.basis[i] == EPS; });
-    cost_row_list<NT, Real>(
-        T, ld, M, K + 1, L.rows, sh.nrows, [&](int) { return (Real)-1; },
+    cost_row_list<NT>(
+        T, M, K + 1, L.rows, sh.nrows, [&](int) { return (Real)-1; },
         [&](int j) { return j < K ? (L.nonb[j] == EPS ? (Real)-1 : (Real)0) : (Real)0; });
     for (int v = tid; v < N2; v += NT) L.bcnt[v] = 0;
     __syncthreads();
@@ -610,7 +625,7 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P
                 cn[s] = 0;
                 if (var < Nn) {
                     const int w = L.where[var];
-                    v[s] = w >= 0 ? 100000.0 : (double)T[M * ld + (-w - 1)];
+                    v[s] = w >= 0 ? 100000.0 : (double)T.row(M)[-w - 1];
                     cn[s] = L.bcnt[var];
                 }
             }
@@ -625,13 +640,13 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int i = lane + MPQP_WAVE * s;
-                    zs[s] = (i < M) ? (L.basis[i] == EPS ? (Real)-1 : (Real)0) * T[i * ld + RHS] : (Real)0;
+                    zs[s] = (i < M) ? (L.basis[i] == EPS ? (Real)-1 : (Real)0) * T.row(i)[RHS] : (Real)0;
                 }
                 zz = (double)tree_sum_s<S, Real>(zs, M, lane);
                 action = ACT_STOP;
             } else {
                 q = -L.where[wv] - 1;
-                r = primal_ratio_wave<Real>(T, ld, M, q, RHS, tl);
+                r = primal_ratio_wave(T, M, q, RHS, tl);
                 if (r < 0) action = ACT_UNBOUNDED;
             }
             if (lane == 0) {
@@ -654,7 +669,7 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real> &L, const DevProb &P
             res.status = ST_UNBOUNDED;
             break;
         }
-        pivot_rows<NT, Real>(T, ld, M + 1, K + 1, r, q, L.colq, [&] { opt_swap(L, r, q); });
+        pivot_rows<NT>(T, M + 1, K + 1, r, q, L.colq, [&] { opt_swap(L, r, q); });
         ++res.iters;
     }
     res.final_set = (res.status == ST_OPTIMAL && -res.z2 > tl.final_tol) ? 1 : 0;
@@ -778,8 +793,8 @@ struct SBits {
 
 // Compact list of the nonbasic e-variables, rho ascending: L.elr[k] (rho),
 // L.els[k] (slot).  Returns their count.
-template <int S>
-__device__ __forceinline__ int elist_build(OptLds<float> &L, const DevProb &P, OptShared &sh) {
+template <int S, typename LT>
+__device__ __forceinline__ int elist_build(LT &L, const DevProb &P, OptShared &sh) {
     const int M = P.M2, e0 = P.N2 - M;
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     if (threadIdx.x < MPQP_WAVE) {
@@ -889,10 +904,10 @@ __device__ __forceinline__ double col_resid(const DevProb &P, const SBits &sb, i
 // x (LDS, by position; L.xv by variable, NaN when nonbasic, a NaN value of
 // a basic variable stored as +inf) := refined solution of B x = xb2; 1 once
 // max|xb2 - B x| <= cert_ptol.  *ne_out: the number of nonbasic e-variables.
-template <int NT, int S>
-__device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
+template <int NT, int S, typename LT>
+__device__ __forceinline__ int cert_primal(LT &L, const DevProb &P, const unsigned *inS, double *red,
                                            OptShared &sh, const int *Sset, int k, int *ne_out) {
-    const int M = P.M2, N2 = P.N2, RHS = P.K2, ld = L.ld, e0 = P.N2 - M;
+    const int M = P.M2, N2 = P.N2, RHS = P.K2, e0 = P.N2 - M;
     double *x = L.cx, *r = L.cr;
     const int ne = elist_build<S>(L, P, sh);
     *ne_out = ne;
@@ -902,7 +917,7 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
     const int xb0 = M - n_c - P.n_x, xb1 = M - n_c;
     for (int v = threadIdx.x; v < N2; v += NT) {
         const int w = L.where[v];
-        const double xw = w >= 0 ? (double)L.T[w * ld + RHS] : 0.0;
+        const double xw = w >= 0 ? (double)L.T.row(w)[RHS] : 0.0;
         L.xv[v] = w >= 0 ? (xw == xw ? xw : INFINITY) : __builtin_nan("");
         if (w >= 0) x[w] = xw;
     }
@@ -978,7 +993,7 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
             // (B32^-1 r)_i: the unit term of a basic e at position i, then
             // the nonbasic e-columns, rho ascending
             const int i = q / MPQP_GB, g = q % MPQP_GB;
-            const float *Ti = L.T + i * ld;
+            const float *Ti = L.T.row(i);
             const int bv = L.basis[i];
             double acc = (g == 0 && bv >= e0) ? r[bv - e0] : 0.0;
             int k = g;
@@ -1015,17 +1030,17 @@ __device__ __forceinline__ int cert_primal(OptLds<float> &L, const DevProb &P, c
 // the FP32 cost row (y_rho = c(e_rho) - d(e_rho), d = 0 for a basic e: the
 // e-columns of A2 are the identity); 1 once max|c_B - B^T y| <= 2e-12.
 // ne: the e-list of cert_primal (same basis).
-template <int NT, int S, typename CostF>
-__device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &P, const unsigned *inS, double *red,
+template <int NT, int S, typename LT, typename CostF>
+__device__ __forceinline__ int cert_dual(const LT &L, const DevProb &P, const unsigned *inS, double *red,
                                          CostF cost, OptShared &sh, int ne) {
-    const int M = P.M2, ld = L.ld, e0 = P.N2 - M;
+    const int M = P.M2, e0 = P.N2 - M;
     double *y = L.cx, *r = L.cr;
     SBits sb;
     sb.load(inS);
     for (int rho = threadIdx.x; rho < M; rho += NT) {
         const int w = L.where[e0 + rho];
         const double c = cost(e0 + rho);
-        y[rho] = w >= 0 ? c : c - (double)L.T[M * ld + (-w - 1)];
+        y[rho] = w >= 0 ? c : c - (double)L.T.row(M)[-w - 1];
     }
     int *ord = L.bcnt;  // positions, long columns first (bcnt is free after phase 1)
     order_flag_first<S>(ord, M, [&](int i) { return long_col(P, L.basis[i]); });
@@ -1049,7 +1064,7 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
             // positions ascending
             const int k = q / MPQP_GT, g = q % MPQP_GT;
             const int rho = L.elr[k];
-            const float *Tk = L.T + L.els[k];
+            const int col = L.els[k];
             double acc = 0.0;
             int i = g;
             for (; i + (MPQP_CBL - 1) * MPQP_GT < M; i += MPQP_CBL * MPQP_GT) {
@@ -1057,12 +1072,12 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
 #pragma unroll
                 for (int u = 0; u < MPQP_CBL; ++u) {
                     rv[u] = r[i + u * MPQP_GT];
-                    tv[u] = (double)Tk[(i + u * MPQP_GT) * ld];
+                    tv[u] = (double)L.T.row(i + u * MPQP_GT)[col];
                 }
 #pragma unroll
                 for (int u = 0; u < MPQP_CBL; ++u) acc = fma(rv[u], tv[u], acc);
             }
-            for (; i < M; i += MPQP_GT) acc = fma(r[i], (double)Tk[i * ld], acc);
+            for (; i < M; i += MPQP_GT) acc = fma(r[i], (double)L.T.row(i)[col], acc);
             acc = team_add<MPQP_GT>(acc);
             if (g == 0) y[rho] = y[rho] + acc;
         }
@@ -1078,8 +1093,8 @@ __device__ __forceinline__ int cert_dual(const OptLds<float> &L, const DevProb &
 
 // min over the selected nonbasic slots of the FP64 reduced cost c_v - a_v^T y
 // (a NaN reduced cost counts as -inf)
-template <int NT, int S, typename Sel, typename CostF>
-__device__ __forceinline__ double min_red_cost(OptLds<float> &L, const DevProb &P, const unsigned *inS,
+template <int NT, int S, typename LT, typename Sel, typename CostF>
+__device__ __forceinline__ double min_red_cost(LT &L, const DevProb &P, const unsigned *inS,
                                                double *red, Sel sel, CostF cost) {
     SBits sb;
     sb.load(inS);
@@ -1102,8 +1117,8 @@ __device__ __forceinline__ double min_red_cost(OptLds<float> &L, const DevProb &
 
 // FP64 tree sum of cb(basis[i]) * x[i] in the reference order (wave 0),
 // broadcast to the workgroup through red[0]
-template <int NT, int S, typename CbF>
-__device__ __forceinline__ double cert_z(const OptLds<float> &L, const DevProb &P, double *red, CbF cbf) {
+template <int NT, int S, typename LT, typename CbF>
+__device__ __forceinline__ double cert_z(const LT &L, const DevProb &P, double *red, CbF cbf) {
     const int M = P.M2, lane = threadIdx.x & (MPQP_WAVE - 1);
     if (threadIdx.x < MPQP_WAVE) {
         double zs[S];
@@ -1123,8 +1138,8 @@ __device__ __forceinline__ double cert_z(const OptLds<float> &L, const DevProb &
 
 // FP64 tree sum of xb2[rho] * y[rho] (the dual objective of the refined y in
 // L.cx) in the reference order (wave 0), broadcast through red[0]
-template <int NT, int S>
-__device__ __forceinline__ double cert_zdual(const OptLds<float> &L, const DevProb &P, double *red) {
+template <int NT, int S, typename LT>
+__device__ __forceinline__ double cert_zdual(const LT &L, const DevProb &P, double *red) {
     const int M = P.M2, lane = threadIdx.x & (MPQP_WAVE - 1);
     if (threadIdx.x < MPQP_WAVE) {
         double zs[S];
@@ -1143,8 +1158,8 @@ __device__ __forceinline__ double cert_zdual(const OptLds<float> &L, const DevPr
 }
 
 // minimum of x over the workgroup
-template <int NT>
-__device__ __forceinline__ double cert_xmin(const OptLds<float> &L, int M, double *red) {
+template <int NT, typename LT>
+__device__ __forceinline__ double cert_xmin(const LT &L, int M, double *red) {
     double mn = INFINITY;
     for (int i = threadIdx.x; i < M; i += NT) mn = fmin(mn, L.cx[i]);
     return -block_max<NT>(-mn, red);
@@ -1156,21 +1171,24 @@ __device__ __forceinline__ double cert_xmin(const OptLds<float> &L, int M, doubl
 
 // Rounds the phase-1 state (T, basis, nonbasic) into the store of the next
 // level, in the store's element type.
-template <typename Real, typename Dst, int NT>
-__device__ __forceinline__ void opt_store(const OptLds<Real> &L, int M, int K, unsigned char *dst) {
+template <typename Real, typename Dst, int NT, bool SPLIT>
+__device__ __forceinline__ void opt_store(const OptLds<Real, SPLIT> &L, int M, int K, unsigned char *dst) {
     // the state holds rows of the destination precision's stride
     const int tw = opt_tw<Dst>(M, K);
     Dst *dT = reinterpret_cast<Dst *>(dst);
     constexpr int V = 16 / sizeof(Real);
     if (sizeof(Dst) == sizeof(Real)) {
+        // a split tableau: vectors [0, nlo) from LDS, the rest from global memory
         int4 *d4 = reinterpret_cast<int4 *>(dT);
-        const int4 *s4 = reinterpret_cast<const int4 *>(L.T);
-        for (int e = threadIdx.x; e < tw / V; e += NT) d4[e] = s4[e];
+        const int nlo = SPLIT ? L.T.R * L.ld / V : tw / V;
+        const int4 *slo = reinterpret_cast<const int4 *>(L.T.lo);
+        const int4 *shi = reinterpret_cast<const int4 *>(L.T.hi) - nlo;
+        for (int e = threadIdx.x; e < tw / V; e += NT) d4[e] = (SPLIT && e >= nlo) ? shi[e] : slo[e];
     } else {
         const int sw = K + 2, ldd = opt_ld<Dst>(K);
         for (int e = threadIdx.x; e < M * sw; e += NT) {
             const int i = e / sw, j = e - i * sw;
-            dT[i * ldd + j] = (Dst)L.T[i * L.ld + j];
+            dT[i * ldd + j] = (Dst)L.T.row(i)[j];
         }
     }
     int *db = reinterpret_cast<int *>(dT + (long long)tw);
@@ -1195,9 +1213,9 @@ __device__ __forceinline__ void opt_body(const OptArgs &a) {
     const int tid = threadIdx.x;
     const int k = a.level;
     const int n = a.n_dev ? *a.n_dev : a.n;
-    OptLds<Real> L;
+    OptLds<Real, GT> L;
     L.carve(smem, M, K, N2, MIXED,
-            GT ? reinterpret_cast<Real *>(a.gtab + (long long)blockIdx.x * a.gtab_stride) : nullptr);
+            GT ? reinterpret_cast<Real *>(a.gtab + (long long)blockIdx.x * a.gtab_stride) : nullptr, a.split_rows);
     const int ld = L.ld;
     const DevTol tl = MIXED ? dev_tol32(O) : dev_tol64(O);
     // persistent mode: the first work item is taken here; each later one is

@@ -88,6 +88,8 @@ class MpqpOptions(ctypes.Structure):
         ("pivot_rel32", ctypes.c_double),
         ("licq_screen", ctypes.c_double),
         ("max_chunk", ctypes.c_longlong),
+        ("split_wgs", ctypes.c_int),
+        ("reserved2", ctypes.c_int),
     ]
 
 

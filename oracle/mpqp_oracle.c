@@ -154,6 +154,7 @@ void mpqp_default_options(MpqpOptions *o) {
     o->pivot_rel32 = 1e-3;
     o->licq_screen = 1e-14;
     o->max_chunk = 0; /* no effect on the CPU */
+    o->split_wgs = 0; /* no effect on the CPU */
 }
 
 static const MpqpOptions *opts(void) {

@@ -255,12 +255,19 @@ def test_parity_config4_first_levels(hip, orc, precision):
     orc.set_options()
 
 
+SPLITS = [dict(), dict(batching=1, split_wgs=2, opt_threads=512), dict(batching=1, split_wgs=1, opt_threads=512),
+          dict(batching=1, split_wgs=1, opt_threads=1024), dict(batching=1)]
+
+
+@pytest.mark.parametrize("split", range(len(SPLITS)), ids=["plain", "512x2", "512x1", "1024x1", "autotuned"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_parity_config4_two_levels(hip, orc, precision):
-    """|S| <= 2 (7,261 candidates), record for record with bases."""
-    _log(f"case: config 4 |S| <= 2, precision {precision}, every record and basis vs the oracle")
+def test_parity_config4_two_levels(hip, orc, precision, split):
+    """|S| <= 2 (7,261 candidates), record for record with bases, for every
+    way of splitting the tableau between LDS and global memory (and the
+    autotuner's choice)."""
+    _log(f"case: config 4 |S| <= 2, precision {precision}, split {SPLITS[split]}, every record and basis vs the oracle")
     prog, _ = config_problem(4)
-    g, o = _run_pair(hip, orc, assemble_lps(prog), 2, precision=precision)
+    g, o = _run_pair(hip, orc, assemble_lps(prog), 2, precision=precision, **SPLITS[split])
     hip.set_options()
     orc.set_options()
     assert len(g.records) == 7261
