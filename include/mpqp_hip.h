@@ -222,6 +222,66 @@ int mpqp_get_tuning(int *feas_threads, int *opt_threads);
  * ds_read_b128 streaming kernel over all CUs; <0 on error. */
 double mpqp_measure_lds_bandwidth(void);
 
+/* ---- Critical regions -------------------------------------------------
+ * mpqp_regions() replaces compute_critical_region (utils.py:23-97) applied
+ * to every final active set, as active_sets_to_regions does
+ * (utils.py:128-144, called at main.py:154-157).  Arguments are those of
+ * compute_critical_region, as row-major float64 numpy arrays (C order, not
+ * the column-major layout of mpqp()): A n_c x n_x, b n_c, c n_x, H n_x x n_t,
+ * inv_Q n_x x n_x (the reference passes pinv(Q), utils.py:155), F n_c x n_t,
+ * CRA n_cp x n_t, CRb n_cp; and the sets in the layout of
+ * GPUResults.active_sets: n_sets rows of `width` int32 (width = n_x there),
+ * ascending constraint indices padded with -1 (decoded as utils.py:133-138).
+ *
+ * Per set i (any output pointer may be NULL; m0 = n_c + n_cp):
+ *   info[i]                  status, half-spaces kept, Chebyshev radius
+ *   center       n_t         a centre of the largest ball (the radius is
+ *                            unique, the centre need not be)
+ *   parameter_A  n_x x n_t,  parameter_b n_x      x(theta) (utils.py:34-35)
+ *   lagrange_A   width x n_t, lagrange_b width    lambda(theta) (utils.py:
+ *                            29-30), rows past |S| zero
+ *   cr_A         m0 x n_t,   cr_b m0              the unit-norm half-spaces
+ *                            CR_As, CR_bs (utils.py:61), first n_rows rows
+ *                            valid, the rest zero
+ * The reference keeps a set (returns a region) exactly when status is
+ * MPQP_REGION_FOUND.  A set whose (A Q^-1 A^T)_SS is not positive definite
+ * gets MPQP_REGION_SINGULAR (the reference uses pinv there; sets returned
+ * by mpqp() with the LICQ screen on never are).  Requires warm_up().
+ * Returns the number of regions found, -1 on error (mpqp_last_error). */
+enum {
+    MPQP_REGION_EMPTY = 0,      /* Chebyshev LP infeasible or radius < 1e-8 */
+    MPQP_REGION_FOUND = 1,      /* radius >= 1e-8 */
+    MPQP_REGION_UNBOUNDED = 2,  /* Chebyshev LP unbounded (no region in the reference) */
+    MPQP_REGION_ITER_LIMIT = 3, /* simplex iteration limit reached */
+    MPQP_REGION_SINGULAR = 4    /* (A Q^-1 A^T)_SS not positive definite */
+};
+
+typedef struct MpqpRegionInfo {
+    int status;     /* MPQP_REGION_* */
+    int n_rows;     /* half-spaces kept (rows of cr_A) */
+    double radius;  /* Chebyshev radius (the LP optimum; may be < 0 when empty) */
+    int iters;      /* simplex pivots of the Chebyshev LP */
+    int reserved;
+} MpqpRegionInfo;
+
+int mpqp_regions(const double *A, const double *b, const double *c, const double *H,
+                 const double *inv_Q, const double *F, const double *CRA, const double *CRb,
+                 int n_c, int n_x, int n_t, int n_cp, const int *active_sets, int n_sets,
+                 int width, MpqpRegionInfo *info, double *center, double *parameter_A,
+                 double *parameter_b, double *lagrange_A, double *lagrange_b, double *cr_A,
+                 double *cr_b);
+
+/* Work and time of the last mpqp_regions() call (HIP-event intervals). */
+typedef struct MpqpRegionStats {
+    long long sets, found, pivots;
+    double ms_prep;  /* the two products Q^-1 [A^T | H | c] and A Q^-1 [...] */
+    double ms_sets;  /* the per-set kernel, summed over its launches */
+    int launches;
+    int reserved;
+} MpqpRegionStats;
+
+int mpqp_get_region_stats(MpqpRegionStats *stats);
+
 /* Copies the last error message; returns its length (0 if none). */
 int mpqp_last_error(char *buf, int len);
 

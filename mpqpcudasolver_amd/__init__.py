@@ -2,7 +2,8 @@
 
 The product is the C-ABI library `libmpqp_hip.so` (hand-written HIP kernels);
 this package holds the ctypes host that mirrors the reference's binding
-(main.py:11-60), the problem generators and the LP assembly (main.py:87-138).
+(main.py:11-60), the problem generators, the LP assembly (main.py:87-138) and
+the critical-region calls (utils.py:23-144).
 """
 from .problem import (  # noqa: F401
     MPQPProb, LPData, assemble_lps, synthetic_mpqp, precision_stress_mpqp,
@@ -11,4 +12,5 @@ from .problem import (  # noqa: F401
 from .host import (  # noqa: F401
     GPUResults, MpqpOptions, MpqpRecord, MpqpLib, MpqpRun, library_path,
     load_library, decode_active_sets, calculate_mpqp_with_HIP,
+    RegionBatch, compute_critical_region, active_sets_to_regions, pad_active_sets,
 )

@@ -7,7 +7,7 @@ import subprocess
 
 _DIR = os.path.dirname(os.path.abspath(__file__))
 SOURCES = [os.path.join(_DIR, "csrc", "mpqp_hip.hip")]
-DEPS = [os.path.join(_DIR, "csrc", f) for f in ("lp_engine.hpp", "kernels.hpp", "opt_kernel.hpp")] + [
+DEPS = [os.path.join(_DIR, "csrc", f) for f in ("lp_engine.hpp", "kernels.hpp", "opt_kernel.hpp", "regions.hpp")] + [
     os.path.join(os.path.dirname(_DIR), "include", "mpqp_hip.h")]
 OUT = os.path.join(_DIR, "libmpqp_hip.so")
 

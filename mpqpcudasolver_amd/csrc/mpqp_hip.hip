@@ -39,9 +39,11 @@
 
 #include "../../include/mpqp_hip.h"
 #include "kernels.hpp"
+#include "regions.hpp"
 
 static_assert(sizeof(DevRecord) == sizeof(MpqpRecord), "record layout");
 static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
+static_assert(sizeof(DevRegion) == sizeof(MpqpRegionInfo), "region info layout");
 
 // Device memory of mpqp_noalloc(): every call on a GPU uses that GPU's copy
 // of this array (it holds the last call's contents; overlapping calls on
@@ -147,6 +149,7 @@ struct BufView {
 enum Kind {
     K_A1, K_XB1, K_A2, K_XB2, K_AMAT, K_GW, K_G, K_SPI, K_SPD, K_ROOTG, K_ROOT, K_ROOTI, K_CTR, K_GT64, K_GT32,
     K_SUM, K_COLQ,
+    R_IN, R_SETS, R_OUT, R_INFO,
     L_SETS, L_PAR, L_P1PAR, L_REC, L_FLAG, L_POS, L_LIST, L_FBL, L_CNT, L_OFF, L_FS, L_OS, L_FB, L_B1, L_B2,
     L_FIDX, L_FROWS, L_LICQ, L_SCAN,
     K_N
@@ -262,6 +265,7 @@ struct Ctx {
     long long lds_limit = 0;  // bytes of LDS one workgroup may allocate
     int n_cu = 0;
     MpqpStats stats{};
+    MpqpRegionStats rstats{};
     // autotuner: (M1, K1, M2, K2, n_c, FP32) -> feasibility and optimality
     // threads per workgroup and, for a split tableau, the workgroups per CU
     // its LDS share is sized for (FP32, FP64); the sizes of the last call
@@ -1343,6 +1347,157 @@ Output run_call(const Input &in, Arena &A, bool noalloc) {
     return r.out;
 }
 
+// mpqp_regions(): per problem Y = Q^-1 [A^T | H | c] and GX = A Y on the
+// FP64 matrix cores, then one region_kernel workgroup per set, the sets in
+// chunks whose outputs fit a fixed share of device memory.
+struct RegionIn {
+    const double *A, *b, *c, *H, *iQ, *F, *CRA, *CRb;
+    int n_c, n_x, n_t, n_cp;
+    const int *sets;
+    int n_sets, width;
+};
+struct RegionOut {
+    MpqpRegionInfo *info;
+    double *center, *par_A, *par_b, *lam_A, *lam_b, *cr_A, *cr_b;
+};
+
+int run_regions(const RegionIn &in, const RegionOut &out) {
+    Ctx &C = ctx();
+    const int n_c = in.n_c, n_x = in.n_x, p = in.n_t, n_cp = in.n_cp, width = in.width;
+    if (n_c <= 0 || n_x <= 0 || p <= 0 || n_cp < 0 || width <= 0 || in.n_sets < 0)
+        throw HipError{"mpqp_regions: bad dimensions"};
+    if (!in.A || !in.b || !in.c || !in.H || !in.iQ || !in.F || (n_cp > 0 && (!in.CRA || !in.CRb)) ||
+        (in.n_sets > 0 && !in.sets))
+        throw HipError{"mpqp_regions: null input"};
+    // sets: ascending indices in [0, n_c), then -1 padding
+    int kw = 1;
+    for (long long s = 0; s < in.n_sets; ++s) {
+        const int *S = in.sets + s * width;
+        int k = 0;
+        while (k < width && S[k] >= 0) {
+            if (S[k] >= n_c || (k > 0 && S[k] <= S[k - 1]))
+                throw HipError{"mpqp_regions: set " + std::to_string(s) + " is not ascending in [0, n_c)"};
+            ++k;
+        }
+        for (int t = k; t < width; ++t)
+            if (S[t] != -1) throw HipError{"mpqp_regions: set " + std::to_string(s) + " is not -1 padded"};
+        kw = std::max(kw, k);
+    }
+    for (int q = 0; q < n_cp; ++q) {
+        bool nz = false;
+        for (int t = 0; t < p; ++t) nz |= in.CRA[(long long)q * p + t] != 0.0;
+        if (!nz) throw HipError{"mpqp_regions: row " + std::to_string(q) + " of CRA is zero"};
+    }
+    ensure_ready();
+    const long long lds = region_lds_bytes(n_c, p, n_cp, kw);
+    if (lds > C.lds_limit)
+        throw HipError{"mpqp_regions: a region needs " + std::to_string(lds) + " bytes of LDS, more than one workgroup's " +
+                       std::to_string(C.lds_limit)};
+    hipStream_t st = C.stream;
+    MpqpRegionStats rs{};
+    const int m0 = n_c + n_cp, ldy = n_c + p + 1;
+    // inputs: A, inv_Q, W = [A^T | H | c], Y, GX, b, F, CRA, CRb
+    std::vector<double> W((size_t)n_x * ldy);
+    for (int i = 0; i < n_x; ++i) {
+        for (int j = 0; j < n_c; ++j) W[(size_t)i * ldy + j] = in.A[(size_t)j * n_x + i];
+        for (int t = 0; t < p; ++t) W[(size_t)i * ldy + n_c + t] = in.H[(size_t)i * p + t];
+        W[(size_t)i * ldy + n_c + p] = in.c[i];
+    }
+    const size_t nA = (size_t)n_c * n_x, nQ = (size_t)n_x * n_x, nW = (size_t)n_x * ldy, nGX = (size_t)n_c * ldy,
+                 nF = (size_t)n_c * p, nCA = (size_t)n_cp * p;
+    const size_t in_doubles = nA + nQ + 2 * nW + nGX + n_c + nF + nCA + n_cp;
+    BufView din = C.pools.get(R_IN, -1, in_doubles * 8);
+    double *dA = din.as<double>(), *dQ = dA + nA, *dW = dQ + nQ, *dY = dW + nW, *dGX = dY + nW, *db = dGX + nGX,
+           *dF = db + n_c, *dCA = dF + nF, *dCb = dCA + nCA;
+    HIP_OK(hipMemcpyAsync(dA, in.A, nA * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dQ, in.iQ, nQ * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dW, W.data(), nW * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(db, in.b, (size_t)n_c * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dF, in.F, nF * 8, hipMemcpyHostToDevice, st));
+    if (n_cp) {
+        HIP_OK(hipMemcpyAsync(dCA, in.CRA, nCA * 8, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(dCb, in.CRb, (size_t)n_cp * 8, hipMemcpyHostToDevice, st));
+    }
+    HIP_OK(hipEventRecord(C.ev[0], st));
+    const unsigned ty = (unsigned)(((n_x + 15) / 16) * ((ldy + 15) / 16));
+    hipLaunchKernelGGL(gemm_f64_mfma, dim3(ty), dim3(64), 0, st, dQ, n_x, dW, ldy, dY, ldy, n_x, ldy, n_x);
+    const unsigned tg = (unsigned)(((n_c + 15) / 16) * ((ldy + 15) / 16));
+    hipLaunchKernelGGL(gemm_f64_mfma, dim3(tg), dim3(64), 0, st, dA, n_x, dY, ldy, dGX, ldy, n_c, ldy, n_x);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(C.ev[1], st));
+    HIP_OK(hipEventSynchronize(C.ev[1]));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, C.ev[0], C.ev[1]));
+    rs.ms_prep = ms;
+    // outputs per set, in one buffer: par_A, par_b, lam_A, lam_b, cr_A, cr_b, center
+    const size_t o_pa = (size_t)n_x * p, o_pb = n_x, o_la = (size_t)width * p, o_lb = width, o_ca = (size_t)m0 * p,
+                 o_cb = m0, o_ce = p;
+    const size_t per_set = o_pa + o_pb + o_la + o_lb + o_ca + o_cb + o_ce;
+    const long long chunk_cap = std::max<long long>(1, std::min<long long>(1 << 16, ((long long)256 << 20) /
+                                                                                       (long long)(per_set * 8 + 64)));
+    const int max_iter = 20 * (m0 + 2 * p + 2) + 1000;
+    for (long long s0 = 0; s0 < in.n_sets; s0 += chunk_cap) {
+        const int n = (int)std::min<long long>(chunk_cap, in.n_sets - s0);
+        BufView dsets = C.pools.get(R_SETS, -1, (size_t)n * width * 4);
+        BufView dout = C.pools.get(R_OUT, -1, (size_t)n * per_set * 8);
+        BufView dinfo = C.pools.get(R_INFO, -1, (size_t)n * sizeof(DevRegion));
+        HIP_OK(hipMemcpyAsync(dsets.p, in.sets + s0 * width, (size_t)n * width * 4, hipMemcpyHostToDevice, st));
+        RegionArgs ra{};
+        ra.n_c = n_c;
+        ra.n_x = n_x;
+        ra.n_t = p;
+        ra.n_cp = n_cp;
+        ra.kw = kw;
+        ra.max_iter = max_iter;
+        ra.b = db;
+        ra.F = dF;
+        ra.CRA = dCA;
+        ra.CRb = dCb;
+        ra.Y = dY;
+        ra.ldy = ldy;
+        ra.GX = dGX;
+        ra.sets = dsets.as<const int>();
+        ra.width = width;
+        double *o = dout.as<double>();
+        ra.par_A = o;
+        ra.par_b = ra.par_A + (size_t)n * o_pa;
+        ra.lam_A = ra.par_b + (size_t)n * o_pb;
+        ra.lam_b = ra.lam_A + (size_t)n * o_la;
+        ra.cr_A = ra.lam_b + (size_t)n * o_lb;
+        ra.cr_b = ra.cr_A + (size_t)n * o_ca;
+        ra.center = ra.cr_b + (size_t)n * o_cb;
+        ra.info = dinfo.as<DevRegion>();
+        HIP_OK(hipEventRecord(C.ev[2], st));
+        hipLaunchKernelGGL(region_kernel<256>, dim3((unsigned)n), dim3(256), (size_t)lds, st, ra);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(C.ev[3], st));
+        std::vector<DevRegion> info(n);
+        HIP_OK(hipMemcpyAsync(info.data(), ra.info, (size_t)n * sizeof(DevRegion), hipMemcpyDeviceToHost, st));
+        auto back = [&](double *dst, const double *src, size_t per) {
+            if (dst) HIP_OK(hipMemcpyAsync(dst + (size_t)s0 * per, src, (size_t)n * per * 8, hipMemcpyDeviceToHost, st));
+        };
+        back(out.par_A, ra.par_A, o_pa);
+        back(out.par_b, ra.par_b, o_pb);
+        back(out.lam_A, ra.lam_A, o_la);
+        back(out.lam_b, ra.lam_b, o_lb);
+        back(out.cr_A, ra.cr_A, o_ca);
+        back(out.cr_b, ra.cr_b, o_cb);
+        back(out.center, ra.center, o_ce);
+        HIP_OK(hipStreamSynchronize(st));
+        HIP_OK(hipEventElapsedTime(&ms, C.ev[2], C.ev[3]));
+        rs.ms_sets += ms;
+        rs.launches += 1;
+        for (int i = 0; i < n; ++i) {
+            rs.found += info[i].status == MPQP_REGION_FOUND;
+            rs.pivots += info[i].iters;
+        }
+        if (out.info) std::memcpy(out.info + s0, info.data(), (size_t)n * sizeof(DevRegion));
+    }
+    rs.sets = in.n_sets;
+    C.rstats = rs;
+    return (int)rs.found;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1508,6 +1663,35 @@ double mpqp_measure_lds_bandwidth(void) {
         ctx().err = e.msg;
         return -1.0;
     }
+}
+
+int mpqp_regions(const double *A, const double *b, const double *c, const double *H, const double *inv_Q,
+                 const double *F, const double *CRA, const double *CRb, int n_c, int n_x, int n_t, int n_cp,
+                 const int *active_sets, int n_sets, int width, MpqpRegionInfo *info, double *center,
+                 double *parameter_A, double *parameter_b, double *lagrange_A, double *lagrange_b, double *cr_A,
+                 double *cr_b) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx &C = ctx();
+    C.err.clear();
+    try {
+        if (!C.ready) throw HipError{"mpqp_regions: call warm_up() first"};
+        const RegionIn in{A, b, c, H, inv_Q, F, CRA, CRb, n_c, n_x, n_t, n_cp, active_sets, n_sets, width};
+        const RegionOut out{info, center, parameter_A, parameter_b, lagrange_A, lagrange_b, cr_A, cr_b};
+        return run_regions(in, out);
+    } catch (const HipError &e) {
+        C.err = e.msg;
+        return -1;
+    } catch (const std::exception &e) {
+        C.err = e.what();
+        return -1;
+    }
+}
+
+int mpqp_get_region_stats(MpqpRegionStats *stats) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!stats) return -1;
+    *stats = ctx().rstats;
+    return 0;
 }
 
 int mpqp_last_error(char *buf, int len) {

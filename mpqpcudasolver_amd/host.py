@@ -33,6 +33,13 @@ __all__ = [
     "decode_active_sets",
     "calculate_mpqp_with_HIP",
     "STATUS_NAMES",
+    "MpqpRegionInfo",
+    "MpqpRegionStats",
+    "RegionBatch",
+    "REGION_STATUS_NAMES",
+    "pad_active_sets",
+    "compute_critical_region",
+    "active_sets_to_regions",
 ]
 
 _DIR = os.path.dirname(os.path.abspath(__file__))
@@ -40,6 +47,8 @@ _DIR = os.path.dirname(os.path.abspath(__file__))
 MPQP_FP64, MPQP_MIXED = 0, 1
 MPQP_PLAIN, MPQP_PERSISTENT = 0, 1
 STATUS_NAMES = {0: "not_run", 1: "optimal", 2: "unbounded", 3: "iter_cap", 4: "stall", 5: "singular"}
+REGION_EMPTY, REGION_FOUND, REGION_UNBOUNDED, REGION_ITER_LIMIT, REGION_SINGULAR = 0, 1, 2, 3, 4
+REGION_STATUS_NAMES = {0: "empty", 1: "found", 2: "unbounded", 3: "iter_limit", 4: "singular"}
 
 
 class GPUResults(ctypes.Structure):
@@ -144,6 +153,36 @@ class MpqpStats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class MpqpRegionInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("n_rows", ctypes.c_int),
+        ("radius", ctypes.c_double),
+        ("iters", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+    ]
+
+
+REGION_INFO_DTYPE = np.dtype([("status", np.int32), ("n_rows", np.int32), ("radius", np.float64),
+                              ("iters", np.int32), ("reserved", np.int32)])
+assert REGION_INFO_DTYPE.itemsize == ctypes.sizeof(MpqpRegionInfo)
+
+
+class MpqpRegionStats(ctypes.Structure):
+    _fields_ = [
+        ("sets", ctypes.c_longlong),
+        ("found", ctypes.c_longlong),
+        ("pivots", ctypes.c_longlong),
+        ("ms_prep", ctypes.c_double),
+        ("ms_sets", ctypes.c_double),
+        ("launches", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 RECORD_DTYPE = np.dtype([
     ("level", np.int32), ("parent", np.int32),
     ("feas_status", np.int32), ("feas_iters", np.int32), ("feasible", np.int32),
@@ -209,6 +248,61 @@ class MpqpRun:
     p2_basis: np.ndarray | None = None
 
 
+@dataclass
+class RegionBatch:
+    """Outputs of mpqp_regions() for n sets (m0 = n_c + n_cp half-space rows)."""
+
+    status: np.ndarray       # n, REGION_STATUS_NAMES
+    n_rows: np.ndarray       # n, half-spaces kept
+    radius: np.ndarray       # n, Chebyshev radius
+    iters: np.ndarray        # n, simplex pivots
+    center: np.ndarray       # n x n_t
+    parameter_A: np.ndarray  # n x n_x x n_t
+    parameter_b: np.ndarray  # n x n_x
+    lagrange_A: np.ndarray   # n x width x n_t
+    lagrange_b: np.ndarray   # n x width
+    cr_A: np.ndarray         # n x m0 x n_t
+    cr_b: np.ndarray         # n x m0
+    sets: np.ndarray         # n x width, -1 padded
+
+    def region(self, i: int) -> dict | None:
+        """Set i as compute_critical_region returns it (utils.py:93-101):
+        None unless a region was found; the Chebyshev radius and centre are
+        added under their own keys."""
+        if self.status[i] != REGION_FOUND:
+            return None
+        k = int((self.sets[i] >= 0).sum())
+        m = int(self.n_rows[i])
+        return {
+            "parameter_A": self.parameter_A[i],
+            "parameter_b": self.parameter_b[i].reshape(-1, 1),
+            "lagrange_A": self.lagrange_A[i, :k],
+            "lagrange_b": self.lagrange_b[i, :k].reshape(-1, 1),
+            "CR_As": self.cr_A[i, :m],
+            "CR_bs": self.cr_b[i, :m].reshape(-1, 1),
+            "active_set": [int(v) for v in self.sets[i, :k]],
+            "chebyshev_radius": float(self.radius[i]),
+            "chebyshev_center": self.center[i].copy(),
+        }
+
+
+def pad_active_sets(active_sets, width: int) -> np.ndarray:
+    """List of sets (or an already padded 2-D array) -> n x width int32 rows,
+    each ascending and padded with -1 (the layout of GPUResults.active_sets)."""
+    if isinstance(active_sets, np.ndarray) and active_sets.ndim == 2:
+        out = np.full((active_sets.shape[0], width), -1, dtype=np.intc)
+        w = min(width, active_sets.shape[1])
+        out[:, :w] = active_sets[:, :w]
+        return out
+    out = np.full((len(active_sets), width), -1, dtype=np.intc)
+    for i, S in enumerate(active_sets):
+        S = sorted(int(v) for v in S)
+        if len(S) > width:
+            raise ValueError(f"set {i} has {len(S)} > {width} entries")
+        out[i, :len(S)] = S
+    return out
+
+
 def decode_active_sets(ptr, n_sets: int, num_x: int) -> list:
     """utils.py:109-115: rows of num_x ints, each ended by the first -1."""
     if n_sets <= 0:
@@ -260,6 +354,13 @@ class MpqpLib:
         if hasattr(L, "mpqp_get_tuning"):
             L.mpqp_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
             L.mpqp_get_tuning.restype = ctypes.c_int
+        self.has_regions = hasattr(L, "mpqp_regions")
+        if self.has_regions:
+            L.mpqp_regions.argtypes = [_dp] * 8 + [ctypes.c_int] * 4 + [_ip, ctypes.c_int, ctypes.c_int,
+                                                                         ctypes.POINTER(MpqpRegionInfo)] + [_dp] * 7
+            L.mpqp_regions.restype = ctypes.c_int
+            L.mpqp_get_region_stats.argtypes = [ctypes.POINTER(MpqpRegionStats)]
+            L.mpqp_get_region_stats.restype = ctypes.c_int
 
     @property
     def backend(self) -> str:
@@ -279,6 +380,43 @@ class MpqpLib:
         st = MpqpStats()
         if not self.has_stats or self.lib.mpqp_get_stats(ctypes.byref(st)) != 0:
             raise RuntimeError("mpqp_get_stats unavailable")
+        return st.as_dict()
+
+    def regions(self, prog: MPQPProb, active_sets, width: int | None = None, outputs: bool = True) -> RegionBatch:
+        """mpqp_regions() over `active_sets` (a list of index lists or an
+        n x width -1 padded array) of problem `prog` (its A, b, c, H, inv_Q,
+        F, A_t, b_t, as compute_critical_region reads them, utils.py:129).
+        outputs=False copies back only the per-set info."""
+        if not self.has_regions:
+            raise RuntimeError("library has no mpqp_regions")
+        n_c, n_x = prog.A.shape
+        n_cp, n_t = prog.A_t.shape
+        width = width or n_x
+        sets = pad_active_sets(active_sets, width)
+        n = sets.shape[0]
+        m0 = n_c + n_cp
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        A, b, c, H, iQ, F = f64(prog.A), f64(prog.b).ravel(), f64(prog.c).ravel(), f64(prog.H), f64(prog.inv_Q), f64(prog.F)
+        CRA, CRb = f64(prog.A_t), f64(prog.b_t).ravel()
+        info = np.zeros(n, dtype=REGION_INFO_DTYPE)
+        z = lambda *shape: np.zeros(shape, dtype=np.float64) if outputs else None
+        center, pA, pb = z(n, n_t), z(n, n_x, n_t), z(n, n_x)
+        lA, lb, cA, cb = z(n, width, n_t), z(n, width), z(n, m0, n_t), z(n, m0)
+        opt = lambda a: _ptr(a, _dp) if a is not None else None
+        rc = self.lib.mpqp_regions(
+            _ptr(A, _dp), _ptr(b, _dp), _ptr(c, _dp), _ptr(H, _dp), _ptr(iQ, _dp), _ptr(F, _dp),
+            _ptr(CRA, _dp), _ptr(CRb, _dp), n_c, n_x, n_t, n_cp, _ptr(sets, _ip), n, width,
+            info.ctypes.data_as(ctypes.POINTER(MpqpRegionInfo)), opt(center), opt(pA), opt(pb), opt(lA), opt(lb),
+            opt(cA), opt(cb))
+        if rc < 0:
+            raise RuntimeError(f"mpqp_regions failed: {self.last_error()}")
+        return RegionBatch(status=info["status"].copy(), n_rows=info["n_rows"].copy(), radius=info["radius"].copy(),
+                           iters=info["iters"].copy(), center=center, parameter_A=pA, parameter_b=pb,
+                           lagrange_A=lA, lagrange_b=lb, cr_A=cA, cr_b=cb, sets=sets)
+
+    def region_stats(self) -> dict:
+        st = MpqpRegionStats()
+        self.lib.mpqp_get_region_stats(ctypes.byref(st))
         return st.as_dict()
 
     def measure_lds_bandwidth(self) -> float:
@@ -395,12 +533,42 @@ def load_library(path: str | None = None) -> MpqpLib:
     return MpqpLib(path or library_path())
 
 
+def compute_critical_region(A, b, c, H, inv_Q, F, CRA, CRb, active_set, lib: MpqpLib | None = None):
+    """utils.py:23-97 for one set, on the device: the region dict or None."""
+    lib = lib or load_library()
+    prog = MPQPProb.__new__(MPQPProb)
+    prog.A, prog.b, prog.c, prog.H, prog.inv_Q, prog.F = A, b, c, H, inv_Q, F
+    prog.A_t, prog.b_t = CRA, CRb
+    return lib.regions(prog, [list(active_set)], width=max(len(active_set), 1)).region(0)
+
+
+def active_sets_to_regions(prog: MPQPProb, all_active_sets, lib: MpqpLib | None = None) -> list:
+    """utils.py:128-144 on the device: decodes prog.n_active_sets rows of
+    prog.num_x ints (a flat sequence, as GPUResults.active_sets) or takes a
+    list of sets, and returns the regions found, in set order."""
+    lib = lib or load_library()
+    if isinstance(all_active_sets, (list, tuple)) and (not all_active_sets or not np.isscalar(all_active_sets[0])):
+        sets = pad_active_sets(all_active_sets, prog.num_x)
+    else:
+        n = prog.n_active_sets
+        flat = np.ctypeslib.as_array(all_active_sets, shape=(n * prog.num_x,)) if not hasattr(
+            all_active_sets, "__len__") else np.asarray(all_active_sets[: n * prog.num_x])
+        sets = np.asarray(flat, dtype=np.intc).reshape(n, prog.num_x)
+        # rows end at their first -1 (utils.py:133-138)
+        ended = np.cumsum(sets == -1, axis=1) > 0
+        sets = np.where(ended, -1, sets).astype(np.intc)
+    batch = lib.regions(prog, sets, width=prog.num_x)
+    return [r for r in (batch.region(i) for i in range(sets.shape[0])) if r is not None]
+
+
 def calculate_mpqp_with_HIP(problems, k_max: int | None = None, lib: MpqpLib | None = None,
-                            reserved_memory: int = 0):
-    """Counterpart of calculate_mpqp_with_CUDA (main.py:11-173) up to the
-    GPU call: assembles each problem's LPs, calls mpqp() and decodes the
-    final active sets.  Returns one (active_sets, MpqpRun) per problem.
-    Critical-region reconstruction (utils.py:23-144, Gurobi) is out of scope."""
+                            reserved_memory: int = 0, regions: bool = False):
+    """Counterpart of calculate_mpqp_with_CUDA (main.py:11-173): assembles
+    each problem's LPs, calls mpqp(), decodes the final active sets and, with
+    regions=True, builds their critical regions on the device
+    (active_sets_to_regions, main.py:154-157).  Returns one
+    (active_sets, MpqpRun) per problem, or (active_sets, MpqpRun, regions)
+    with regions=True."""
     lib = lib or load_library()
     lib.warm_up()
     out = []
@@ -409,5 +577,8 @@ def calculate_mpqp_with_HIP(problems, k_max: int | None = None, lib: MpqpLib | N
         km = prog.num_x if k_max is None else k_max
         run = lib.run(lp, k_max=km, k_min=0, reserved_memory=reserved_memory)
         prog.n_active_sets = run.n_final_lps
-        out.append((run.active_sets, run))
+        if regions:
+            out.append((run.active_sets, run, active_sets_to_regions(prog, run.active_sets, lib)))
+        else:
+            out.append((run.active_sets, run))
     return out

@@ -123,6 +123,9 @@ static void set_err(const char *msg) {
     snprintf(g_err, sizeof g_err, "%s", msg);
 }
 
+/* error setter shared with regions.c */
+void oracle_set_error(const char *msg) { set_err(msg); }
+
 void mpqp_default_options(MpqpOptions *o) {
     memset(o, 0, sizeof *o);
     o->precision = MPQP_FP64;

@@ -134,3 +134,48 @@ def brute_force_min(Aeq: np.ndarray, beq: np.ndarray, c: np.ndarray, tol: float 
             if best is None or val < best:
                 best = val
     return best
+
+
+def critical_region_np(prog, S):
+    """The critical region of S as utils.py:23-61 defines it, restated with
+    numpy's pinv and the original (x-space) products: returns
+    (LA, Lb, XA, Xb, CR_As, CR_bs)."""
+    A, b, c, H, F = prog.A, prog.b.reshape(-1, 1), prog.c.reshape(-1, 1), prog.H, prog.F
+    iQ = np.linalg.pinv(prog.Q)
+    S = list(S)
+    m = A.shape[0]
+    I = [i for i in range(m) if i not in set(S)]
+    aux = A[S] @ iQ
+    G_inv = np.linalg.pinv(aux @ A[S].T)
+    LA = -G_inv @ (aux @ H + F[S])
+    Lb = -G_inv @ (b[S] + aux @ c)
+    XA = -iQ @ A[S].T @ LA - iQ @ H
+    Xb = -iQ @ A[S].T @ Lb - iQ @ c
+    # lambda >= 0 and inactive rows stay inactive, then the parameter box
+    rows_A = [-LA, A[I] @ XA - F[I]]
+    rows_b = [Lb, b[I] - A[I] @ Xb]
+    keep = [np.any(Ra != 0.0, axis=1) for Ra in rows_A]
+    CA = np.vstack([rows_A[0][keep[0]], rows_A[1][keep[1]], prog.A_t])
+    Cb = np.vstack([rows_b[0][keep[0]], rows_b[1][keep[1]], prog.b_t.reshape(-1, 1)])
+    nrm = np.linalg.norm(CA, axis=1, keepdims=True)
+    return LA, Lb, XA, Xb, CA / nrm, Cb / nrm
+
+
+def chebyshev_highs(CA, Cb):
+    """The ball LP of utils.py:63-91 with HiGHS: max r s.t. CA theta + r ||CA_i||
+    <= Cb, r >= 0 (theta free).  Returns (radius, theta) or (None, None) when
+    the reference keeps no region (LP not optimal, or r < 1e-8); the raw
+    optimum is returned third (None when the LP is not optimal)."""
+    p = CA.shape[1]
+    nr = np.linalg.norm(CA, axis=1)
+    Aub = np.hstack([CA, nr[:, None]])
+    cobj = np.zeros(p + 1)
+    cobj[-1] = -1.0
+    bounds = [(None, None)] * p + [(0, None)]
+    r = linprog(cobj, A_ub=Aub, b_ub=Cb.ravel(), bounds=bounds, method="highs")
+    if r.status != 0:
+        return None, None, None
+    rad = -r.fun
+    if rad < 1e-8:
+        return None, None, rad
+    return rad, r.x[:p], rad
