@@ -30,6 +30,13 @@ bound is the LDS bandwidth measured on the device when the tableau lives in
 LDS, and HBM (8 TB/s) when it does not fit there and lives in global memory.
 cpu_baseline is the CPU restatement of the same algorithm (FP64, OpenMP over
 the candidates of a level) on a bounded sample of the workload, best of 3.
+
+"regions" times the next stage of the pipeline on the headline problem's
+final sets: mpqp_regions() (critical region and Chebyshev radius of every
+set), in sets/s of device time, beside the CPU restatement on the same sets.
+"pop" runs the reference's own end-to-end benchmark, the POP_mpQP1 problem
+lists of main.py:218-220, and sets its cumulative time beside the RTX 4090
+times the reference published.
 """
 from __future__ import annotations
 
@@ -93,6 +100,95 @@ def cpu_baseline(lp, k_sample: int, repeats: int = 3) -> dict:
             "cpu": cpu_model(),
             "sample": f"levels |S|<={k_sample} of the same problem, FP64: {n} checks in {best:.2f} s "
                       f"(best of {repeats})"}
+
+
+def regions_bench(lib, cfg: int, steps: int) -> tuple[dict, object, list]:
+    """mpqp_regions() over the final sets of the workload's (first) problem:
+    the critical region and Chebyshev radius of every set (utils.py:23-144).
+    Rate = sets per second of device time (the two per-problem products plus
+    the per-set kernel, HIP events), best of `steps`; the wall rate includes
+    the host-array copies across the C ABI."""
+    prog, k_max = config_problem(cfg)
+    prog = prog[0] if isinstance(prog, list) else prog
+    lib.set_options()
+    sets = lib.run(assemble_lps(prog), k_max=k_max).active_sets
+    best, wall = None, None
+    for _ in range(max(steps, 1) + 1):
+        t0 = time.perf_counter()
+        batch = lib.regions(prog, sets)
+        dt = time.perf_counter() - t0
+        rs = lib.region_stats()
+        ms = rs["ms_prep"] + rs["ms_sets"]
+        best = ms if best is None else min(best, ms)
+        wall = dt if wall is None else min(wall, dt)
+    n = len(sets)
+    return ({"sets": n, "regions_found": int(rs["found"]), "value": n / (best * 1e-3) if best else None,
+             "unit": "sets/s", "device_ms": best, "wall_ms": wall * 1e3, "wall_value": n / wall,
+             "pivots": int(rs["pivots"]), "workload": f"final sets of {_workload(cfg)}"}, prog, sets)
+
+
+def cpu_regions(prog, sets, repeats: int = 3) -> dict:
+    """The oracle's critical-region restatement (OpenMP over the sets) on the
+    same sets, best of `repeats`."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleLib  # test infrastructure: the CPU baseline leg
+
+    orc = OracleLib(os.path.join(ROOT, "oracle", "_build", "libmpqp_oracle.so"))
+    best = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        orc.regions(prog, sets)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": len(sets) / best, "unit": "sets/s", "cores": threads, "kind": "port",
+            "sample": f"all {len(sets)} final sets, FP64, best of {repeats}: {best * 1e3:.1f} ms"}
+
+
+POP_NPZ = os.path.join(ROOT, "tests", "golden", "pop_mpqp1.npz")
+# cumulative end-to-end solve time of the reference on an RTX 4090, read off
+# its published plots (algorithms_performance_len43.png, _len62.png; +-10%)
+POP_REFERENCE_S = {43: 3.6, 62: 23.0}
+
+
+def pop_bench(lib) -> tuple[dict, list]:
+    """The POP_mpQP1 driver (main.py:11-173) on the reference's 43- and
+    62-problem lists: cumulative total_time (mpqp() with k_max = num_x, the
+    second preprocessing and the critical regions, main.py:143-159), first
+    pass and second pass, beside the reference's published RTX 4090 figures."""
+    from mpqpcudasolver_amd.pop import REFERENCE_TEST_IDS, REFERENCE_TEST_IDS_62, calculate_mpqp_with_HIP, load_pop_npz
+
+    ds = load_pop_npz(POP_NPZ)
+    lib.set_options()
+    out = {"workload": "POP_mpQP1 end to end (preprocessing, mpqp with k_max=num_x, critical regions), "
+                       "default options (FP64, plain launches)"}
+    for ids in (REFERENCE_TEST_IDS, REFERENCE_TEST_IDS_62):
+        n = len(ids)
+        first = calculate_mpqp_with_HIP(ds, ids, lib=lib)
+        second = calculate_mpqp_with_HIP(ds, ids, lib=lib)
+        out[f"{n}"] = {"cumulative_s_first_pass": float(first[:, 2].sum()),
+                       "cumulative_s": float(second[:, 2].sum()),
+                       "gpu_call_s": float(second[:, 3].sum()),
+                       "regions": int(second[:, 1].sum()),
+                       "reference_rtx4090_s": POP_REFERENCE_S[n],
+                       "speedup_vs_reference": POP_REFERENCE_S[n] / float(second[:, 2].sum())}
+        print(f"# POP {n}: {out[str(n)]['cumulative_s']:.3f} s cumulative "
+              f"(reference RTX 4090 ~{POP_REFERENCE_S[n]} s)", file=sys.stderr, flush=True)
+    return out, ds
+
+
+def cpu_pop(ds, ids) -> dict:
+    """The oracle through the same driver on the 43-problem list."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleLib  # test infrastructure: the CPU baseline leg
+    from mpqpcudasolver_amd.pop import calculate_mpqp_with_HIP
+
+    orc = OracleLib(os.path.join(ROOT, "oracle", "_build", "libmpqp_oracle.so"))
+    orc.set_options(record=0)
+    res = calculate_mpqp_with_HIP(ds, ids, lib=orc)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": float(res[:, 2].sum()), "unit": "s cumulative", "cores": threads, "kind": "port",
+            "sample": f"the {len(ids)}-problem list, FP64, one pass"}
 
 
 def timed(lib, lps, k_max, steps, warmup, **opts):
@@ -346,8 +442,22 @@ def main() -> int:
             print(f"# config {cfg}: {extras[str(cfg)]['value']:.4g} checks/s, "
                   f"{extras[str(cfg)]['speedup']:.3f}x FP64 plain", file=sys.stderr, flush=True)
         out["configs"] = extras
+    reg = None
+    if not args.no_extras:
+        out["regions"], reg_prog, reg_sets = regions_bench(lib, args.config, args.steps)
+        reg = (reg_prog, reg_sets)
+        print(f"# regions: {out['regions']['value']:.4g} sets/s on the device", file=sys.stderr, flush=True)
+    pop_ds = None
+    if not args.no_extras:
+        out["pop"], pop_ds = pop_bench(lib)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(h["lps"][0], min(args.cpu_k, h["k_max"]))
+        if reg is not None:
+            out["regions"]["cpu_baseline"] = cpu_regions(*reg)
+        if pop_ds is not None:
+            from mpqpcudasolver_amd.pop import REFERENCE_TEST_IDS
+
+            out["pop"]["cpu_baseline"] = cpu_pop(pop_ds, REFERENCE_TEST_IDS)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -271,6 +271,23 @@ int mpqp_regions(const double *A, const double *b, const double *c, const double
                  double *parameter_b, double *lagrange_A, double *lagrange_b, double *cr_A,
                  double *cr_b);
 
+/* ---- Constraint preprocessing ------------------------------------------
+ * Redundant rows of the polyhedron {z : G z <= h} (G m x nz row-major,
+ * z = (x, theta)), the LP part of find_redundant_constraints (utils.py:
+ * 184-190, ppopt 1.5.1; ppopt is a dependency the reference does not ship).
+ * Row i is redundant when max g_i z over the other kept rows, with row i
+ * relaxed to g_i z <= h_i + 1, is <= h_i (rows scaled to unit norm first,
+ * tolerance 1e-9 max(1, |h_i|)).  Rows are tested in order and a redundant
+ * row is dropped before the next is tested: of two identical rows the first
+ * is dropped (the second implies it) and the second is kept.  All-zero rows
+ * with h >= 0 are dropped.
+ * fixed (may be NULL): rows that are kept and never tested (equalities).
+ * keep[i] = 1 for kept rows.  The LPs run on the device (one wave per LP).
+ * Returns the number of kept rows; -1 on error, including an empty
+ * polyhedron.  *n_lps (may be NULL) receives the number of LPs solved. */
+int mpqp_redundant_rows(const double *G, const double *h, int m, int nz, const int *fixed,
+                        int *keep, long long *n_lps);
+
 /* Work and time of the last mpqp_regions() call (HIP-event intervals). */
 typedef struct MpqpRegionStats {
     long long sets, found, pivots;

@@ -1498,6 +1498,88 @@ int run_regions(const RegionIn &in, const RegionOut &out) {
     return (int)rs.found;
 }
 
+// mpqp_redundant_rows(): rows scaled to unit norm on the host, then one
+// batch of redundancy LPs (every testable row against all the others) and a
+// sequential pass, in row order, over the rows the batch flagged: each is
+// re-tested against the rows still kept and dropped if still redundant.
+// Rows the batch did not flag are not redundant against any subset either
+// (removing rows only enlarges the polyhedron), so the result equals the
+// sequential test of every row in order.
+int run_redundant(const double *G, const double *h, int m, int nz, const int *fixed, int *keep, long long *n_lps) {
+    Ctx &C = ctx();
+    if (m <= 0 || nz <= 0 || !G || !h || !keep) throw HipError{"mpqp_redundant_rows: bad arguments"};
+    ensure_ready();
+    const long long lds = redund_lds_bytes(m, nz);
+    if (lds > C.lds_limit)
+        throw HipError{"mpqp_redundant_rows: an LP needs " + std::to_string(lds) + " bytes of LDS, more than " +
+                       std::to_string(C.lds_limit)};
+    std::vector<double> Gs((size_t)m * nz), hs(m);
+    std::vector<int> active(m, 1), tests;
+    for (int j = 0; j < m; ++j) {
+        double nrm = 0.0;
+        for (int c = 0; c < nz; ++c) nrm += G[(size_t)j * nz + c] * G[(size_t)j * nz + c];
+        nrm = std::sqrt(nrm);
+        if (nrm == 0.0) {
+            if (h[j] < 0.0) throw HipError{"mpqp_redundant_rows: row " + std::to_string(j) + " is 0 <= h < 0"};
+            active[j] = 0;  // 0 <= h_j holds everywhere
+            continue;
+        }
+        for (int c = 0; c < nz; ++c) Gs[(size_t)j * nz + c] = G[(size_t)j * nz + c] / nrm;
+        hs[j] = h[j] / nrm;
+        if (!(fixed && fixed[j])) tests.push_back(j);
+    }
+    hipStream_t st = C.stream;
+    BufView din = C.pools.get(R_IN, -1, ((size_t)m * nz + m) * 8 + (size_t)m * 4);
+    double *dG = din.as<double>(), *dh = dG + (size_t)m * nz;
+    int *dact = reinterpret_cast<int *>(dh + m);
+    HIP_OK(hipMemcpyAsync(dG, Gs.data(), (size_t)m * nz * 8, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dh, hs.data(), (size_t)m * 8, hipMemcpyHostToDevice, st));
+    const int max_iter = 50 * (m + 2 * nz + 1) + 1000;
+    long long lps = 0;
+    // one launch over `rows`; returns whether each is redundant against the active rows
+    auto launch = [&](const std::vector<int> &rows) {
+        const int n = (int)rows.size();
+        std::vector<char> red(n, 0);
+        if (!n) return red;
+        HIP_OK(hipMemcpyAsync(dact, active.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
+        BufView dt = C.pools.get(R_SETS, -1, (size_t)n * 4);
+        BufView dv = C.pools.get(R_OUT, -1, (size_t)n * 8);
+        BufView ds = C.pools.get(R_INFO, -1, (size_t)n * 8);
+        HIP_OK(hipMemcpyAsync(dt.p, rows.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+        RedundArgs ra{m, nz, max_iter, dG, dh, dact, dt.as<int>(), dv.as<double>(), ds.as<int>(), ds.as<int>() + n};
+        hipLaunchKernelGGL(redundancy_kernel<64>, dim3((unsigned)n), dim3(64), (size_t)lds, st, ra);
+        HIP_OK(hipGetLastError());
+        std::vector<double> val(n);
+        std::vector<int> sts(2 * (size_t)n);
+        HIP_OK(hipMemcpyAsync(val.data(), dv.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(sts.data(), ds.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        lps += n;
+        for (int t = 0; t < n; ++t) {
+            if (sts[t] == RED_EMPTY) throw HipError{"mpqp_redundant_rows: the polyhedron is empty"};
+            if (sts[t] != RED_OPTIMAL)
+                throw HipError{"mpqp_redundant_rows: the LP of row " + std::to_string(rows[t]) + " did not finish (status " +
+                               std::to_string(sts[t]) + ")"};
+            const double hi = hs[rows[t]];
+            red[t] = val[t] <= hi + 1e-9 * std::max(1.0, std::fabs(hi));
+        }
+        return red;
+    };
+    const std::vector<char> flag = launch(tests);
+    for (size_t t = 0; t < tests.size(); ++t) {
+        if (!flag[t]) continue;
+        const int i = tests[t];
+        if (launch(std::vector<int>{i})[0]) active[i] = 0;
+    }
+    int kept = 0;
+    for (int j = 0; j < m; ++j) {
+        keep[j] = active[j];
+        kept += active[j];
+    }
+    if (n_lps) *n_lps = lps;
+    return kept;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1678,6 +1760,23 @@ int mpqp_regions(const double *A, const double *b, const double *c, const double
         const RegionIn in{A, b, c, H, inv_Q, F, CRA, CRb, n_c, n_x, n_t, n_cp, active_sets, n_sets, width};
         const RegionOut out{info, center, parameter_A, parameter_b, lagrange_A, lagrange_b, cr_A, cr_b};
         return run_regions(in, out);
+    } catch (const HipError &e) {
+        C.err = e.msg;
+        return -1;
+    } catch (const std::exception &e) {
+        C.err = e.what();
+        return -1;
+    }
+}
+
+int mpqp_redundant_rows(const double *G, const double *h, int m, int nz, const int *fixed, int *keep,
+                        long long *n_lps) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx &C = ctx();
+    C.err.clear();
+    try {
+        if (!C.ready) throw HipError{"mpqp_redundant_rows: call warm_up() first"};
+        return run_redundant(G, h, m, nz, fixed, keep, n_lps);
     } catch (const HipError &e) {
         C.err = e.msg;
         return -1;

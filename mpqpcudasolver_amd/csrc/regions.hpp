@@ -92,10 +92,127 @@ __global__ __launch_bounds__(64) void gemm_f64_mfma(const double *A, int lda, co
     }
 }
 
+// ---------------------------------------------------------------------
+// Dense primal simplex in LDS, shared by the Chebyshev and redundancy LPs:
+// condensed tableau T ((m + 1) x ld), rows [0, m) the constraints, row m the
+// reduced costs of a minimisation, column K the right-hand side;
+// basis[i] / nonb[j] the variable of row i / column j.  The whole workgroup
+// participates in every call.
+// ---------------------------------------------------------------------
+enum { LP_OPTIMAL = 0, LP_UNBOUNDED = 1, LP_ITER_LIMIT = 2 };
+
+template <int NT>
+__device__ void lp_pivot(double *T, int ld, int m, int K, int r, int q, int *basis, int *nonb, double *colq) {
+    pivot_team<NT, double>(T, ld, m + 1, K + 1, r, q, colq);
+    if (threadIdx.x == 0) {
+        const int v = basis[r];
+        basis[r] = nonb[q];
+        nonb[q] = v;
+    }
+    __syncthreads();
+}
+
+// the row of the most negative right-hand side (lowest row on ties), -1 if
+// none is negative
+template <int NT>
+__device__ int lp_most_negative_row(const double *T, int ld, int m, int K) {
+    __shared__ int sh_row;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    if (threadIdx.x < MPQP_WAVE) {
+        double v = 0.0;
+        int key = -1, aux = 0;
+        for (int i = lane; i < m; i += MPQP_WAVE) {
+            const double bi = T[i * ld + K];
+            if (bi < 0.0 && (key < 0 || bi < v)) {
+                v = bi;
+                key = i;
+            }
+        }
+        lex_min(v, key, aux);
+        if (lane == 0) sh_row = key;
+    }
+    __syncthreads();
+    const int r = sh_row;
+    __syncthreads();
+    return r;
+}
+
+// reduced costs of min sum_v cost[v] x_v for the current basis (cost has one
+// entry per variable: K structural / artificial, then one slack per row)
+template <int NT>
+__device__ void lp_set_cost(double *T, int ld, int m, int K, const int *basis, const int *nonb, const double *cost) {
+    for (int j = threadIdx.x; j <= K; j += NT) {
+        double acc = j < K ? cost[nonb[j]] : 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double cb = cost[basis[i]];
+            if (cb != 0.0) acc = fma(-cb, T[i * ld + j], acc);
+        }
+        T[m * ld + j] = acc;
+    }
+    __syncthreads();
+}
+
+// Bland's rule: the lowest-index variable with a reduced cost < -1e-12
+// enters (never `barred`); the leaving row has the minimum ratio over
+// entries > 1e-11, ties to the lowest-index basic variable.  `it` counts
+// pivots across calls.
+template <int NT>
+__device__ int lp_bland(double *T, int ld, int m, int K, int *basis, int *nonb, double *colq, int barred, int max_iter,
+                        int &it) {
+    __shared__ int sh_act, sh_r, sh_q;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    for (; it < max_iter; ++it) {
+        if (threadIdx.x < MPQP_WAVE) {
+            int key = -1, q = -1;
+            double z = 0.0;
+            for (int j = lane; j < K; j += MPQP_WAVE)
+                if (T[m * ld + j] < -1e-12 && nonb[j] != barred && (key < 0 || nonb[j] < key)) {
+                    key = nonb[j];
+                    q = j;
+                }
+            lex_min(z, key, q);
+            int act = 0, r = -1;
+            if (key < 0) {
+                act = 1;
+            } else {
+                double rv = 0.0;
+                int rb = -1, rr = -1;
+                for (int i = lane; i < m; i += MPQP_WAVE) {
+                    const double al = T[i * ld + q];
+                    if (al > 1e-11) {
+                        const double rt = T[i * ld + K] / al;
+                        if (rb < 0 || rt < rv || (rt == rv && basis[i] < rb)) {
+                            rv = rt;
+                            rb = basis[i];
+                            rr = i;
+                        }
+                    }
+                }
+                lex_min(rv, rb, rr);
+                if (rb < 0) act = 2;
+                r = rr;
+            }
+            if (lane == 0) {
+                sh_act = act;
+                sh_r = r;
+                sh_q = q;
+            }
+        }
+        __syncthreads();
+        const int act = sh_act, r = sh_r, q = sh_q;
+        if (act != 0) {
+            __syncthreads();
+            return act == 1 ? LP_OPTIMAL : LP_UNBOUNDED;
+        }
+        lp_pivot<NT>(T, ld, m, K, r, q, basis, nonb, colq);
+    }
+    return LP_ITER_LIMIT;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    __shared__ int sh_k, sh_ok, sh_m, sh_r, sh_q, sh_act;
+    __shared__ int sh_k, sh_ok, sh_m;
     __shared__ double sh_rad;
     constexpr int OPTIMAL = -1;
     const int n_c = a.n_c, n_x = a.n_x, p = a.n_t, n_cp = a.n_cp, kw = a.kw;
@@ -264,82 +381,11 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
     } else if (m == 0) {
         status = MPQP_REGION_UNBOUNDED;
     } else {
-        auto pivot = [&](int r, int q) {
-            pivot_team<NT, double>(T, ld, m + 1, K + 1, r, q, colq);
-            if (tid == 0) {
-                const int v = basis[r];
-                basis[r] = nonb[q];
-                nonb[q] = v;
-            }
-            __syncthreads();
-        };
-        // r- enters at the row of the most negative b_i (lowest row on ties)
-        if (tid < MPQP_WAVE) {
-            double v = 0.0;
-            int key = -1, aux = 0;
-            for (int i = lane; i < m; i += MPQP_WAVE) {
-                const double bi = T[i * ld + K];
-                if (bi < 0.0 && (key < 0 || bi < v)) {
-                    v = bi;
-                    key = i;
-                }
-            }
-            lex_min(v, key, aux);
-            if (lane == 0) sh_r = key;
-        }
-        __syncthreads();
-        if (sh_r >= 0) pivot(sh_r, RM);
-        for (; it < a.max_iter; ++it) {
-            if (tid < MPQP_WAVE) {
-                // entering: the lowest variable index with a negative reduced cost
-                int key = -1, q = -1;
-                double z = 0.0;
-                for (int j = lane; j < K; j += MPQP_WAVE)
-                    if (T[m * ld + j] < -1e-12 && (key < 0 || nonb[j] < key)) {
-                        key = nonb[j];
-                        q = j;
-                    }
-                lex_min(z, key, q);
-                int act = 0, r = -1;
-                if (key < 0) {
-                    act = 1;  // optimal
-                } else {
-                    // leaving: minimum ratio, ties to the lowest basic variable
-                    double rv = 0.0;
-                    int rb = -1, rr = -1;
-                    for (int i = lane; i < m; i += MPQP_WAVE) {
-                        const double al = T[i * ld + q];
-                        if (al > 1e-11) {
-                            const double rt = T[i * ld + K] / al;
-                            if (rb < 0 || rt < rv || (rt == rv && basis[i] < rb)) {
-                                rv = rt;
-                                rb = basis[i];
-                                rr = i;
-                            }
-                        }
-                    }
-                    lex_min(rv, rb, rr);
-                    if (rb < 0) act = 2;  // unbounded
-                    r = rr;
-                }
-                if (lane == 0) {
-                    sh_act = act;
-                    sh_r = r;
-                    sh_q = q;
-                }
-            }
-            __syncthreads();
-            const int act = sh_act, r = sh_r, q = sh_q;
-            if (act == 1) {
-                status = OPTIMAL;
-                break;
-            }
-            if (act == 2) {
-                status = MPQP_REGION_UNBOUNDED;
-                break;
-            }
-            pivot(r, q);
-        }
+        // r- enters at the row of the most negative b_i
+        const int r0 = lp_most_negative_row<NT>(T, ld, m, K);
+        if (r0 >= 0) lp_pivot<NT>(T, ld, m, K, r0, RM, basis, nonb, colq);
+        const int st = lp_bland<NT>(T, ld, m, K, basis, nonb, colq, -1, a.max_iter, it);
+        status = st == LP_OPTIMAL ? OPTIMAL : (st == LP_UNBOUNDED ? MPQP_REGION_UNBOUNDED : MPQP_REGION_ITER_LIMIT);
     }
     double radius = 0.0;
     if (status == OPTIMAL) {
@@ -371,5 +417,120 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
         a.info[c].radius = radius;
         a.info[c].iters = it;
         a.info[c].reserved = 0;
+    }
+}
+
+// ---------------------------------------------------------------------
+// Redundancy LPs of the constraint preprocessing.  Polyhedron
+// P = {z : g_j z <= h_j, j active} (z = (x, theta), rows scaled to unit
+// norm).  Test t of row i: max g_i z over P with row i relaxed to
+// g_i z <= h_i + 1; row i is redundant iff the maximum stays <= h_i.  One
+// wave per test.  z = z+ - z-; an artificial w (-1 in every row) makes the
+// slack basis feasible when some h_j < 0 (phase 1: min w; P is empty if
+// w > 1e-9 at its optimum); phase 2 minimises -g_i z with w barred.
+// ---------------------------------------------------------------------
+enum { RED_OPTIMAL = 0, RED_EMPTY = 1, RED_ITER_LIMIT = 2, RED_UNBOUNDED = 3 };
+
+struct RedundArgs {
+    int m, nz, max_iter;
+    const double *G;    // m x nz, row-major
+    const double *h;    // m
+    const int *active;  // m: row j belongs to P
+    const int *tests;   // rows tested, one per workgroup
+    double *value;      // per test: max g_i z
+    int *status;        // per test: RED_*
+    int *iters;         // per test: pivots
+};
+
+__host__ __device__ inline long long redund_lds_bytes(int m, int nz) {
+    const long long K = 2LL * nz + 1, ld = K + 1;
+    const long long d = (m + 1LL) * ld + (m + 1) + (K + m);
+    const long long i = (long long)m + K + m;
+    return (d * 8 + i * 4 + 15) & ~15LL;
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ int sh_mm, sh_rw;
+    const int m = a.m, nz = a.nz, K = 2 * nz + 1, ld = K + 1, W = 2 * nz;
+    const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1);
+    const int t = blockIdx.x, ti = a.tests[t];
+    double *T = reinterpret_cast<double *>(smem);  // (m + 1) x ld
+    double *colq = T + (m + 1) * ld;               // m + 1
+    double *cost = colq + m + 1;                   // K + m
+    int *rowid = reinterpret_cast<int *>(cost + K + m);
+    int *nonb = rowid + m;
+    int *basis = nonb + K;
+    // the rows of P (and row ti), in order
+    if (tid < MPQP_WAVE) {
+        int base = 0;
+        for (int r0 = 0; r0 < m; r0 += MPQP_WAVE) {
+            const int j = r0 + lane;
+            const bool in = j < m && (a.active[j] != 0 || j == ti);
+            const unsigned long long mk = __ballot(in);
+            if (in) rowid[base + __popcll(mk & ((1ull << lane) - 1ull))] = j;
+            base += __popcll(mk);
+        }
+        if (lane == 0) sh_mm = base;
+    }
+    __syncthreads();
+    const int mm = sh_mm;
+    for (int e = tid; e < mm * ld; e += NT) {
+        const int ii = e / ld, c = e - ii * ld, j = rowid[ii];
+        double v;
+        if (c < nz) v = a.G[(long long)j * nz + c];
+        else if (c < W) v = -a.G[(long long)j * nz + c - nz];
+        else if (c == W) v = -1.0;
+        else v = a.h[j] + (j == ti ? 1.0 : 0.0);
+        T[ii * ld + c] = v;
+    }
+    for (int ii = tid; ii < mm; ii += NT) basis[ii] = K + ii;
+    for (int j = tid; j < K; j += NT) nonb[j] = j;
+    // phase 1 over the slack basis: min w
+    for (int j = tid; j <= K; j += NT) T[mm * ld + j] = j == W ? 1.0 : 0.0;
+    __syncthreads();
+    int it = 0, status = RED_OPTIMAL;
+    const int r0 = lp_most_negative_row<NT>(T, ld, mm, K);
+    if (r0 >= 0) {
+        lp_pivot<NT>(T, ld, mm, K, r0, W, basis, nonb, colq);
+        const int st = lp_bland<NT>(T, ld, mm, K, basis, nonb, colq, -1, a.max_iter, it);
+        if (tid == 0) {
+            int rw = -1;
+            for (int i = 0; i < mm; ++i)
+                if (basis[i] == W) rw = i;
+            sh_rw = rw;
+        }
+        __syncthreads();
+        const int rw = sh_rw;
+        if (st == LP_ITER_LIMIT) status = RED_ITER_LIMIT;
+        else if (rw >= 0 && T[rw * ld + K] > 1e-9) status = RED_EMPTY;
+        else if (rw >= 0) {
+            // w basic at zero: it leaves through any usable column
+            if (tid == 0) {
+                int q = -1;
+                for (int j = 0; j < K && q < 0; ++j)
+                    if (fabs(T[rw * ld + j]) > 1e-9) q = j;
+                sh_rw = q;
+            }
+            __syncthreads();
+            const int q = sh_rw;
+            __syncthreads();
+            if (q >= 0) lp_pivot<NT>(T, ld, mm, K, rw, q, basis, nonb, colq);
+        }
+    }
+    if (status == RED_OPTIMAL) {
+        // phase 2: min -g_i z
+        for (int v = tid; v < K + mm; v += NT)
+            cost[v] = v < nz ? -a.G[(long long)ti * nz + v] : (v < W ? a.G[(long long)ti * nz + v - nz] : 0.0);
+        __syncthreads();
+        lp_set_cost<NT>(T, ld, mm, K, basis, nonb, cost);
+        const int st = lp_bland<NT>(T, ld, mm, K, basis, nonb, colq, W, a.max_iter, it);
+        status = st == LP_OPTIMAL ? RED_OPTIMAL : (st == LP_UNBOUNDED ? RED_UNBOUNDED : RED_ITER_LIMIT);
+    }
+    if (tid == 0) {
+        a.value[t] = status == RED_OPTIMAL ? T[mm * ld + K] : 0.0;
+        a.status[t] = status;
+        a.iters[t] = it;
     }
 }

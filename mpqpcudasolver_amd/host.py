@@ -31,7 +31,6 @@ __all__ = [
     "library_path",
     "load_library",
     "decode_active_sets",
-    "calculate_mpqp_with_HIP",
     "STATUS_NAMES",
     "MpqpRegionInfo",
     "MpqpRegionStats",
@@ -361,6 +360,10 @@ class MpqpLib:
             L.mpqp_regions.restype = ctypes.c_int
             L.mpqp_get_region_stats.argtypes = [ctypes.POINTER(MpqpRegionStats)]
             L.mpqp_get_region_stats.restype = ctypes.c_int
+        if hasattr(L, "mpqp_redundant_rows"):
+            L.mpqp_redundant_rows.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, _ip, _ip,
+                                              ctypes.POINTER(ctypes.c_longlong)]
+            L.mpqp_redundant_rows.restype = ctypes.c_int
 
     @property
     def backend(self) -> str:
@@ -413,6 +416,23 @@ class MpqpLib:
         return RegionBatch(status=info["status"].copy(), n_rows=info["n_rows"].copy(), radius=info["radius"].copy(),
                            iters=info["iters"].copy(), center=center, parameter_A=pA, parameter_b=pb,
                            lagrange_A=lA, lagrange_b=lb, cr_A=cA, cr_b=cb, sets=sets)
+
+    def redundant_rows(self, G, h, fixed=None) -> np.ndarray:
+        """mpqp_redundant_rows(): keep mask (int32) of the rows of G z <= h;
+        `fixed` rows are kept and never tested.  self.redundancy_lps holds
+        the number of LPs solved."""
+        G = np.ascontiguousarray(G, dtype=np.float64)
+        m, nz = G.shape
+        h = np.ascontiguousarray(np.asarray(h, dtype=np.float64).ravel())
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.intc)
+        keep = np.zeros(m, dtype=np.intc)
+        n_lps = ctypes.c_longlong(0)
+        rc = self.lib.mpqp_redundant_rows(_ptr(G, _dp), _ptr(h, _dp), m, nz, _ptr(fx, _ip) if fx is not None else None,
+                                          _ptr(keep, _ip), ctypes.byref(n_lps))
+        if rc < 0:
+            raise RuntimeError(f"mpqp_redundant_rows failed: {self.last_error()}")
+        self.redundancy_lps = n_lps.value
+        return keep
 
     def region_stats(self) -> dict:
         st = MpqpRegionStats()
@@ -559,26 +579,3 @@ def active_sets_to_regions(prog: MPQPProb, all_active_sets, lib: MpqpLib | None 
         sets = np.where(ended, -1, sets).astype(np.intc)
     batch = lib.regions(prog, sets, width=prog.num_x)
     return [r for r in (batch.region(i) for i in range(sets.shape[0])) if r is not None]
-
-
-def calculate_mpqp_with_HIP(problems, k_max: int | None = None, lib: MpqpLib | None = None,
-                            reserved_memory: int = 0, regions: bool = False):
-    """Counterpart of calculate_mpqp_with_CUDA (main.py:11-173): assembles
-    each problem's LPs, calls mpqp(), decodes the final active sets and, with
-    regions=True, builds their critical regions on the device
-    (active_sets_to_regions, main.py:154-157).  Returns one
-    (active_sets, MpqpRun) per problem, or (active_sets, MpqpRun, regions)
-    with regions=True."""
-    lib = lib or load_library()
-    lib.warm_up()
-    out = []
-    for prog in problems:
-        lp = assemble_lps(prog)
-        km = prog.num_x if k_max is None else k_max
-        run = lib.run(lp, k_max=km, k_min=0, reserved_memory=reserved_memory)
-        prog.n_active_sets = run.n_final_lps
-        if regions:
-            out.append((run.active_sets, run, active_sets_to_regions(prog, run.active_sets, lib)))
-        else:
-            out.append((run.active_sets, run))
-    return out

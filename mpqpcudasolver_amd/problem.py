@@ -2,11 +2,10 @@
 
 `MPQPProb` mirrors the attribute surface of the reference's `utils.MPQPProb`
 (utils.py:147-229): A, b, c, H, Q, inv_Q, A_t, b_t, F, num_x, n_active_sets.
-The reference's `process_constraints` calls ppopt (redundancy removal and
-implicit-equality detection, utils.py:163-199); ppopt is not available, so the
-synthetic generators below emit problems that need no such processing and
-`process_constraints` only applies the equilibration scalings that are plain
-numpy in the reference (utils.py:201-229).
+`process_constraints` follows utils.py:163-199: the ppopt routines it calls
+(pure-parameter rows, implicit equalities, redundancy removal) are restated in
+preprocess.py with the redundancy LPs on the device; the equilibration
+scalings (utils.py:201-229) are plain numpy as in the reference.
 
 `assemble_lps` restates the LP layout the reference builds before calling
 `mpqp()` (main.py:87-138): the feasibility LP (A1, xb1, b1), the optimality LP
@@ -53,10 +52,29 @@ class MPQPProb:
         self.num_x = self.A.shape[1]
         self.n_active_sets = 0
 
-    # utils.py:163-199 without the ppopt calls (see module docstring).
-    def process_constraints(self, scale_method=1):
+    # utils.py:163-199; the ppopt routines are restated in preprocess.py,
+    # their LPs run through `lib` (the device library by default)
+    def process_constraints(self, scale_method=1, lib=None):
+        from .preprocess import (find_implicit_equalities, find_redundant_constraints,
+                                 process_program_constraints)
+
+        self.A, self.b, self.F, self.A_t, self.b_t = process_program_constraints(
+            self.A, self.b, self.F, self.A_t, self.b_t)
         if scale_method == 2:
             self.scale_constraints()
+        self.A, self.b, self.F, self.equality_indices = find_implicit_equalities(
+            self.A, self.b, self.F, self.equality_indices)
+        n_c = self.A.shape[0]
+        problem_A = np.block([[self.A, -self.F], [np.zeros((self.A_t.shape[0], self.A.shape[1])), self.A_t]])
+        problem_b = np.vstack([self.b, self.b_t])
+        saved = find_redundant_constraints(problem_A, problem_b, self.equality_indices, lib=lib)
+        upper = [i for i in saved if i < n_c]
+        lower = [i - n_c for i in saved if i >= n_c]
+        self.A, self.F, self.b = self.A[upper], self.F[upper], self.b[upper]
+        self.A_t, self.b_t = self.A_t[lower], self.b_t[lower]
+        # equality indices follow their rows
+        pos = {old: new for new, old in enumerate(upper)}
+        self.equality_indices = [pos[i] for i in self.equality_indices if i in pos]
         if scale_method == 1:
             QHc = np.block([self.Q, self.H, self.c])
             QHc_norm = np.max(np.abs(QHc)) / (2 ** 3)
@@ -66,6 +84,7 @@ class MPQPProb:
             self.H = self.H / QHc_norm
             self.c = self.c / QHc_norm
             self.inv_Q = np.linalg.pinv(self.Q)
+        self.num_x = self.A.shape[1]
 
     # utils.py:201-206
     def scale_constraints(self):

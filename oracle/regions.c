@@ -60,6 +60,63 @@ static void pivot(double *T, int ld, int nrows, int ncols, int r, int q, double 
     }
 }
 
+enum { LP_OPTIMAL = 0, LP_UNBOUNDED = 1, LP_ITER_LIMIT = 2 };
+
+static void lp_pivot(double *T, int ld, int m, int K, int r, int q, int *basis, int *nonb, double *colq) {
+    pivot(T, ld, m + 1, K + 1, r, q, colq);
+    const int v = basis[r];
+    basis[r] = nonb[q];
+    nonb[q] = v;
+}
+
+/* row of the most negative rhs (lowest row on ties), -1 if none */
+static int lp_most_negative_row(const double *T, int ld, int m, int K) {
+    int r0 = -1;
+    for (int i = 0; i < m; ++i)
+        if (T[i * ld + K] < 0.0 && (r0 < 0 || T[i * ld + K] < T[r0 * ld + K])) r0 = i;
+    return r0;
+}
+
+/* reduced costs of min sum_v cost[v] x_v for the current basis */
+static void lp_set_cost(double *T, int ld, int m, int K, const int *basis, const int *nonb, const double *cost) {
+    for (int j = 0; j <= K; ++j) {
+        double acc = j < K ? cost[nonb[j]] : 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double cb = cost[basis[i]];
+            if (cb != 0.0) acc = fma(-cb, T[i * ld + j], acc);
+        }
+        T[m * ld + j] = acc;
+    }
+}
+
+/* Bland's rule: lowest-index variable with reduced cost < -1e-12 enters
+ * (never `barred`); minimum ratio over entries > 1e-11, ties to the
+ * lowest-index basic variable */
+static int lp_bland(double *T, int ld, int m, int K, int *basis, int *nonb, double *colq, int barred, int max_iter,
+                    int *it) {
+    for (; *it < max_iter; ++*it) {
+        int q = -1;
+        for (int j = 0; j < K; ++j)
+            if (T[m * ld + j] < -1e-12 && nonb[j] != barred && (q < 0 || nonb[j] < nonb[q])) q = j;
+        if (q < 0) return LP_OPTIMAL;
+        int r = -1;
+        double rv = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const double al = T[i * ld + q];
+            if (al > 1e-11) {
+                const double rt = T[i * ld + K] / al;
+                if (r < 0 || rt < rv || (rt == rv && basis[i] < basis[r])) {
+                    rv = rt;
+                    r = i;
+                }
+            }
+        }
+        if (r < 0) return LP_UNBOUNDED;
+        lp_pivot(T, ld, m, K, r, q, basis, nonb, colq);
+    }
+    return LP_ITER_LIMIT;
+}
+
 /* One set.  Y: n_x x ldy, GX: n_c x ldy. */
 static void one_region(int n_c, int n_x, int p, int n_cp, int width, const double *b, const double *F,
                        const double *CRA, const double *CRb, const double *Y, const double *GX, int ldy,
@@ -196,44 +253,10 @@ static void one_region(int n_c, int n_x, int p, int n_cp, int width, const doubl
     } else if (m == 0) {
         status = MPQP_REGION_UNBOUNDED;
     } else {
-        int r0 = -1;
-        for (int i = 0; i < m; ++i)
-            if (T[i * ld + K] < 0.0 && (r0 < 0 || T[i * ld + K] < T[r0 * ld + K])) r0 = i;
-        if (r0 >= 0) {
-            pivot(T, ld, m + 1, K + 1, r0, RM, colq);
-            const int v = basis[r0];
-            basis[r0] = nonb[RM];
-            nonb[RM] = v;
-        }
-        for (; it < max_iter; ++it) {
-            int q = -1;
-            for (int j = 0; j < K; ++j)
-                if (T[m * ld + j] < -1e-12 && (q < 0 || nonb[j] < nonb[q])) q = j;
-            if (q < 0) {
-                status = OPTIMAL;
-                break;
-            }
-            int r = -1;
-            double rv = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double al = T[i * ld + q];
-                if (al > 1e-11) {
-                    const double rt = T[i * ld + K] / al;
-                    if (r < 0 || rt < rv || (rt == rv && basis[i] < basis[r])) {
-                        rv = rt;
-                        r = i;
-                    }
-                }
-            }
-            if (r < 0) {
-                status = MPQP_REGION_UNBOUNDED;
-                break;
-            }
-            pivot(T, ld, m + 1, K + 1, r, q, colq);
-            const int v = basis[r];
-            basis[r] = nonb[q];
-            nonb[q] = v;
-        }
+        const int r0 = lp_most_negative_row(T, ld, m, K);
+        if (r0 >= 0) lp_pivot(T, ld, m, K, r0, RM, basis, nonb, colq);
+        const int st = lp_bland(T, ld, m, K, basis, nonb, colq, -1, max_iter, &it);
+        status = st == LP_OPTIMAL ? OPTIMAL : (st == LP_UNBOUNDED ? MPQP_REGION_UNBOUNDED : MPQP_REGION_ITER_LIMIT);
     }
     double radius = 0.0;
     if (status == OPTIMAL) {
@@ -350,4 +373,144 @@ int mpqp_regions(const double *A, const double *b, const double *c, const double
     free(Y);
     free(GX);
     return (int)found;
+}
+
+/* ------------------------------------------------------------------ */
+/* Redundant rows (mpqp_redundant_rows): the device algorithm restated. */
+
+enum { RED_OPTIMAL = 0, RED_EMPTY = 1, RED_ITER_LIMIT = 2, RED_UNBOUNDED = 3 };
+
+/* max g_ti z over {g_j z <= h_j : active_j or j == ti}, row ti relaxed by 1 */
+static int redund_lp(const double *G, const double *h, int m, int nz, const int *active, int ti, int max_iter,
+                     double *value, int *iters) {
+    const int K = 2 * nz + 1, ld = K + 1, W = 2 * nz;
+    double *T = calloc((size_t)(m + 1) * ld, sizeof(double));
+    double *colq = calloc((size_t)m + 1, sizeof(double));
+    double *cost = calloc((size_t)K + m, sizeof(double));
+    int *rowid = calloc((size_t)m, sizeof(int));
+    int *nonb = calloc((size_t)K, sizeof(int));
+    int *basis = calloc((size_t)m, sizeof(int));
+    int mm = 0;
+    for (int j = 0; j < m; ++j)
+        if (active[j] || j == ti) rowid[mm++] = j;
+    for (int ii = 0; ii < mm; ++ii) {
+        const int j = rowid[ii];
+        for (int c = 0; c < nz; ++c) {
+            T[ii * ld + c] = G[(size_t)j * nz + c];
+            T[ii * ld + nz + c] = -G[(size_t)j * nz + c];
+        }
+        T[ii * ld + W] = -1.0;
+        T[ii * ld + K] = h[j] + (j == ti ? 1.0 : 0.0);
+        basis[ii] = K + ii;
+    }
+    for (int j = 0; j < K; ++j) nonb[j] = j;
+    for (int j = 0; j <= K; ++j) T[mm * ld + j] = j == W ? 1.0 : 0.0;
+    int it = 0, status = RED_OPTIMAL;
+    const int r0 = lp_most_negative_row(T, ld, mm, K);
+    if (r0 >= 0) {
+        lp_pivot(T, ld, mm, K, r0, W, basis, nonb, colq);
+        const int st = lp_bland(T, ld, mm, K, basis, nonb, colq, -1, max_iter, &it);
+        int rw = -1;
+        for (int i = 0; i < mm; ++i)
+            if (basis[i] == W) rw = i;
+        if (st == LP_ITER_LIMIT) {
+            status = RED_ITER_LIMIT;
+        } else if (rw >= 0 && T[rw * ld + K] > 1e-9) {
+            status = RED_EMPTY;
+        } else if (rw >= 0) {
+            int q = -1;
+            for (int j = 0; j < K && q < 0; ++j)
+                if (fabs(T[rw * ld + j]) > 1e-9) q = j;
+            if (q >= 0) lp_pivot(T, ld, mm, K, rw, q, basis, nonb, colq);
+        }
+    }
+    if (status == RED_OPTIMAL) {
+        for (int v = 0; v < K + mm; ++v)
+            cost[v] = v < nz ? -G[(size_t)ti * nz + v] : (v < W ? G[(size_t)ti * nz + v - nz] : 0.0);
+        lp_set_cost(T, ld, mm, K, basis, nonb, cost);
+        const int st = lp_bland(T, ld, mm, K, basis, nonb, colq, W, max_iter, &it);
+        status = st == LP_OPTIMAL ? RED_OPTIMAL : (st == LP_UNBOUNDED ? RED_UNBOUNDED : RED_ITER_LIMIT);
+    }
+    *value = status == RED_OPTIMAL ? T[mm * ld + K] : 0.0;
+    *iters = it;
+    free(T);
+    free(colq);
+    free(cost);
+    free(rowid);
+    free(nonb);
+    free(basis);
+    return status;
+}
+
+int mpqp_redundant_rows(const double *G, const double *h, int m, int nz, const int *fixed, int *keep,
+                        long long *n_lps) {
+    if (m <= 0 || nz <= 0 || !G || !h || !keep) {
+        oracle_set_error("mpqp_redundant_rows: bad arguments");
+        return -1;
+    }
+    double *Gs = calloc((size_t)m * nz, sizeof(double));
+    double *hs = calloc((size_t)m, sizeof(double));
+    int *active = malloc(sizeof(int) * (size_t)m);
+    int *tests = malloc(sizeof(int) * (size_t)m);
+    char *flag = calloc((size_t)m, 1);
+    int nt = 0, rc = 0;
+    long long lps = 0;
+    for (int j = 0; j < m; ++j) {
+        double nrm = 0.0;
+        for (int c = 0; c < nz; ++c) nrm += G[(size_t)j * nz + c] * G[(size_t)j * nz + c];
+        nrm = sqrt(nrm);
+        active[j] = 1;
+        if (nrm == 0.0) {
+            if (h[j] < 0.0) {
+                oracle_set_error("mpqp_redundant_rows: a zero row has h < 0");
+                rc = -1;
+                goto done;
+            }
+            active[j] = 0;
+            continue;
+        }
+        for (int c = 0; c < nz; ++c) Gs[(size_t)j * nz + c] = G[(size_t)j * nz + c] / nrm;
+        hs[j] = h[j] / nrm;
+        if (!(fixed && fixed[j])) tests[nt++] = j;
+    }
+    const int max_iter = 50 * (m + 2 * nz + 1) + 1000;
+    /* batch pass against every other row, then the sequential pass */
+    for (int pass = 0; pass < 2 && rc == 0; ++pass) {
+        for (int t = 0; t < nt; ++t) {
+            const int i = tests[t];
+            if (pass == 1 && !flag[t]) continue;
+            /* pass 0 removes nothing: every test sees all rows */
+            double v = 0.0;
+            int it = 0;
+            const int st = redund_lp(Gs, hs, m, nz, active, i, max_iter, &v, &it);
+            ++lps;
+            if (st == RED_EMPTY) {
+                oracle_set_error("mpqp_redundant_rows: the polyhedron is empty");
+                rc = -1;
+                break;
+            }
+            if (st != RED_OPTIMAL) {
+                oracle_set_error("mpqp_redundant_rows: an LP did not finish");
+                rc = -1;
+                break;
+            }
+            const int red = v <= hs[i] + 1e-9 * (fabs(hs[i]) > 1.0 ? fabs(hs[i]) : 1.0);
+            if (pass == 0) flag[t] = (char)red;
+            else if (red) active[i] = 0;
+        }
+    }
+    if (rc == 0) {
+        for (int j = 0; j < m; ++j) {
+            keep[j] = active[j];
+            rc += active[j];
+        }
+        if (n_lps) *n_lps = lps;
+    }
+done:
+    free(Gs);
+    free(hs);
+    free(active);
+    free(tests);
+    free(flag);
+    return rc;
 }
