@@ -320,11 +320,11 @@ def per_problem(runs):
     return piv, feas
 
 
-def measure(lib, cfg, steps, warmup, mixed, batching, compare_fp64=True):
+def measure(lib, cfg, steps, warmup, mixed, batching, compare_fp64=True, restart_pct=25):
     """Headline variant and FP64-plain comparison on config `cfg`."""
     prog, k_max = config_problem(cfg)
     lps = [assemble_lps(p) for p in prog] if isinstance(prog, list) else [assemble_lps(prog)]
-    opts = dict(precision=1 if mixed else 0, batching=1 if batching == "persistent" else 0)
+    opts = dict(precision=1 if mixed else 0, batching=1 if batching == "persistent" else 0, restart_pct=restart_pct)
     st, dt = timed(lib, lps, k_max, steps, warmup, **opts)
     value = st["candidates"] * steps / dt
     tuning = lib.tuning()
@@ -364,6 +364,11 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true",
                     help="time the headline variant only (no FP64-plain run, no verdict comparison); "
                          "for profiler runs")
+    ap.add_argument("--restart-pct", type=int, default=25,
+                    help="mixed precision: FP64 restart when more than this %% of the level-0/1 optimality "
+                         "LPs fail certification (100: never)")
+    ap.add_argument("--regions-only", action="store_true",
+                    help="only time mpqp_regions() on the final sets of --config and exit (profiler runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 4 times (calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -387,8 +392,12 @@ def main() -> int:
     lib = load_library()
     if lib.warm_up() != 0:
         raise RuntimeError(lib.last_error())
+    if args.regions_only:
+        print(json.dumps(regions_bench(lib, args.config, args.steps)[0]), flush=True)
+        return 0
     mixed = args.precision == "mixed"
-    h = measure(lib, args.config, args.steps, args.warmup, mixed, args.batching, not args.no_compare)
+    h = measure(lib, args.config, args.steps, args.warmup, mixed, args.batching, not args.no_compare,
+                args.restart_pct)
     st = h["st"]
     lds_peak = lib.measure_lds_bandwidth()
     if not lds_peak > 0:
@@ -432,7 +441,7 @@ def main() -> int:
         for cfg in (1, 2, 3):
             if cfg == args.config:
                 continue
-            e = measure(lib, cfg, args.steps, args.warmup, mixed, args.batching)
+            e = measure(lib, cfg, args.steps, args.warmup, mixed, args.batching, restart_pct=args.restart_pct)
             extras[str(cfg)] = {
                 "workload": _workload(cfg), "value": e["value"], "ms_per_step": e["dt"] / args.steps * 1e3,
                 "fp64_plain_value": e["v64"], "speedup": e["value"] / e["v64"] if e["v64"] > 0 else None,

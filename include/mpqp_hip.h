@@ -132,7 +132,10 @@ typedef struct MpqpOptions {
     int split_wgs;             /* an optimality tableau larger than LDS keeps as many rows in LDS
                                   as fit when split_wgs workgroups share a CU (1 or 2), the rest
                                   in global memory; 0: chosen per precision */
-    int reserved2;
+    int restart_pct;           /* mixed precision: if more than restart_pct % of the optimality LPs of
+                                  levels 0 and 1 (the empty set and the singletons) fail FP64
+                                  certification, the call restarts in FP64; 25 (default);
+                                  100: never restart (failed LPs are re-solved in FP64 one by one) */
 } MpqpOptions;
 
 /* One record per candidate active set of the last mpqp() call, in

@@ -322,6 +322,7 @@ void default_opts(MpqpOptions *o) {
     o->licq_screen = 1e-14;
     o->max_chunk = 0;
     o->split_wgs = 0;
+    o->restart_pct = 25;
 }
 
 const MpqpOptions &opts() {
@@ -1113,7 +1114,9 @@ struct Run {
         if (mixed && level <= 1) {
             probe_opt += nf;
             probe_fb += n_fb;
-            if (level == 1 && probe_opt > 0 && (double)probe_fb > 0.25 * (double)probe_opt) return false;
+            if (level == 1 && o.restart_pct < 100 && probe_opt > 0 &&
+                (double)probe_fb * 100.0 > (double)o.restart_pct * (double)probe_opt)
+                return false;
         }
         if (probe_only) return true;
 
@@ -1690,7 +1693,7 @@ int mpqp_set_options(const MpqpOptions *o) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!o || o->max_iter_feas < 0 || o->max_iter_phase1 < 0 || o->max_iter_phase2 < 0 ||
         !(o->tol_plus >= 0.0) || !(o->tol_minus <= 0.0) || !(o->licq_tol >= 0.0) || !(o->licq_screen >= 0.0) ||
-        o->max_chunk < 0 || o->split_wgs < 0 || o->split_wgs > 2 ||
+        o->max_chunk < 0 || o->split_wgs < 0 || o->split_wgs > 2 || o->restart_pct < 0 || o->restart_pct > 100 ||
         (o->opt_threads != 0 && o->opt_threads != 256 && o->opt_threads != 512 && o->opt_threads != 1024) ||
         (o->precision != MPQP_FP64 && o->precision != MPQP_MIXED) ||
         (o->batching != MPQP_PLAIN && o->batching != MPQP_PERSISTENT)) {

@@ -97,7 +97,7 @@ class MpqpOptions(ctypes.Structure):
         ("licq_screen", ctypes.c_double),
         ("max_chunk", ctypes.c_longlong),
         ("split_wgs", ctypes.c_int),
-        ("reserved2", ctypes.c_int),
+        ("restart_pct", ctypes.c_int),
     ]
 
 

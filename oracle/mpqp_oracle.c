@@ -158,6 +158,7 @@ void mpqp_default_options(MpqpOptions *o) {
     o->licq_screen = 1e-14;
     o->max_chunk = 0; /* no effect on the CPU */
     o->split_wgs = 0; /* no effect on the CPU */
+    o->restart_pct = 25;
 }
 
 static const MpqpOptions *opts(void) {
@@ -1189,11 +1190,11 @@ static double now_s(void) {
 }
 
 /* Mixed-precision policy: the empty set and every singleton (levels 0
- * and 1, n_c + 1 LPs) are solved with FP32 pivots; if more than a quarter
- * of their optimality LPs fail FP64 certification, the problem is too
- * ill-conditioned for FP32 pivots and is solved in FP64 throughout.
+ * and 1, n_c + 1 LPs) are solved with FP32 pivots; if more than
+ * restart_pct % (default 25) of their optimality LPs fail FP64
+ * certification, the problem is too ill-conditioned for FP32 pivots and is
+ * solved in FP64 throughout (never with restart_pct = 100).
  * Returns 1 to keep mixed precision. */
-#define MIXED_PROBE_MAX_FALLBACK 0.25
 /* A candidate's phase-1 state warm-starts its children whenever its phase 1
  * ended optimal (copy_basis_dual_1 :553-590 takes only roots with
  * |z| < 1e-12; phase 1 here ends with the dual-feasibility cleanup, so any
@@ -1223,7 +1224,7 @@ static int mixed_probe(const Prob *P, const MpqpOptions *o) {
     fstate_free(&F0); fstate_free(&F1);
     opack_free(&W0); opack_free(&W1); opack_free(&W2);
     scratch_free(&s);
-    return !(n_opt > 0 && (double)n_fb > MIXED_PROBE_MAX_FALLBACK * (double)n_opt);
+    return !(o->restart_pct < 100 && n_opt > 0 && (double)n_fb * 100.0 > (double)o->restart_pct * (double)n_opt);
 }
 
 GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max,
