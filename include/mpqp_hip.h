@@ -176,6 +176,31 @@ int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_
                  const double *A2, const double *xb2, const int *b2, const int *non_b2,
                  int M2, int N2, int n_eqs, GPUResults *out, int *active_sets, int max_sets);
 
+/* The arguments of one mpqp() call (main.py:144), for mpqp_many(). */
+typedef struct MpqpProblem {
+    double *AxT;
+    int n_c, n_x, n_t, k_min, k_max;
+    double *A1, *xb1;
+    int *b1;
+    int M1, N1;
+    double *A2, *xb2;
+    int *b2, *non_b2;
+    int M2, N2, n_eqs;
+} MpqpProblem;
+
+/* Many problems in one call, for a host that loops over problems as
+ * main.py:64-170 does: the results equal n_problems mpqp() calls with the
+ * current options (records are not kept).  `workers` host threads (0: 4),
+ * each with its own stream and a share of the device memory, take the
+ * problems in order, so the launches of several problems overlap on the
+ * device.  results[i] receives problem i's result, owned by the caller
+ * (free each with mpqp_free_results; later calls do not free them).
+ * mpqp_get_stats() then sums the problems' statistics (kernel times of
+ * overlapping launches add up).  Returns n_problems, or -1 on error (every
+ * result freed and NULL; see mpqp_last_error). */
+int mpqp_many(const MpqpProblem *problems, int n_problems, int reserved_memory, int workers,
+              GPUResults **results);
+
 /* Frees a result returned by mpqp() (the reference has no free symbol).  A
  * result not freed by the caller is freed by the next mpqp() call, so a host
  * that never calls this holds at most one result. */

@@ -134,8 +134,8 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     }
     for (int w = tid; w < nw; w += NT) inS[w] = 0u;
     __syncthreads();
-    if (tid < a.level) {  // one thread per member of S
-        const int s = a.sets[(long long)c * a.kw + tid];
+    for (int t = tid; t < a.level; t += NT) {  // the members of S
+        const int s = a.sets[(long long)c * a.kw + t];
         atomicOr(&inS[s >> 5], 1u << (s & 31));
     }
     const int par = a.parent[c];

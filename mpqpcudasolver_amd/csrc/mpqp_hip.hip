@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -33,6 +34,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -44,6 +46,7 @@
 static_assert(sizeof(DevRecord) == sizeof(MpqpRecord), "record layout");
 static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
 static_assert(sizeof(DevRegion) == sizeof(MpqpRegionInfo), "region info layout");
+static_assert(sizeof(MpqpProblem) == 112, "MpqpProblem layout");
 
 // Device memory of mpqp_noalloc(): every call on a GPU uses that GPU's copy
 // of this array (it holds the last call's contents; overlapping calls on
@@ -54,7 +57,14 @@ static __device__ __align__(256) unsigned char scratch[MPQP_SCRATCH_BYTES];
 
 namespace {
 
-constexpr int FEAS_NT = 256;
+// Threads per feasibility LP when the autotuner does not choose: about 16
+// tableau entries per thread, one wave for a small LP, at most 512.
+int feas_threads_for(int M1, int K1) {
+    const long long entries = (long long)(M1 + 1) * (K1 + 1);
+    int nt = 64;
+    while (nt < 512 && (long long)nt * 16 < entries) nt *= 2;
+    return nt;
+}
 constexpr int OPT_NT = 512;
 
 template <int NT, bool FP32, bool GT>
@@ -87,6 +97,7 @@ const void *opt_fn(bool fp32, int nt, int S, bool gt) {
 // The feasibility kernel at a workgroup size the autotuner may choose.
 void launch_feas(int nt, int grid, size_t lds, hipStream_t st, const FeasArgs &fa) {
     switch (nt) {
+    case 64: hipLaunchKernelGGL(feas_kernel<64>, dim3(grid), dim3(64), lds, st, fa); break;
     case 128: hipLaunchKernelGGL(feas_kernel<128>, dim3(grid), dim3(128), lds, st, fa); break;
     case 512: hipLaunchKernelGGL(feas_kernel<512>, dim3(grid), dim3(512), lds, st, fa); break;
     default: hipLaunchKernelGGL(feas_kernel<256>, dim3(grid), dim3(256), lds, st, fa); break;
@@ -190,13 +201,21 @@ struct PoolArena : Arena {
         }
         return BufView{bytes ? b.p : nullptr, bytes};
     }
+    // most bytes this arena's budget may report (workers of mpqp_many share the device)
+    size_t limit = ~(size_t)0;
     size_t budget(size_t reserved) override {
         size_t fr = 0, tot = 0, held = 0;
         HIP_OK(hipMemGetInfo(&fr, &tot));
         for (const Buf &b : bufs) held += b.cap;
         const size_t margin = (size_t)1 << 30;
         const size_t avail = fr + held;
-        return avail > reserved + margin ? avail - reserved - margin : (size_t)64 << 20;
+        const size_t b = avail > reserved + margin ? avail - reserved - margin : (size_t)64 << 20;
+        return std::min(b, std::max(limit, (size_t)64 << 20));
+    }
+    void release() {
+        for (Buf &b : bufs)
+            if (b.p) hipFree(b.p);
+        bufs.clear();
     }
 };
 
@@ -497,7 +516,7 @@ struct Run {
     long long fstride = 0, flds = 0, ostride = 0, lds64 = 0, lds32 = 0;
     bool gt64 = false, gt32 = false;
     int R64 = 0, R32 = 0;  // rows of a split tableau kept in LDS
-    int S = 1, feas_nt = FEAS_NT, opt_nt = OPT_NT, cap64 = 0, cap32 = 0;
+    int S = 1, feas_nt = 256, opt_nt = OPT_NT, cap64 = 0, cap32 = 0;
     int w32 = 2, w64 = 1;                // workgroups per CU a split tableau is sized for
     size_t gtab64_max = 0, gtab32_max = 0;  // global rows of the largest variant
     const void *fn64 = nullptr, *fn32 = nullptr;
@@ -705,7 +724,7 @@ struct Run {
         gt64 = opt_lds_bytes_t<double>(M2, K2, N2, false, M2 + 1) > budget;
         gt32 = opt_lds_bytes_t<float>(M2, K2, N2, true, M2 + 1) > budget;
         S = red_slots(std::max(M2, N2 - in.n_eqs));
-        feas_nt = FEAS_NT;
+        feas_nt = feas_threads_for(M1, K1);
         // 1024 threads only where every optimality kernel of the call splits its tableau
         const bool all_split = gt64 && (gt32 || o.precision != MPQP_MIXED);
         const int nt0 = o.opt_threads == 1024 ? (all_split ? 1024 : OPT_NT)
@@ -955,7 +974,7 @@ struct Run {
             FeasArgs ft = fa;
             ft.n = std::min(n, 8192);
             float best = INFINITY;
-            for (int nt : {128, 256, 512}) {
+            for (int nt : {64, 128, 256, 512}) {
                 HIP_OK(hipEventRecord(e0, st));
                 launch_feas(nt, ft.n, (size_t)flds, st, ft);
                 HIP_OK(hipGetLastError());
@@ -1328,9 +1347,9 @@ struct Run {
     }
 };
 
-// shared by mpqp() and mpqp_noalloc(): validation and the traversal
-Output run_call(const Input &in, Arena &A, bool noalloc) {
-    Ctx &C = ctx();
+// shared by mpqp(), mpqp_noalloc() and the workers of mpqp_many():
+// validation and the traversal
+Output run_call_ctx(Ctx &C, const Input &in, Arena &A, bool noalloc, bool many = false) {
     const std::string v = validate(in.n_c, in.n_x, in.n_t, in.b1, in.M1, in.N1, in.xb1, in.b2, in.non_b2, in.M2,
                                    in.N2, in.n_eqs, in.k_min, in.k_max);
     if (!v.empty()) throw HipError{v};
@@ -1341,13 +1360,144 @@ Output run_call(const Input &in, Arena &A, bool noalloc) {
     if (opt_lds_bytes_t<double>(in.M2, in.N2 - in.M2, in.N2, false, 1) > C.lds_limit - 512)
         throw HipError{"mpqp: optimality LP bookkeeping exceeds the LDS of one workgroup (" +
                        std::to_string(C.lds_limit) + " bytes)"};
-    MpqpOptions o = opts();
+    MpqpOptions o = many ? C.opt : opts();
     // record-free, device-side enumeration without pinned staging
-    if (noalloc) o.record = 0;
+    if (noalloc || many) o.record = 0;
     A.begin_call();
     Run r(C, o, A, in, noalloc);
     r.run();
     return r.out;
+}
+
+Output run_call(const Input &in, Arena &A, bool noalloc) { return run_call_ctx(ctx(), in, A, noalloc); }
+
+// the GPUResults of one call, rows of n_x entries (-1 padded)
+GPUResults *make_result(const Output &out, int n_x) {
+    GPUResults *res = static_cast<GPUResults *>(std::calloc(1, sizeof(GPUResults)));
+    const int kw = out.kw;
+    const int nfin = (int)(out.fin.size() / (size_t)kw);
+    res->active_sets = static_cast<int *>(std::malloc(sizeof(int) * (size_t)std::max(nfin, 1) * n_x));
+    for (int row = 0; row < nfin; ++row)
+        for (int t = 0; t < n_x; ++t)
+            res->active_sets[(size_t)row * n_x + t] = t < kw ? out.fin[(size_t)row * kw + t] : -1;
+    res->n_batches = out.n_batches;
+    res->n_splits = 1;
+    res->n_feasible_lps = out.n_feasible;
+    res->n_final_lps = nfin;
+    res->time_feas = out.tf;
+    res->time_opti = out.to;
+    res->time_GPU = out.t_gpu;
+    return res;
+}
+
+void add_stats(MpqpStats &d, const MpqpStats &s) {
+    d.levels += s.levels;
+    d.launches_feas += s.launches_feas;
+    d.launches_opt += s.launches_opt;
+    d.precision_used = s.precision_used;
+    d.candidates += s.candidates;
+    d.opt_lps += s.opt_lps;
+    d.pivots_feas += s.pivots_feas;
+    d.pivots_p1 += s.pivots_p1;
+    d.pivots_p2 += s.pivots_p2;
+    d.ms_feas += s.ms_feas;
+    d.ms_opt += s.ms_opt;
+    d.bytes_feas += s.bytes_feas;
+    d.bytes_opt += s.bytes_opt;
+    d.flops_feas += s.flops_feas;
+    d.flops_opt += s.flops_opt;
+    d.fallbacks += s.fallbacks;
+    d.launches_fallback += s.launches_fallback;
+    d.tableau_global |= s.tableau_global;
+    d.ms_fallback += s.ms_fallback;
+    d.bytes_fallback += s.bytes_fallback;
+    d.flops_fallback += s.flops_fallback;
+}
+
+// mpqp_many(): `workers` host threads, each with its own stream, events and
+// pool arena (a share of the device-memory budget), take the problems in
+// order from a shared counter and run each as mpqp() would.  Their launches
+// overlap on the device, so the small levels of one problem share the GPU
+// with the large levels of another.  Kernels never wait on one another (the
+// persistent grids drain their work counters and exit), so concurrent
+// launches cannot deadlock.  Tuned shapes are shared through the main context.
+int run_many(const MpqpProblem *pr, int n, int reserved_memory, int workers, GPUResults **res) {
+    Ctx &M = ctx();
+    ensure_ready();
+    const MpqpOptions o = opts();
+    const int W = std::max(1, std::min(workers > 0 ? workers : 4, n));
+    size_t fr = 0, tot = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    const size_t reserved = (size_t)std::max(reserved_memory, 0) << 20;
+    const size_t share = fr > reserved + ((size_t)1 << 30) ? (fr - reserved - ((size_t)1 << 30)) / (size_t)W : 0;
+    std::vector<Ctx> wc(W);
+    for (Ctx &c : wc) {
+        c.ready = true;
+        c.device = M.device;
+        HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        for (hipEvent_t &e : c.ev) HIP_OK(hipEventCreate(&e));
+        c.opt = o;
+        c.opt_set = true;
+        c.lds_limit = M.lds_limit;
+        c.n_cu = M.n_cu;
+        c.tuned = M.tuned;
+        c.pools.limit = share;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::string> errs(n);
+    std::vector<MpqpStats> tot_stats(W);
+    auto work = [&](int w) {
+        Ctx &C = wc[w];
+        if (hipSetDevice(C.device) != hipSuccess) {
+            errs[0] = "mpqp_many: hipSetDevice failed";
+            return;
+        }
+        for (int i = next++; i < n; i = next++) {
+            const MpqpProblem &p = pr[i];
+            try {
+                const Input in{p.AxT, p.n_c, p.n_x, p.n_t, p.k_min, p.k_max, p.A1, p.xb1, p.b1, p.M1, p.N1,
+                               p.A2, p.xb2, p.b2, p.non_b2, p.M2, p.N2, p.n_eqs, reserved_memory};
+                const Output out = run_call_ctx(C, in, C.pools, false, true);
+                res[i] = make_result(out, p.n_x);
+                add_stats(tot_stats[w], C.stats);
+            } catch (const HipError &e) {
+                errs[i] = "problem " + std::to_string(i) + ": " + e.msg;
+            } catch (const std::exception &e) {
+                errs[i] = "problem " + std::to_string(i) + ": " + e.what();
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    work(0);
+    for (std::thread &t : th) t.join();
+    MpqpStats st{};
+    for (int w = 0; w < W; ++w) {
+        add_stats(st, tot_stats[w]);
+        for (const auto &kv : wc[w].tuned) M.tuned.insert(kv);
+        hipStreamSynchronize(wc[w].stream);
+        wc[w].pools.release();
+        for (hipEvent_t e : wc[w].ev) hipEventDestroy(e);
+        hipStreamDestroy(wc[w].stream);
+        wc[w].stream = nullptr;
+    }
+    M.stats = st;
+    M.last_feas_nt = wc[0].last_feas_nt;
+    M.last_opt_nt = wc[0].last_opt_nt;
+    M.rec.clear();
+    M.sets.clear();
+    M.fb.clear();
+    M.b1.clear();
+    M.b2.clear();
+    for (int i = 0; i < n; ++i)
+        if (!errs[i].empty()) {
+            for (int j = 0; j < n; ++j) {
+                free_result(res[j]);
+                res[j] = nullptr;
+            }
+            throw HipError{errs[i]};
+        }
+    return n;
 }
 
 // mpqp_regions(): per problem Y = Q^-1 [A^T | H | c] and GX = A Y on the
@@ -1611,20 +1761,7 @@ GPUResults *mpqp(double *AxT, int n_c, int n_x, int n_t, int k_min, int k_max, i
         const Input in{AxT, n_c, n_x, n_t, k_min, k_max, A1, xb1, b1, M1, N1, A2, xb2, b2, non_b2, M2, N2, n_eqs,
                        reserved_memory};
         const Output out = run_call(in, C.pools, false);
-        GPUResults *res = static_cast<GPUResults *>(std::calloc(1, sizeof(GPUResults)));
-        const int kw = out.kw;
-        const int nfin = (int)(out.fin.size() / (size_t)kw);
-        res->active_sets = static_cast<int *>(std::malloc(sizeof(int) * (size_t)std::max(nfin, 1) * n_x));
-        for (int row = 0; row < nfin; ++row)
-            for (int t = 0; t < n_x; ++t)
-                res->active_sets[(size_t)row * n_x + t] = t < kw ? out.fin[(size_t)row * kw + t] : -1;
-        res->n_batches = out.n_batches;
-        res->n_splits = 1;
-        res->n_feasible_lps = out.n_feasible;
-        res->n_final_lps = nfin;
-        res->time_feas = out.tf;
-        res->time_opti = out.to;
-        res->time_GPU = out.t_gpu;
+        GPUResults *res = make_result(out, n_x);
         C.last_res = res;
         return res;
     } catch (const HipError &e) {
@@ -1671,6 +1808,26 @@ int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_
             for (int t = 0; t < n_x; ++t)
                 active_sets[(size_t)row * n_x + t] = t < kw ? o.fin[(size_t)row * kw + t] : -1;
         return 0;
+    } catch (const HipError &e) {
+        C.err = e.msg;
+        return -1;
+    } catch (const std::exception &e) {
+        C.err = e.what();
+        return -1;
+    }
+}
+
+int mpqp_many(const MpqpProblem *problems, int n_problems, int reserved_memory, int workers, GPUResults **results) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    Ctx &C = ctx();
+    C.err.clear();
+    if (n_problems < 0 || (n_problems > 0 && (!problems || !results)) || workers < 0) {
+        C.err = "mpqp_many: null arguments or negative counts";
+        return -1;
+    }
+    for (int i = 0; i < n_problems; ++i) results[i] = nullptr;
+    try {
+        return run_many(problems, n_problems, reserved_memory, workers, results);
     } catch (const HipError &e) {
         C.err = e.msg;
         return -1;

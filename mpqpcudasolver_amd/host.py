@@ -32,6 +32,7 @@ __all__ = [
     "load_library",
     "decode_active_sets",
     "STATUS_NAMES",
+    "MpqpProblem",
     "MpqpRegionInfo",
     "MpqpRegionStats",
     "RegionBatch",
@@ -220,6 +221,18 @@ MPQP_ARGTYPES = [
 ]
 
 
+class MpqpProblem(ctypes.Structure):
+    """The arguments of one mpqp() call, for mpqp_many()."""
+
+    _fields_ = [
+        ("AxT", _dp), ("n_c", ctypes.c_int), ("n_x", ctypes.c_int), ("n_t", ctypes.c_int),
+        ("k_min", ctypes.c_int), ("k_max", ctypes.c_int),
+        ("A1", _dp), ("xb1", _dp), ("b1", _ip), ("M1", ctypes.c_int), ("N1", ctypes.c_int),
+        ("A2", _dp), ("xb2", _dp), ("b2", _ip), ("non_b2", _ip),
+        ("M2", ctypes.c_int), ("N2", ctypes.c_int), ("n_eqs", ctypes.c_int),
+    ]
+
+
 def library_path() -> str:
     return os.path.join(_DIR, "libmpqp_hip.so")
 
@@ -353,6 +366,10 @@ class MpqpLib:
         if hasattr(L, "mpqp_get_tuning"):
             L.mpqp_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
             L.mpqp_get_tuning.restype = ctypes.c_int
+        if hasattr(L, "mpqp_many"):
+            L.mpqp_many.argtypes = [ctypes.POINTER(MpqpProblem), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.POINTER(GPUResults))]
+            L.mpqp_many.restype = ctypes.c_int
         self.has_regions = hasattr(L, "mpqp_regions")
         if self.has_regions:
             L.mpqp_regions.argtypes = [_dp] * 8 + [ctypes.c_int] * 4 + [_ip, ctypes.c_int, ctypes.c_int,
@@ -528,10 +545,45 @@ class MpqpLib:
             n_final_lps=out.n_final_lps, time_feas=out.time_feas, time_opti=out.time_opti,
             time_GPU=out.time_GPU, active_sets=decode_active_sets(out.active_sets, out.n_final_lps, lp.n_x))
 
-    def run_many(self, lps, k_max: int, records: bool = False, decode: bool = True) -> tuple[list, dict]:
+    def run_batch(self, lps, k_max: int, k_min: int = 0, reserved_memory: int = 0, workers: int = 0,
+                  decode: bool = True) -> tuple[list, dict]:
+        """mpqp_many(): every problem of `lps` in one call, `workers` host
+        threads (0: the library's default) overlapping their launches on the
+        device.  Returns the runs and the summed stats."""
+        n = len(lps)
+        arr = (MpqpProblem * max(n, 1))()
+        self._keep = list(lps)
+        for i, lp in enumerate(lps):
+            arr[i] = MpqpProblem(_ptr(lp.A0, _dp), lp.n_c, lp.n_x, lp.n_t, k_min, k_max,
+                                 _ptr(lp.A1, _dp), _ptr(lp.xb1, _dp), _ptr(lp.b1, _ip), lp.M1, lp.N1,
+                                 _ptr(lp.A2, _dp), _ptr(lp.xb2, _dp), _ptr(lp.b2, _ip), _ptr(lp.non_b2, _ip),
+                                 lp.M2, lp.N2, lp.n_eqs)
+        res = (ctypes.POINTER(GPUResults) * max(n, 1))()
+        rc = self.lib.mpqp_many(arr, n, reserved_memory, workers, res)
+        if rc < 0:
+            raise RuntimeError(f"mpqp_many failed: {self.last_error()}")
+        runs = []
+        for i, lp in enumerate(lps):
+            r = res[i].contents
+            runs.append(MpqpRun(
+                n_batches=r.n_batches, n_splits=r.n_splits, n_feasible_lps=r.n_feasible_lps,
+                n_final_lps=r.n_final_lps, time_feas=r.time_feas, time_opti=r.time_opti, time_GPU=r.time_GPU,
+                active_sets=decode_active_sets(r.active_sets, r.n_final_lps, lp.n_x) if decode else None))
+            self.lib.mpqp_free_results(res[i])
+        return runs, self.stats()
+
+    def run_many(self, lps, k_max: int, records: bool = False, decode: bool = True,
+                 concurrent: bool | None = None) -> tuple[list, dict]:
         """One mpqp() call per problem, as the reference's host loops over its
         problems (main.py:60-170); returns the runs and the summed stats
-        (counts and kernel times added, precision_used of the last call)."""
+        (counts and kernel times added, precision_used of the last call).
+        concurrent (default: persistent batching, no records, several
+        problems) runs them through mpqp_many instead, launches overlapping."""
+        if concurrent is None:
+            concurrent = (not records and len(lps) > 1 and hasattr(self.lib, "mpqp_many")
+                          and self.get_options().batching == MPQP_PERSISTENT)
+        if concurrent:
+            return self.run_batch(lps, k_max, decode=decode)
         runs, tot = [], None
         for lp in lps:
             runs.append(self.run(lp, k_max=k_max, records=records, decode=decode))

@@ -1459,6 +1459,45 @@ int mpqp_noalloc(const double *AxT, int n_c, int n_x, int n_t, int k_min, int k_
     return rc;
 }
 
+/* mpqp_many(): the problems one after another through mpqp() (the CPU has
+ * no launches to overlap); results owned by the caller, statistics summed. */
+int mpqp_many(const MpqpProblem *pr, int n, int reserved_memory, int workers, GPUResults **res) {
+    (void)workers;
+    if (n < 0 || (n > 0 && (!pr || !res))) { set_err("mpqp_many: null arguments or negative counts"); return -1; }
+    MpqpStats tot;
+    memset(&tot, 0, sizeof tot);
+    MpqpOptions keep = *opts(), o1 = keep;
+    o1.record = 0;
+    mpqp_set_options(&o1);
+    for (int i = 0; i < n; i++) res[i] = NULL;
+    for (int i = 0; i < n; i++) {
+        const MpqpProblem *p = &pr[i];
+        GPUResults *r = mpqp(p->AxT, p->n_c, p->n_x, p->n_t, p->k_min, p->k_max, 0, p->A1, p->xb1, p->b1, p->M1,
+                             p->N1, p->A2, p->xb2, p->b2, p->non_b2, p->M2, p->N2, p->n_eqs, reserved_memory);
+        if (!r) {
+            for (int j = 0; j < i; j++) { free(res[j]->active_sets); free(res[j]); res[j] = NULL; }
+            mpqp_set_options(&keep);
+            return -1;
+        }
+        g_last = NULL; /* owned by the caller */
+        res[i] = r;
+        tot.levels += g_stats.levels;
+        tot.launches_feas += g_stats.launches_feas;
+        tot.candidates += g_stats.candidates;
+        tot.opt_lps += g_stats.opt_lps;
+        tot.pivots_feas += g_stats.pivots_feas;
+        tot.pivots_p1 += g_stats.pivots_p1;
+        tot.pivots_p2 += g_stats.pivots_p2;
+        tot.fallbacks += g_stats.fallbacks;
+        tot.ms_feas += g_stats.ms_feas;
+        tot.ms_opt += g_stats.ms_opt;
+        tot.precision_used = g_stats.precision_used;
+    }
+    mpqp_set_options(&keep);
+    g_stats = tot;
+    return n;
+}
+
 int mpqp_get_stats(MpqpStats *st) {
     if (!st) return -1;
     *st = g_stats;

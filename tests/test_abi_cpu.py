@@ -8,7 +8,7 @@ import pytest
 
 from mpqpcudasolver_amd import (GPUResults, MpqpOptions, MpqpRecord, assemble_lps, config_problem,
                                 library_path, load_library)
-from mpqpcudasolver_amd.host import MPQP_ARGTYPES, MpqpStats
+from mpqpcudasolver_amd.host import MPQP_ARGTYPES, MpqpProblem, MpqpRegionInfo, MpqpStats
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mpqp_hip.h")
@@ -51,6 +51,8 @@ def test_struct_sizes():
     assert ctypes.sizeof(MpqpRecord) == 80
     assert ctypes.sizeof(MpqpOptions) == 48 + 16 * 8 + 8 + 8
     assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8 + 8 + 3 * 8
+    assert ctypes.sizeof(MpqpProblem) == 112
+    assert ctypes.sizeof(MpqpRegionInfo) == 24
 
 
 def test_backend_and_options(hip):

@@ -17,7 +17,7 @@ import pytest
 
 from basis_check import check_run
 from mpqpcudasolver_amd import (MPQPProb, assemble_lps, config_problem, load_library, precision_stress_mpqp,
-                                synthetic_mpqp)
+                                synthetic_mpqp, uneven_mpqps)
 from oracle_lib import OracleLib
 from parity import FLOAT_FIELDS, GOLDEN, INT_FIELDS, REL_TOL, Golden, compare_runs
 from reftree import pred_dmin, pred_dratio, pred_p2, tree_argmin, tree_sum
@@ -619,3 +619,34 @@ def test_reference_call_kmax_nx(hip, orc):
     orc.set_options()
     assert compare_runs(g, c, bases=False) == []
     assert c.n_batches > g.n_batches
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_many_matches_one_call_per_problem(hip, precision):
+    """mpqp_many (several host threads, launches overlapping on the device)
+    returns, problem by problem, what one mpqp() call per problem returns."""
+    _log(f"case: mpqp_many vs mpqp, 64 uneven problems, |S| <= 2, precision {precision}")
+    progs = uneven_mpqps(64, seed=2)
+    lps = [assemble_lps(p) for p in progs]
+    hip.set_options(precision=precision, batching=1)
+    seq, st_seq = hip.run_many(lps, k_max=2, concurrent=False)
+    many, st_many = hip.run_many(lps, k_max=2, concurrent=True)
+    hip.set_options()
+    for i, (a, b) in enumerate(zip(seq, many)):
+        assert a.active_sets == b.active_sets, i
+        assert (a.n_feasible_lps, a.n_final_lps) == (b.n_feasible_lps, b.n_final_lps), i
+    for key in ("candidates", "opt_lps", "pivots_feas", "pivots_p1", "pivots_p2", "fallbacks"):
+        assert st_seq[key] == st_many[key], key
+
+
+def test_many_reports_a_failing_problem(hip):
+    progs = uneven_mpqps(6, seed=5)
+    lps = [assemble_lps(p) for p in progs]
+    bad = lps[3]
+    bad.b2 = bad.b2.copy()
+    bad.b2[0] = bad.b2[1]  # not the KKT basis
+    with pytest.raises(RuntimeError, match="problem 3"):
+        hip.run_batch(lps, k_max=2)
+    # the library still works afterwards
+    runs, _ = hip.run_batch([lps[0], lps[1]], k_max=2)
+    assert len(runs) == 2
