@@ -165,10 +165,13 @@ def pop_bench(lib) -> tuple[dict, list]:
     for ids in (REFERENCE_TEST_IDS, REFERENCE_TEST_IDS_62):
         n = len(ids)
         first = calculate_mpqp_with_HIP(ds, ids, lib=lib)
-        second = calculate_mpqp_with_HIP(ds, ids, lib=lib)
+        det = []
+        second = calculate_mpqp_with_HIP(ds, ids, lib=lib, details=det)
         out[f"{n}"] = {"cumulative_s_first_pass": float(first[:, 2].sum()),
                        "cumulative_s": float(second[:, 2].sum()),
                        "gpu_call_s": float(second[:, 3].sum()),
+                       "split_s": {"mpqp": sum(d.t_mpqp for d in det), "preprocess": sum(d.t_preprocess for d in det),
+                                   "regions": sum(d.t_regions for d in det)},
                        "regions": int(second[:, 1].sum()),
                        "reference_rtx4090_s": POP_REFERENCE_S[n],
                        "speedup_vs_reference": POP_REFERENCE_S[n] / float(second[:, 2].sum())}

@@ -1653,8 +1653,9 @@ int run_regions(const RegionIn &in, const RegionOut &out) {
 
 // mpqp_redundant_rows(): rows scaled to unit norm on the host, then one
 // batch of redundancy LPs (every testable row against all the others) and a
-// sequential pass, in row order, over the rows the batch flagged: each is
-// re-tested against the rows still kept and dropped if still redundant.
+// sequential pass, in row order, over the rows the batch flagged (one
+// launch of one workgroup): each is re-tested against the rows still kept
+// and dropped if still redundant.
 // Rows the batch did not flag are not redundant against any subset either
 // (removing rows only enlarges the polyhedron), so the result equals the
 // sequential test of every row in order.
@@ -1663,7 +1664,7 @@ int run_redundant(const double *G, const double *h, int m, int nz, const int *fi
     if (m <= 0 || nz <= 0 || !G || !h || !keep) throw HipError{"mpqp_redundant_rows: bad arguments"};
     ensure_ready();
     const long long lds = redund_lds_bytes(m, nz);
-    if (lds > C.lds_limit)
+    if (lds + (long long)m * 4 > C.lds_limit)
         throw HipError{"mpqp_redundant_rows: an LP needs " + std::to_string(lds) + " bytes of LDS, more than " +
                        std::to_string(C.lds_limit)};
     std::vector<double> Gs((size_t)m * nz), hs(m);
@@ -1689,8 +1690,11 @@ int run_redundant(const double *G, const double *h, int m, int nz, const int *fi
     HIP_OK(hipMemcpyAsync(dh, hs.data(), (size_t)m * 8, hipMemcpyHostToDevice, st));
     const int max_iter = 50 * (m + 2 * nz + 1) + 1000;
     long long lps = 0;
-    // one launch over `rows`; returns whether each is redundant against the active rows
-    auto launch = [&](const std::vector<int> &rows) {
+    // one launch over `rows`: the batch (seq = false: every row against the
+    // active rows, one workgroup each; returns which are redundant) or the
+    // sequential pass (seq = true: one workgroup, rows in order, `active`
+    // updated on the device and read back)
+    auto launch = [&](const std::vector<int> &rows, bool seq) {
         const int n = (int)rows.size();
         std::vector<char> red(n, 0);
         if (!n) return red;
@@ -1700,12 +1704,17 @@ int run_redundant(const double *G, const double *h, int m, int nz, const int *fi
         BufView ds = C.pools.get(R_INFO, -1, (size_t)n * 8);
         HIP_OK(hipMemcpyAsync(dt.p, rows.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
         RedundArgs ra{m, nz, max_iter, dG, dh, dact, dt.as<int>(), dv.as<double>(), ds.as<int>(), ds.as<int>() + n};
-        hipLaunchKernelGGL(redundancy_kernel<64>, dim3((unsigned)n), dim3(64), (size_t)lds, st, ra);
+        if (seq)
+            hipLaunchKernelGGL(redundancy_seq_kernel<64>, dim3(1), dim3(64), (size_t)(lds + (long long)m * 4), st, ra, n,
+                               dact);
+        else
+            hipLaunchKernelGGL(redundancy_kernel<64>, dim3((unsigned)n), dim3(64), (size_t)lds, st, ra);
         HIP_OK(hipGetLastError());
         std::vector<double> val(n);
         std::vector<int> sts(2 * (size_t)n);
         HIP_OK(hipMemcpyAsync(val.data(), dv.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(sts.data(), ds.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+        if (seq) HIP_OK(hipMemcpyAsync(active.data(), dact, (size_t)m * 4, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         lps += n;
         for (int t = 0; t < n; ++t) {
@@ -1718,12 +1727,11 @@ int run_redundant(const double *G, const double *h, int m, int nz, const int *fi
         }
         return red;
     };
-    const std::vector<char> flag = launch(tests);
-    for (size_t t = 0; t < tests.size(); ++t) {
-        if (!flag[t]) continue;
-        const int i = tests[t];
-        if (launch(std::vector<int>{i})[0]) active[i] = 0;
-    }
+    const std::vector<char> flag = launch(tests, false);
+    std::vector<int> flagged;
+    for (size_t t = 0; t < tests.size(); ++t)
+        if (flag[t]) flagged.push_back(tests[t]);
+    launch(flagged, true);
     int kept = 0;
     for (int j = 0; j < m; ++j) {
         keep[j] = active[j];

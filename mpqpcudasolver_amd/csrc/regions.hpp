@@ -449,13 +449,15 @@ __host__ __device__ inline long long redund_lds_bytes(int m, int nz) {
     return (d * 8 + i * 4 + 15) & ~15LL;
 }
 
+// One redundancy LP of row ti over the rows j with active[j] != 0 (and row
+// ti), in the LDS at smem (redund_lds_bytes).  Returns RED_*; *value is
+// max g_ti z when RED_OPTIMAL.
 template <int NT>
-__global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+__device__ int redund_lp(const RedundArgs &a, const int *active, int ti, unsigned char *smem, double &value,
+                         int &it) {
     __shared__ int sh_mm, sh_rw;
     const int m = a.m, nz = a.nz, K = 2 * nz + 1, ld = K + 1, W = 2 * nz;
     const int tid = threadIdx.x, lane = tid & (MPQP_WAVE - 1);
-    const int t = blockIdx.x, ti = a.tests[t];
     double *T = reinterpret_cast<double *>(smem);  // (m + 1) x ld
     double *colq = T + (m + 1) * ld;               // m + 1
     double *cost = colq + m + 1;                   // K + m
@@ -467,7 +469,7 @@ __global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
         int base = 0;
         for (int r0 = 0; r0 < m; r0 += MPQP_WAVE) {
             const int j = r0 + lane;
-            const bool in = j < m && (a.active[j] != 0 || j == ti);
+            const bool in = j < m && (active[j] != 0 || j == ti);
             const unsigned long long mk = __ballot(in);
             if (in) rowid[base + __popcll(mk & ((1ull << lane) - 1ull))] = j;
             base += __popcll(mk);
@@ -490,7 +492,8 @@ __global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
     // phase 1 over the slack basis: min w
     for (int j = tid; j <= K; j += NT) T[mm * ld + j] = j == W ? 1.0 : 0.0;
     __syncthreads();
-    int it = 0, status = RED_OPTIMAL;
+    it = 0;
+    int status = RED_OPTIMAL;
     const int r0 = lp_most_negative_row<NT>(T, ld, mm, K);
     if (r0 >= 0) {
         lp_pivot<NT>(T, ld, mm, K, r0, W, basis, nonb, colq);
@@ -503,6 +506,7 @@ __global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
         }
         __syncthreads();
         const int rw = sh_rw;
+        __syncthreads();
         if (st == LP_ITER_LIMIT) status = RED_ITER_LIMIT;
         else if (rw >= 0 && T[rw * ld + K] > 1e-9) status = RED_EMPTY;
         else if (rw >= 0) {
@@ -528,9 +532,50 @@ __global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
         const int st = lp_bland<NT>(T, ld, mm, K, basis, nonb, colq, W, a.max_iter, it);
         status = st == LP_OPTIMAL ? RED_OPTIMAL : (st == LP_UNBOUNDED ? RED_UNBOUNDED : RED_ITER_LIMIT);
     }
-    if (tid == 0) {
-        a.value[t] = status == RED_OPTIMAL ? T[mm * ld + K] : 0.0;
-        a.status[t] = status;
+    value = status == RED_OPTIMAL ? T[mm * ld + K] : 0.0;
+    __syncthreads();
+    return status;
+}
+
+// row i is redundant when the maximum stays within h_i (rows of unit norm)
+__device__ inline bool redundant_value(double value, double h) { return value <= h + 1e-9 * fmax(1.0, fabs(h)); }
+
+// the batch: one workgroup per tested row, every test against the same rows
+template <int NT>
+__global__ __launch_bounds__(NT) void redundancy_kernel(RedundArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int t = blockIdx.x;
+    double value = 0.0;
+    int it = 0;
+    const int st = redund_lp<NT>(a, a.active, a.tests[t], smem, value, it);
+    if (threadIdx.x == 0) {
+        a.value[t] = value;
+        a.status[t] = st;
         a.iters[t] = it;
     }
+}
+
+// the sequential pass, one workgroup: the tests in order, each against the
+// rows still active; a redundant row is dropped before the next test.  The
+// mask lives in LDS after the LP's space and is written back to `active`.
+template <int NT>
+__global__ __launch_bounds__(NT) void redundancy_seq_kernel(RedundArgs a, int n_tests, int *active) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    int *act = reinterpret_cast<int *>(smem + redund_lds_bytes(a.m, a.nz));
+    for (int j = threadIdx.x; j < a.m; j += NT) act[j] = a.active[j];
+    __syncthreads();
+    for (int t = 0; t < n_tests; ++t) {
+        const int ti = a.tests[t];
+        double value = 0.0;
+        int it = 0;
+        const int st = redund_lp<NT>(a, act, ti, smem, value, it);
+        if (threadIdx.x == 0) {
+            a.value[t] = value;
+            a.status[t] = st;
+            a.iters[t] = it;
+            if (st == RED_OPTIMAL && redundant_value(value, a.h[ti])) act[ti] = 0;
+        }
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < a.m; j += NT) active[j] = act[j];
 }

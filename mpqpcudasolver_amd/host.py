@@ -258,6 +258,7 @@ class MpqpRun:
     feas_basis: np.ndarray | None = None
     p1_basis: np.ndarray | None = None
     p2_basis: np.ndarray | None = None
+    active_rows: np.ndarray | None = None  # the n_final_lps x n_x rows of GPUResults.active_sets
 
 
 @dataclass
@@ -277,25 +278,31 @@ class RegionBatch:
     cr_b: np.ndarray         # n x m0
     sets: np.ndarray         # n x width, -1 padded
 
-    def region(self, i: int) -> dict | None:
+    def region(self, i: int, k: int | None = None) -> dict | None:
         """Set i as compute_critical_region returns it (utils.py:93-101):
         None unless a region was found; the Chebyshev radius and centre are
-        added under their own keys."""
+        added under their own keys.  Arrays are views into the batch."""
         if self.status[i] != REGION_FOUND:
             return None
-        k = int((self.sets[i] >= 0).sum())
+        if k is None:
+            k = int((self.sets[i] >= 0).sum())
         m = int(self.n_rows[i])
         return {
             "parameter_A": self.parameter_A[i],
-            "parameter_b": self.parameter_b[i].reshape(-1, 1),
+            "parameter_b": self.parameter_b[i, :, None],
             "lagrange_A": self.lagrange_A[i, :k],
-            "lagrange_b": self.lagrange_b[i, :k].reshape(-1, 1),
+            "lagrange_b": self.lagrange_b[i, :k, None],
             "CR_As": self.cr_A[i, :m],
-            "CR_bs": self.cr_b[i, :m].reshape(-1, 1),
-            "active_set": [int(v) for v in self.sets[i, :k]],
+            "CR_bs": self.cr_b[i, :m, None],
+            "active_set": self.sets[i, :k].tolist(),
             "chebyshev_radius": float(self.radius[i]),
-            "chebyshev_center": self.center[i].copy(),
+            "chebyshev_center": self.center[i],
         }
+
+    def regions_found(self) -> list:
+        """region(i) of every set with a region, in set order."""
+        ks = (self.sets >= 0).sum(axis=1).tolist()
+        return [self.region(i, ks[i]) for i in np.flatnonzero(self.status == REGION_FOUND).tolist()]
 
 
 def pad_active_sets(active_sets, width: int) -> np.ndarray:
@@ -502,11 +509,14 @@ class MpqpLib:
         if not res:
             raise RuntimeError(f"mpqp failed: {self.last_error()}")
         r = res.contents
+        rows = (np.ctypeslib.as_array(r.active_sets, shape=(r.n_final_lps, lp.n_x)).copy()
+                if r.n_final_lps > 0 else np.zeros((0, lp.n_x), dtype=np.intc))
         run = MpqpRun(
             n_batches=r.n_batches, n_splits=r.n_splits, n_feasible_lps=r.n_feasible_lps,
             n_final_lps=r.n_final_lps, time_feas=r.time_feas, time_opti=r.time_opti,
             time_GPU=r.time_GPU,
             active_sets=decode_active_sets(r.active_sets, r.n_final_lps, lp.n_x) if decode else None,
+            active_rows=rows,
         )
         if records:
             n = self.lib.mpqp_get_records(None, None, None, None, None, 0)
@@ -616,10 +626,13 @@ def compute_critical_region(A, b, c, H, inv_Q, F, CRA, CRb, active_set, lib: Mpq
 
 def active_sets_to_regions(prog: MPQPProb, all_active_sets, lib: MpqpLib | None = None) -> list:
     """utils.py:128-144 on the device: decodes prog.n_active_sets rows of
-    prog.num_x ints (a flat sequence, as GPUResults.active_sets) or takes a
-    list of sets, and returns the regions found, in set order."""
+    prog.num_x ints (a flat sequence, as GPUResults.active_sets), or takes
+    those rows as a 2-D array or a list of sets, and returns the regions
+    found, in set order."""
     lib = lib or load_library()
-    if isinstance(all_active_sets, (list, tuple)) and (not all_active_sets or not np.isscalar(all_active_sets[0])):
+    if isinstance(all_active_sets, np.ndarray) and all_active_sets.ndim == 2:
+        sets = pad_active_sets(all_active_sets, prog.num_x)  # rows as GPUResults holds them
+    elif isinstance(all_active_sets, (list, tuple)) and (not all_active_sets or not np.isscalar(all_active_sets[0])):
         sets = pad_active_sets(all_active_sets, prog.num_x)
     else:
         n = prog.n_active_sets
@@ -630,4 +643,4 @@ def active_sets_to_regions(prog: MPQPProb, all_active_sets, lib: MpqpLib | None 
         ended = np.cumsum(sets == -1, axis=1) > 0
         sets = np.where(ended, -1, sets).astype(np.intc)
     batch = lib.regions(prog, sets, width=prog.num_x)
-    return [r for r in (batch.region(i) for i in range(sets.shape[0])) if r is not None]
+    return batch.regions_found()

@@ -75,6 +75,9 @@ class PopResult:
     n_cp: int
     active_sets: list
     regions: list
+    t_mpqp: float = 0.0        # the mpqp() call (wall)
+    t_preprocess: float = 0.0  # the second process_constraints inside the timed span
+    t_regions: float = 0.0     # active_sets_to_regions
 
 
 def solve_pop_problem(problem, lib: MpqpLib, prob_id: int = -1, reserved_memory: int = 0) -> PopResult:
@@ -88,15 +91,18 @@ def solve_pop_problem(problem, lib: MpqpLib, prob_id: int = -1, reserved_memory:
     num_x = lp.n_x
     t0 = time.time()
     run = lib.run(lp, k_max=num_x, k_min=0, reserved_memory=reserved_memory)
+    t1 = time.time()
     prog_copy.process_constraints(scale_method=2, lib=lib)
     prog_copy.num_x = num_x
     prog_copy.n_active_sets = run.n_final_lps
     prog.n_active_sets = run.n_final_lps
-    regions = active_sets_to_regions(prog_copy, run.active_sets, lib)
-    total = time.time() - t0
-    return PopResult(prob_id=prob_id, n_regions=len(regions), total_time=total, time_GPU=run.time_GPU,
+    t2 = time.time()
+    regions = active_sets_to_regions(prog_copy, run.active_rows, lib)
+    t3 = time.time()
+    return PopResult(prob_id=prob_id, n_regions=len(regions), total_time=t3 - t0, time_GPU=run.time_GPU,
                      n_final=run.n_final_lps, n_c=lp.n_c, n_x=lp.n_x, n_t=lp.n_t, n_cp=lp.n_cp,
-                     active_sets=run.active_sets, regions=regions)
+                     active_sets=run.active_sets, regions=regions, t_mpqp=t1 - t0, t_preprocess=t2 - t1,
+                     t_regions=t3 - t2)
 
 
 def calculate_mpqp_with_HIP(data_set, test_ids, lib: MpqpLib | None = None, details: list | None = None) -> np.ndarray:

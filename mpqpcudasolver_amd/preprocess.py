@@ -63,12 +63,14 @@ def find_implicit_equalities(A, b, F, equality_indices, tol: float = 1e-10):
     Hn = H / nrm
     eq = list(equality_indices)
     drop = set()
+    # opposite[i, j]: rows i and j cancel (all pairs at once)
+    opposite = np.abs(Hn[:, None, :] + Hn[None, :, :]).max(axis=2, initial=0.0) <= tol
     for i in range(m):
         if i in drop:
             continue
-        for j in range(i + 1, m):
-            if j not in drop and np.abs(Hn[i] + Hn[j]).max() <= tol:
-                drop.add(j)
+        for j in np.flatnonzero(opposite[i, i + 1:]) + i + 1:
+            if j not in drop:
+                drop.add(int(j))
                 if i not in eq:
                     eq.append(i)
     keep = [i for i in range(m) if i not in drop]
