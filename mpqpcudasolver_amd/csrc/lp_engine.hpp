@@ -97,6 +97,7 @@ enum { ST_NOT_RUN = 0, ST_OPTIMAL = 1, ST_UNBOUNDED = 2, ST_ITER_CAP = 3, ST_STA
 template <typename Real, bool SPLIT>
 struct TabRef {
     typedef Real real_t;
+    static constexpr bool split = SPLIT;
     Real *lo, *hi;
     int R, M, ld;
     __device__ __forceinline__ Real *row(int i) const {
@@ -502,7 +503,7 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
 
 // Three-phase form of the same pivot (column copy; division of row r and
 // zeroing of column q, while the last thread runs `between`; update), four
-// rows in flight per wave.
+// rows in flight per wave (eight for a split tableau).
 template <int NT, typename TB, typename F>
 __device__ __forceinline__ void pivot_rows3(const TB &T, int nrows, int ncols, int r, int q,
                                             typename TB::real_t *colq, F between) {
@@ -520,7 +521,7 @@ __device__ __forceinline__ void pivot_rows3(const TB &T, int nrows, int ncols, i
     if (tid == NT - 1) between();
     __syncthreads();
     constexpr int NW = NT / MPQP_WAVE;
-    constexpr int U = 4;
+    constexpr int U = TB::split ? 8 : 4;
     const int lane = tid & (MPQP_WAVE - 1);
     const int wave = tid / MPQP_WAVE;
     for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
@@ -640,9 +641,11 @@ __device__ __forceinline__ void pivot_rows(const TB &T, int nrows, int ncols, in
     }
     // (B) on column pairs: lane c of a wave step holds columns 2c, 2c + 1
     // as one 8-byte vector (ld even, T 8-byte aligned; a pair that runs past
-    // ncols updates a padding column of the row, which nothing reads)
+    // ncols updates a padding column of the row, which nothing reads).  A
+    // split tableau keeps eight rows in flight per wave (its global rows
+    // wait on L2), an LDS one four.
     constexpr int NW = NT / MPQP_WAVE;
-    constexpr int U = 4;
+    constexpr int U = TB::split ? 8 : 4;
     const int lane = fresh_lane();
     const int wave = tid / MPQP_WAVE;
     const int npair = (ncols + 1) >> 1;
