@@ -56,12 +56,12 @@ struct RegionArgs {
     DevRegion *info;
 };
 
-// LDS of one region workgroup: G_SS, [LA | Lb], the half-spaces, the LP
-// tableau ((n_c + n_cp + 1) rows; columns theta+, theta-, r+, r-, rhs), the
-// pivot column and the index arrays.
+// LDS of one region workgroup: G_SS, [LA | Lb], the LP tableau
+// ((n_c + n_cp + 1) rows; columns theta+, theta-, r+, r-, rhs), the pivot
+// column and the index arrays.
 __host__ __device__ inline long long region_lds_bytes(int n_c, int n_t, int n_cp, int kw) {
     const long long m0 = n_c + n_cp, K = 2LL * n_t + 2, ld = K + 1;
-    const long long d = (long long)kw * kw + (long long)kw * (n_t + 1) + m0 * (n_t + 1) + (m0 + 1) * ld + (m0 + 1);
+    const long long d = (long long)kw * kw + (long long)kw * (n_t + 1) + (m0 + 1) * ld + (m0 + 1);
     const long long i = m0 + K + m0;
     return (d * 8 + i * 4 + 15) & ~15LL;
 }
@@ -224,8 +224,7 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
     const int *S = a.sets + c * a.width;
     double *Lg = reinterpret_cast<double *>(smem);  // kw x kw: G_SS, then its Cholesky factor
     double *Lr = Lg + kw * kw;                      // kw x P1: [LA | Lb]
-    double *R = Lr + kw * P1;                       // m0 x P1: half-spaces [a | b]
-    double *T = R + m0 * P1;                        // (m0 + 1) x ld: LP tableau
+    double *T = Lr + kw * P1;                       // (m0 + 1) x ld: LP tableau
     double *colq = T + (m0 + 1) * ld;               // m0 + 1
     int *keep = reinterpret_cast<int *>(colq + m0 + 1);  // m0: kept half-spaces, ascending
     int *nonb = keep + m0;                               // K: variable of each column
@@ -302,7 +301,8 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
         else a.par_b[c * n_x + i] = v;
     }
     // half-spaces: lambda rows (in the order of S), the constraints not in
-    // S (ascending), the rows of A_t
+    // S (ascending), the rows of A_t; row rho of the LP tableau first holds
+    // its theta coefficients (columns [0, p)) and its bound (column K)
     for (int e = tid; e < m0 * P1; e += NT) {
         const int rho = e / P1, t = e - rho * P1;
         double v;
@@ -324,27 +324,53 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
             const int q = rho - n_c;
             v = t < p ? a.CRA[(long long)q * p + t] : a.CRb[q];
         }
-        R[e] = v;
+        T[rho * ld + (t < p ? t : K)] = v;
     }
     __syncthreads();
-    // keep every row of A_t and the other rows with a nonzero theta
-    // coefficient, in order (wave 0 compacts)
+    // Every row of A_t is kept, the other rows when a theta coefficient is
+    // nonzero.  A kept row becomes the LP row theta+ (a), theta- (-a), r+
+    // (1), r- (-1) | b, scaled to unit norm; a dropped row becomes all zero
+    // with rhs 1: it never enters a ratio test and no pivot changes it, and
+    // the slacks keep their order, so the pivots are those of the LP over
+    // the kept rows alone.
+    for (int rho = tid; rho < m0; rho += NT) {
+        double *Ti = T + rho * ld;
+        bool nz = rho >= n_c;
+        for (int t = 0; t < p; ++t) nz |= Ti[t] != 0.0;
+        if (nz) {
+            double nrm = 0.0;
+            for (int t = 0; t < p; ++t) nrm = fma(Ti[t], Ti[t], nrm);
+            nrm = sqrt(nrm);
+            for (int t = 0; t < p; ++t) {
+                const double v = Ti[t] / nrm;
+                Ti[t] = v;
+                Ti[p + t] = -v;
+            }
+            Ti[RP] = 1.0;
+            Ti[RM] = -1.0;
+            Ti[K] = Ti[K] / nrm;
+        } else {
+            for (int t = 0; t < K; ++t) Ti[t] = 0.0;
+            Ti[K] = 1.0;
+        }
+        basis[rho] = K + rho;
+    }
+    __syncthreads();
+    // the kept rows, in order (wave 0 compacts; r+ is 1 exactly in them)
     if (tid < MPQP_WAVE) {
         int base = 0;
         for (int r0 = 0; r0 < m0; r0 += MPQP_WAVE) {
             const int rho = r0 + lane;
-            bool nz = rho >= n_c && rho < m0;
-            if (rho < n_c)
-                for (int t = 0; t < p; ++t) nz |= R[rho * P1 + t] != 0.0;
-            const unsigned long long mk = __ballot(nz);
-            if (nz) keep[base + __popcll(mk & ((1ull << lane) - 1ull))] = rho;
+            const bool kept = rho < m0 && T[rho * ld + RP] != 0.0;
+            const unsigned long long mk = __ballot(kept);
+            if (kept) keep[base + __popcll(mk & ((1ull << lane) - 1ull))] = rho;
             base += __popcll(mk);
         }
         if (lane == 0) sh_m = base;
     }
     __syncthreads();
     const int m = sh_m;
-    // scaled rows out; LP rows: theta+ (a), theta- (-a), r+ (1), r- (-1) | b
+    // the scaled half-spaces out, kept rows first
     for (int i = tid; i < m0; i += NT) {
         double *out = a.cr_A + (c * m0 + i) * p;
         if (i >= m) {
@@ -352,26 +378,12 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
             a.cr_b[c * m0 + i] = 0.0;
             continue;
         }
-        const double *ri = R + keep[i] * P1;
-        double nrm = 0.0;
-        for (int t = 0; t < p; ++t) nrm = fma(ri[t], ri[t], nrm);
-        nrm = sqrt(nrm);
-        double *Ti = T + i * ld;
-        for (int t = 0; t < p; ++t) {
-            const double v = ri[t] / nrm;
-            out[t] = v;
-            Ti[t] = v;
-            Ti[p + t] = -v;
-        }
-        const double bv = ri[p] / nrm;
-        a.cr_b[c * m0 + i] = bv;
-        Ti[RP] = 1.0;
-        Ti[RM] = -1.0;
-        Ti[K] = bv;
-        basis[i] = K + i;
+        const double *Ti = T + keep[i] * ld;
+        for (int t = 0; t < p; ++t) out[t] = Ti[t];
+        a.cr_b[c * m0 + i] = Ti[K];
     }
     // cost row of min -r+ + r- over the slack basis
-    for (int j = tid; j <= K; j += NT) T[m * ld + j] = j == RP ? -1.0 : (j == RM ? 1.0 : 0.0);
+    for (int j = tid; j <= K; j += NT) T[m0 * ld + j] = j == RP ? -1.0 : (j == RM ? 1.0 : 0.0);
     for (int j = tid; j < K; j += NT) nonb[j] = j;
     __syncthreads();
 
@@ -382,16 +394,16 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
         status = MPQP_REGION_UNBOUNDED;
     } else {
         // r- enters at the row of the most negative b_i
-        const int r0 = lp_most_negative_row<NT>(T, ld, m, K);
-        if (r0 >= 0) lp_pivot<NT>(T, ld, m, K, r0, RM, basis, nonb, colq);
-        const int st = lp_bland<NT>(T, ld, m, K, basis, nonb, colq, -1, a.max_iter, it);
+        const int r0 = lp_most_negative_row<NT>(T, ld, m0, K);
+        if (r0 >= 0) lp_pivot<NT>(T, ld, m0, K, r0, RM, basis, nonb, colq);
+        const int st = lp_bland<NT>(T, ld, m0, K, basis, nonb, colq, -1, a.max_iter, it);
         status = st == LP_OPTIMAL ? OPTIMAL : (st == LP_UNBOUNDED ? MPQP_REGION_UNBOUNDED : MPQP_REGION_ITER_LIMIT);
     }
     double radius = 0.0;
     if (status == OPTIMAL) {
         if (tid == 0) {
             double rp = 0.0, rm = 0.0;
-            for (int i = 0; i < m; ++i) {
+            for (int i = 0; i < m0; ++i) {
                 if (basis[i] == RP) rp = T[i * ld + K];
                 if (basis[i] == RM) rm = T[i * ld + K];
             }
@@ -399,7 +411,7 @@ __global__ __launch_bounds__(NT) void region_kernel(RegionArgs a) {
         }
         for (int t = tid; t < p; t += NT) {
             double tp = 0.0, tm = 0.0;
-            for (int i = 0; i < m; ++i) {
+            for (int i = 0; i < m0; ++i) {
                 if (basis[i] == t) tp = T[i * ld + K];
                 if (basis[i] == p + t) tm = T[i * ld + K];
             }
