@@ -101,10 +101,36 @@ __global__ __launch_bounds__(64) void gemm_f64_mfma(const double *A, int lda, co
 // ---------------------------------------------------------------------
 enum { LP_OPTIMAL = 0, LP_UNBOUNDED = 1, LP_ITER_LIMIT = 2 };
 
+// Pivot on (r, q) over the m + 1 rows and K + 1 columns, with the entry
+// arithmetic of pivot_team (row r divided by p, 1 / p at q; every other
+// entry fma(-T[i][q], T'[r][j], T[i][j]), column q from 0; a column whose
+// divided pivot-row entry is zero left as it is).  These tableaus are
+// narrower than a wave, so the (row, column) items of the update are spread
+// over all threads, item e = i * (K + 1) + j.
 template <int NT>
 __device__ void lp_pivot(double *T, int ld, int m, int K, int r, int q, int *basis, int *nonb, double *colq) {
-    pivot_team<NT, double>(T, ld, m + 1, K + 1, r, q, colq);
-    if (threadIdx.x == 0) {
+    const int tid = threadIdx.x, nr = m + 1, nc = K + 1;
+    for (int i = tid; i < nr; i += NT) colq[i] = T[i * ld + q];
+    __syncthreads();
+    const double p = colq[r];
+    double *tr = T + r * ld;
+    for (int j = tid; j < nc; j += NT)
+        if (j != q) tr[j] = tr[j] / p;
+    if (tid == 0) tr[q] = 1.0 / p;
+    __syncthreads();
+    const int dq = NT / nc, dj = NT - dq * nc;
+    int i = tid / nc, j = tid - (tid / nc) * nc;
+    for (; i < nr; ) {
+        const double prj = tr[j];
+        if (i != r && prj != 0.0) T[i * ld + j] = fma(-colq[i], prj, j == q ? 0.0 : T[i * ld + j]);
+        j += dj;
+        i += dq;
+        if (j >= nc) {
+            j -= nc;
+            ++i;
+        }
+    }
+    if (tid == 0) {
         const int v = basis[r];
         basis[r] = nonb[q];
         nonb[q] = v;
