@@ -180,6 +180,30 @@ def pop_bench(lib) -> tuple[dict, list]:
     return out, ds
 
 
+def pop_all_bench(lib, precision: str, batching: str) -> dict:
+    """Every POP_mpQP1 problem through the driver, one pass: cumulative
+    total_time, candidates, regions, and the largest problems."""
+    from mpqpcudasolver_amd.pop import ALL_IDS_ORDERED, load_pop_npz, solve_pop_problem
+
+    ds = load_pop_npz(POP_NPZ)
+    lib.set_options(precision=1 if precision == "mixed" else 0, batching=1 if batching == "persistent" else 0)
+    rows = []
+    for pid in ALL_IDS_ORDERED:
+        r = solve_pop_problem(ds[pid], lib, pid)
+        cand = lib.stats()["candidates"]
+        r.regions = r.batch = None
+        rows.append({"id": pid, "candidates": cand, "final": r.n_final, "regions": r.n_regions,
+                     "total_s": r.total_time, "gpu_s": r.time_GPU})
+        print(f"# POP {pid}: {cand} candidates, {r.n_regions} regions, {r.total_time:.3f} s", file=sys.stderr,
+              flush=True)
+    lib.set_options()
+    big = sorted(rows, key=lambda x: -x["total_s"])[:5]
+    return {"workload": "all 100 POP_mpQP1 problems end to end, k_max = num_x", "variant": f"{precision}-{batching}",
+            "problems": len(rows), "cumulative_s": sum(x["total_s"] for x in rows),
+            "candidates": sum(x["candidates"] for x in rows), "regions": sum(x["regions"] for x in rows),
+            "largest": big}
+
+
 def cpu_pop(ds, ids) -> dict:
     """The oracle through the same driver on the 43-problem list."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -370,6 +394,8 @@ def main() -> int:
     ap.add_argument("--restart-pct", type=int, default=25,
                     help="mixed precision: FP64 restart when more than this %% of the level-0/1 optimality "
                          "LPs fail certification (100: never)")
+    ap.add_argument("--pop-all", action="store_true",
+                    help="only run all 100 POP_mpQP1 problems end to end (k_max = num_x) and exit; minutes")
     ap.add_argument("--regions-only", action="store_true",
                     help="only time mpqp_regions() on the final sets of --config and exit (profiler runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -397,6 +423,9 @@ def main() -> int:
         raise RuntimeError(lib.last_error())
     if args.regions_only:
         print(json.dumps(regions_bench(lib, args.config, args.steps)[0]), flush=True)
+        return 0
+    if args.pop_all:
+        print(json.dumps(pop_all_bench(lib, args.precision, args.batching)), flush=True)
         return 0
     mixed = args.precision == "mixed"
     h = measure(lib, args.config, args.steps, args.warmup, mixed, args.batching, not args.no_compare,

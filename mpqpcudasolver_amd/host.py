@@ -437,9 +437,11 @@ class MpqpLib:
             opt(cA), opt(cb))
         if rc < 0:
             raise RuntimeError(f"mpqp_regions failed: {self.last_error()}")
-        return RegionBatch(status=info["status"].copy(), n_rows=info["n_rows"].copy(), radius=info["radius"].copy(),
-                           iters=info["iters"].copy(), center=center, parameter_A=pA, parameter_b=pb,
-                           lagrange_A=lA, lagrange_b=lb, cr_A=cA, cr_b=cb, sets=sets)
+        batch = RegionBatch(status=info["status"].copy(), n_rows=info["n_rows"].copy(), radius=info["radius"].copy(),
+                            iters=info["iters"].copy(), center=center, parameter_A=pA, parameter_b=pb,
+                            lagrange_A=lA, lagrange_b=lb, cr_A=cA, cr_b=cb, sets=sets)
+        self.last_region_batch = batch
+        return batch
 
     def redundant_rows(self, G, h, fixed=None) -> np.ndarray:
         """mpqp_redundant_rows(): keep mask (int32) of the rows of G z <= h;

@@ -27,7 +27,7 @@ from .host import MpqpLib, active_sets_to_regions, load_library
 from .problem import MPQPProb, assemble_lps
 
 __all__ = ["load_pop_mpqp1", "load_pop_npz", "problem_fields", "PopResult", "solve_pop_problem",
-           "calculate_mpqp_with_HIP", "REFERENCE_TEST_IDS", "REFERENCE_TEST_IDS_62"]
+           "calculate_mpqp_with_HIP", "REFERENCE_TEST_IDS", "REFERENCE_TEST_IDS_62", "ALL_IDS_ORDERED"]
 
 FIELDS = ("A", "b", "c", "F", "Q", "Ht", "CRA", "CRb")
 
@@ -39,6 +39,12 @@ REFERENCE_TEST_IDS = [38, 51, 80, 41, 30, 92, 88, 99, 70, 63, 66, 93, 53, 72, 68
 # the longer list of main.py:218 (the first 62 problems in ascending solve time)
 REFERENCE_TEST_IDS_62 = REFERENCE_TEST_IDS + [26, 65, 95, 10, 83, 43, 85, 42, 44, 11, 16, 24, 25, 69, 36, 52, 62, 12,
                                               81]
+
+
+# every problem of the set: the reference's ascending-time order (main.py:216,
+# which names problem 31 twice and problem 32 never), then problem 32
+ALL_IDS_ORDERED = REFERENCE_TEST_IDS_62 + [28, 96, 6, 75, 82, 37, 60, 2, 98, 48, 0, 40, 86, 77, 22, 17, 27, 61, 90, 76,
+                                           31, 5, 4, 39, 29, 7, 46, 87, 23, 59, 50, 14, 78, 18, 35, 94, 1, 32]
 
 
 def load_pop_npz(path: str) -> list:
@@ -78,6 +84,7 @@ class PopResult:
     t_mpqp: float = 0.0        # the mpqp() call (wall)
     t_preprocess: float = 0.0  # the second process_constraints inside the timed span
     t_regions: float = 0.0     # active_sets_to_regions
+    batch: object = None       # the RegionBatch behind `regions`
 
 
 def solve_pop_problem(problem, lib: MpqpLib, prob_id: int = -1, reserved_memory: int = 0) -> PopResult:
@@ -99,10 +106,11 @@ def solve_pop_problem(problem, lib: MpqpLib, prob_id: int = -1, reserved_memory:
     t2 = time.time()
     regions = active_sets_to_regions(prog_copy, run.active_rows, lib)
     t3 = time.time()
+    batch = getattr(lib, "last_region_batch", None)
     return PopResult(prob_id=prob_id, n_regions=len(regions), total_time=t3 - t0, time_GPU=run.time_GPU,
                      n_final=run.n_final_lps, n_c=lp.n_c, n_x=lp.n_x, n_t=lp.n_t, n_cp=lp.n_cp,
                      active_sets=run.active_sets, regions=regions, t_mpqp=t1 - t0, t_preprocess=t2 - t1,
-                     t_regions=t3 - t2)
+                     t_regions=t3 - t2, batch=batch)
 
 
 def calculate_mpqp_with_HIP(data_set, test_ids, lib: MpqpLib | None = None, details: list | None = None) -> np.ndarray:

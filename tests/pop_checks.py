@@ -106,3 +106,55 @@ def check_coverage(f, regions, n_samples=40, seed=0, min_cover=0.97):
     assert feas > 0
     assert cov >= min_cover * feas, (cov, feas)
     return feas, cov
+
+
+def check_coverage_batch(f, batch, n_samples=20, seed=0, min_cover=0.97):
+    """check_coverage over a RegionBatch, the region tests vectorized (for
+    problems with hundreds of thousands of regions)."""
+    from mpqpcudasolver_amd.host import REGION_FOUND
+    rng = np.random.default_rng(seed)
+    CRA, CRb = f["CRA"], f["CRb"].ravel()
+    p = CRA.shape[1]
+    lo, hi = np.full(p, -1e3), np.full(p, 1e3)
+    for t in range(p):
+        for s in (1.0, -1.0):
+            e = np.zeros(p)
+            e[t] = -s
+            r = linprog(e, A_ub=CRA, b_ub=CRb, bounds=[(None, None)] * p, method="highs")
+            if r.status == 0:
+                (hi if s > 0 else lo)[t] = r.x[t]
+    A, b, c, F, Q, H = f["A"], f["b"].ravel(), f["c"].ravel(), f["F"], f["Q"], f["Ht"]
+    found = np.flatnonzero(batch.status == REGION_FOUND)
+    cA, cb = batch.cr_A[found], batch.cr_b[found]
+    valid = np.arange(cA.shape[1])[None, :] < batch.n_rows[found][:, None]
+    feas = cov = tries = 0
+    while feas < n_samples and tries < 20 * n_samples:
+        tries += 1
+        if tries % 2:
+            # a point near the centre of a random region: inside the feasible
+            # parameter set, and often across into a neighbouring region
+            i = found[rng.integers(found.size)]
+            u = rng.standard_normal(p)
+            th = batch.center[i] + 1.5 * batch.radius[i] * u / np.linalg.norm(u)
+        else:
+            th = lo + (hi - lo) * rng.random(p)
+        if np.any(CRA @ th > CRb):
+            continue
+        ok, x_qp, f_qp = qp_at(f, th)
+        inside = np.all(((cA @ th) <= cb + 1e-9) | ~valid, axis=1)
+        idx = found[inside]
+        if not ok:
+            assert idx.size == 0, "a region covers a parameter with an infeasible QP"
+            continue
+        feas += 1
+        if idx.size == 0:
+            continue
+        cov += 1
+        for i in idx[:8]:
+            x = batch.parameter_A[i] @ th + batch.parameter_b[i]
+            assert np.max(A @ x - b - F @ th) <= 1e-6 * max(1.0, np.abs(b).max()), "x(theta) infeasible"
+            fx = 0.5 * x @ Q @ x + (c + H @ th) @ x
+            assert fx <= f_qp + 1e-6 * max(1.0, abs(f_qp)), ("x(theta) worse than the QP optimum", fx, f_qp)
+    assert feas > 0
+    assert cov >= min_cover * feas, (cov, feas)
+    return feas, cov

@@ -86,3 +86,16 @@ def test_reference_driver_table(lib, pop):
     assert list(res[:, 0].astype(int)) == REFERENCE_TEST_IDS[:12]
     assert np.all(res[:, 1] == [d.n_regions for d in details]) and np.all(res[:, 1] > 0)
     assert np.all(res[:, 2] >= res[:, 3])
+
+
+def test_pop_problem50_full_depth(lib, pop):
+    """POP problem 50 (56 constraints, 7 variables) at k_max = num_x: 51 M
+    candidates, levels far larger than any single allocation the old
+    per-level buffers allowed; the traversal chunks them to the device
+    memory.  Its ~970 k regions are checked against QP solutions."""
+    print("case: POP problem 50, k_max = num_x", flush=True)
+    r = solve_pop_problem(pop[50], lib, 50)
+    st = lib.stats()
+    assert st["candidates"] > 40_000_000
+    assert 0.99 * r.n_final <= r.n_regions <= r.n_final
+    pop_checks.check_coverage_batch(problem_fields(pop[50]), r.batch, n_samples=20, seed=50)
