@@ -159,9 +159,9 @@ def pop_bench(lib) -> tuple[dict, list]:
     from mpqpcudasolver_amd.pop import REFERENCE_TEST_IDS, REFERENCE_TEST_IDS_62, calculate_mpqp_with_HIP, load_pop_npz
 
     ds = load_pop_npz(POP_NPZ)
-    lib.set_options()
+    lib.set_options(batching=1)
     out = {"workload": "POP_mpQP1 end to end (preprocessing, mpqp with k_max=num_x, critical regions), "
-                       "default options (FP64, plain launches)"}
+                       "FP64 with persistent launches (records identical to plain launches)"}
     for ids in (REFERENCE_TEST_IDS, REFERENCE_TEST_IDS_62):
         n = len(ids)
         first = calculate_mpqp_with_HIP(ds, ids, lib=lib)
@@ -182,7 +182,9 @@ def pop_bench(lib) -> tuple[dict, list]:
 
 def pop_all_bench(lib, precision: str, batching: str) -> dict:
     """Every POP_mpQP1 problem through the driver, one pass: cumulative
-    total_time, candidates, regions, and the largest problems."""
+    total_time, candidates, regions, and the largest problems.  FP64 unless
+    --precision mixed is given explicitly (mixed is not exact on these
+    badly scaled problems)."""
     from mpqpcudasolver_amd.pop import ALL_IDS_ORDERED, load_pop_npz, solve_pop_problem
 
     ds = load_pop_npz(POP_NPZ)
@@ -425,7 +427,8 @@ def main() -> int:
         print(json.dumps(regions_bench(lib, args.config, args.steps)[0]), flush=True)
         return 0
     if args.pop_all:
-        print(json.dumps(pop_all_bench(lib, args.precision, args.batching)), flush=True)
+        prec = "mixed" if "--precision" in sys.argv and args.precision == "mixed" else "fp64"
+        print(json.dumps(pop_all_bench(lib, prec, args.batching)), flush=True)
         return 0
     mixed = args.precision == "mixed"
     h = measure(lib, args.config, args.steps, args.warmup, mixed, args.batching, not args.no_compare,

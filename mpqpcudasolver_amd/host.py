@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+from collections.abc import Sequence
 from dataclasses import dataclass
 
 import numpy as np
@@ -36,6 +37,7 @@ __all__ = [
     "MpqpRegionInfo",
     "MpqpRegionStats",
     "RegionBatch",
+    "RegionList",
     "REGION_STATUS_NAMES",
     "pad_active_sets",
     "compute_critical_region",
@@ -299,10 +301,35 @@ class RegionBatch:
             "chebyshev_center": self.center[i],
         }
 
-    def regions_found(self) -> list:
-        """region(i) of every set with a region, in set order."""
-        ks = (self.sets >= 0).sum(axis=1).tolist()
-        return [self.region(i, ks[i]) for i in np.flatnonzero(self.status == REGION_FOUND).tolist()]
+    def regions_found(self) -> "RegionList":
+        """region(i) of every set with a region, in set order, as a lazy
+        sequence (each dict is built when it is accessed)."""
+        return RegionList(self)
+
+
+class RegionList(Sequence):
+    """The regions of a RegionBatch as the list active_sets_to_regions
+    returns in the reference (utils.py:140-144): len(), indexing, slicing
+    and iteration, each region's dict built on access (a batch can hold
+    millions of regions)."""
+
+    def __init__(self, batch: RegionBatch):
+        self.batch = batch
+        self.idx = np.flatnonzero(batch.status == REGION_FOUND)
+        self.k = (batch.sets >= 0).sum(axis=1)
+
+    def __len__(self) -> int:
+        return int(self.idx.size)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        c = int(self.idx[i])
+        return self.batch.region(c, int(self.k[c]))
 
 
 def pad_active_sets(active_sets, width: int) -> np.ndarray:
@@ -426,7 +453,9 @@ class MpqpLib:
         A, b, c, H, iQ, F = f64(prog.A), f64(prog.b).ravel(), f64(prog.c).ravel(), f64(prog.H), f64(prog.inv_Q), f64(prog.F)
         CRA, CRb = f64(prog.A_t), f64(prog.b_t).ravel()
         info = np.zeros(n, dtype=REGION_INFO_DTYPE)
-        z = lambda *shape: np.zeros(shape, dtype=np.float64) if outputs else None
+        # the kernel writes every entry of its outputs (rows past a set's
+        # size or kept half-spaces as zeros), so the arrays need no fill
+        z = lambda *shape: np.empty(shape, dtype=np.float64) if outputs else None
         center, pA, pb = z(n, n_t), z(n, n_x, n_t), z(n, n_x)
         lA, lb, cA, cb = z(n, width, n_t), z(n, width), z(n, m0, n_t), z(n, m0)
         opt = lambda a: _ptr(a, _dp) if a is not None else None

@@ -23,7 +23,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .host import MpqpLib, active_sets_to_regions, load_library
+from .host import MPQP_PERSISTENT, MpqpLib, active_sets_to_regions, load_library
 from .problem import MPQPProb, assemble_lps
 
 __all__ = ["load_pop_mpqp1", "load_pop_npz", "problem_fields", "PopResult", "solve_pop_problem",
@@ -116,8 +116,12 @@ def solve_pop_problem(problem, lib: MpqpLib, prob_id: int = -1, reserved_memory:
 def calculate_mpqp_with_HIP(data_set, test_ids, lib: MpqpLib | None = None, details: list | None = None) -> np.ndarray:
     """main.py:11-173 on the device: one row [prob_id, n_regions, total_time,
     time_GPU] per problem of `test_ids`.  `details`, if given, receives each
-    problem's PopResult."""
-    lib = lib or load_library()
+    problem's PopResult.  A library passed in is used with its options; the
+    default one runs FP64 with persistent launches (the records of FP64 plain
+    launches, faster on large problems)."""
+    if lib is None:
+        lib = load_library()
+        lib.set_options(batching=MPQP_PERSISTENT)
     if lib.warm_up() != 0:
         raise RuntimeError(lib.last_error())
     reserved_memory = 0
