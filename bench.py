@@ -27,7 +27,11 @@ pivot counts of the FP64 path (bit-identical to the CPU restatement's) times
 2(M+1)(K+1)s + (M+K+2)s bytes per pivot at the kernel's element size s,
 divided by the kernel's time from HIP events on the library's stream.  Its
 bound is the LDS bandwidth measured on the device when the tableau lives in
-LDS, and HBM (8 TB/s) when it does not fit there and lives in global memory.
+LDS.  When it does not fit there, its first R rows stay in LDS and the
+others live in global memory, where the pivots update them in batches of
+lazy_batch; the bound is then the algorithmic bytes over the larger of the
+LDS time of R + 1 rows per pivot and the HBM (8 TB/s) time of the other rows
+once per batch.
 cpu_baseline is the CPU restatement of the same algorithm (FP64, OpenMP over
 the candidates of a level) on a bounded sample of the workload, best of 3.
 
@@ -263,6 +267,7 @@ def roofline(st: dict, st64: dict | None, lps, lds_peak: float, fp32: bool) -> d
     took longer in the last step.  Algorithmic bytes use the FP64 path's
     pivot counts (st64, the plain FP64 run of the same workload) when given."""
     ms_opt = st["ms_opt"] - st["ms_fallback"]
+    split = None
     if st64 is None and "per_problem_pivots" not in st:
         return {"bound": None, "kernel": None, "note": "no pivot counts for this workload without --compare"}
     if ms_opt >= st["ms_feas"]:
@@ -277,6 +282,19 @@ def roofline(st: dict, st64: dict | None, lps, lds_peak: float, fp32: bool) -> d
             flops += 2.0 * (lp.M2 + 1) * (p1 * (lp.K2 + 1) + p2 * (kp2 + 1))
         ms = ms_opt
         in_global = bool(st["tableau_global"] & (2 if fp32 else 1))
+        if in_global and st.get("lazy_batch", 0) > 0:
+            # a split tableau with deferred updates: per pivot its R + 1 LDS
+            # rows move through LDS, its M - R global rows through HBM once
+            # per lazy_batch pivots; the bound is the slower of the two
+            R = st["split_rows_fp32" if fp32 else "split_rows_fp64"]
+            kb = st["lazy_batch"]
+            t_lds = t_hbm = 0.0
+            for lp, (p1, p2) in zip(lps, (st64 or st)["per_problem_pivots"]):
+                kp2 = lp.N2 - lp.n_eqs - lp.M2
+                for cnt, kk in ((p1, lp.K2), (p2, kp2)):
+                    t_lds += cnt * 2.0 * (R + 1) * (kk + 1) * s / lds_peak
+                    t_hbm += cnt * 2.0 * (lp.M2 - R) * (kk + 1) * s / kb / HBM_PEAK
+            split = {"rows_lds": R, "lazy_batch": kb, "t_lds_s": t_lds, "t_hbm_s": t_hbm}
     else:
         name = "feas_kernel"
         launches, ms = st["launches_feas"], st["ms_feas"]
@@ -288,9 +306,24 @@ def roofline(st: dict, st64: dict | None, lps, lds_peak: float, fp32: bool) -> d
     launches = max(launches, 1)
     avg_s = ms / launches * 1e-3
     achieved = byts / launches / avg_s / 1e9 if avg_s > 0 else 0.0
-    peak = HBM_PEAK if in_global else lds_peak
+    if split:
+        # the rate at which this schedule could touch the algorithmic bytes
+        # with LDS and HBM both at their peaks
+        peak = byts / max(split["t_lds_s"], split["t_hbm_s"])
+        bound = "lds+hbm"
+        note = (f"deferred updates: per pivot {split['rows_lds'] + 1} LDS rows through LDS "
+                f"({lds_peak / 1e12:.1f} TB/s, measured), the other rows through HBM (8 TB/s) once per "
+                f"{split['lazy_batch']} pivots; peak = algorithmic bytes / max(LDS time, HBM time) of that traffic "
+                f"({split['t_lds_s'] * 1e3:.1f} ms LDS, {split['t_hbm_s'] * 1e3:.1f} ms HBM per step)")
+    else:
+        peak = HBM_PEAK if in_global else lds_peak
+        bound = "hbm" if in_global else "lds"
+        note = ("HBM3E 8 TB/s: the tableau exceeds one workgroup's LDS and lives in global memory"
+                if in_global else
+                f"LDS read bandwidth measured on this device by mpqp_measure_lds_bandwidth "
+                f"(nominal 256 CU x 128 B/clk x 2.4 GHz = {LDS_NOMINAL / 1e12:.1f} TB/s)")
     return {
-        "bound": "hbm" if in_global else "lds",
+        "bound": bound,
         "kernel": name,
         "achieved": achieved,
         "peak": peak / 1e9,
@@ -302,10 +335,7 @@ def roofline(st: dict, st64: dict | None, lps, lds_peak: float, fp32: bool) -> d
         "tflops": flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0,
         "bytes_per_launch": byts / launches,
         "pivot_counts": "FP64 path" if st64 is not None else "this variant",
-        "peak_note": ("HBM3E 8 TB/s: the tableau exceeds one workgroup's LDS and lives in global memory"
-                      if in_global else
-                      f"LDS read bandwidth measured on this device by mpqp_measure_lds_bandwidth "
-                      f"(nominal 256 CU x 128 B/clk x 2.4 GHz = {LDS_NOMINAL / 1e12:.1f} TB/s)"),
+        "peak_note": note,
     }
 
 

@@ -237,6 +237,10 @@ typedef struct MpqpStats {
     int tableau_global;       /* optimality tableau in global memory (it exceeds one
                                  workgroup's LDS): bit 0 FP64 kernel, bit 1 FP32 kernel */
     double ms_fallback, bytes_fallback, flops_fallback;
+    /* rows of the optimality tableau held in LDS (M2 when it fits there; for a
+       split tableau the others are global rows whose phase-1 / phase-2 updates
+       are applied in batches of lazy_batch pivots) of the FP64 and FP32 kernels */
+    int split_rows_fp64, split_rows_fp32, lazy_batch, reserved_stats;
 } MpqpStats;
 
 int mpqp_get_stats(MpqpStats *stats);

@@ -501,6 +501,50 @@ __device__ __forceinline__ void pivot_team(Real *T, int ld, int nrows, int ncols
     __syncthreads();
 }
 
+// Update of rows [0, nrows) (row accessor `row`) by a divided pivot row
+// pr (pr[q] = 1/p, column q of the other rows already zero) with factors
+// colq: lane = column, U rows in flight per wave.  A lane whose pivot-row
+// entry is zero skips its column; a full block of U rows without row rr is
+// stored without a branch, the block with row rr (-1: none) and the last
+// partial block go row by row and leave row rr alone.
+template <int NT, int U, typename Real, typename RowF>
+__device__ __forceinline__ void update_cols(RowF row, int nrows, int ncols, int rr, const Real *pr,
+                                            const Real *colq) {
+    constexpr int NW = NT / MPQP_WAVE;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    const int wave = threadIdx.x / MPQP_WAVE;
+    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
+        const int j = j0 + lane;
+        const Real prj = j < ncols ? pr[j] : Real(0);
+        if (prj != Real(0)) {
+            int i0 = wave * U;
+            for (; i0 + U <= nrows; i0 += NW * U) {
+                if ((unsigned)(rr - i0) < (unsigned)U) {
+                    for (int i = i0; i < i0 + U; ++i)
+                        if (i != rr) {
+                            Real *ti = row(i) + j;
+                            *ti = fma(-colq[i], prj, *ti);
+                        }
+                    continue;
+                }
+                Real t[U], f[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    t[u] = row(i0 + u)[j];
+                    f[u] = colq[i0 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) row(i0 + u)[j] = fma(-f[u], prj, t[u]);
+            }
+            for (int i = i0; i < nrows; ++i)
+                if (i != rr) {
+                    Real *ti = row(i) + j;
+                    *ti = fma(-colq[i], prj, *ti);
+                }
+        }
+    }
+}
+
 // Three-phase form of the same pivot (column copy; division of row r and
 // zeroing of column q, while the last thread runs `between`; update), four
 // rows in flight per wave (eight for a split tableau).
@@ -520,31 +564,7 @@ __device__ __forceinline__ void pivot_rows3(const TB &T, int nrows, int ncols, i
         if (i != r) T.row(i)[q] = Real(0);
     if (tid == NT - 1) between();
     __syncthreads();
-    constexpr int NW = NT / MPQP_WAVE;
-    constexpr int U = TB::split ? 8 : 4;
-    const int lane = tid & (MPQP_WAVE - 1);
-    const int wave = tid / MPQP_WAVE;
-    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
-        const int j = j0 + lane;
-        if (j < ncols) {
-            const Real prj = tr[j];
-            if (prj != Real(0))
-                for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
-                    Real t[U], f[U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int i = i0 + u;
-                        t[u] = i < nrows ? T.row(i)[j] : Real(0);
-                        f[u] = i < nrows ? colq[i] : Real(0);
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const int i = i0 + u;
-                        if (i < nrows && i != r) T.row(i)[j] = fma(-f[u], prj, t[u]);
-                    }
-                }
-        }
-    }
+    update_cols<NT, TB::split ? 8 : 4>([&](int i) { return T.row(i); }, nrows, ncols, r, tr, colq);
     __syncthreads();
 }
 
@@ -607,77 +627,120 @@ __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, 
     __syncthreads();
 }
 
-// Same pivot, rows updated four at a time per wave so that several LDS loads
+// Update of rows [0, nrows) of an FP32 tableau (row accessor `row`, rows
+// 8-byte aligned) by a divided pivot row pr (pr[j] for j != q; column q takes
+// the factor pinv, its entries of the other rows already zero) with factors
+// colq (16-byte aligned): lane c of a wave step holds columns 2c, 2c + 1 as
+// one 8-byte vector (a pair that runs past ncols updates a padding column of
+// the row, which nothing reads).  A block of U rows takes its factors from
+// colq as 16-byte vectors and stores every row without a branch: the update
+// of an entry is one fma, kept only where the pivot-row entry is nonzero.
+// The one block that holds row rr (-1: none), and the last partial block,
+// go row by row and leave row rr alone.  A pair of zero pivot-row entries
+// is unchanged (column q never is).
+template <int NT, int U, typename RowF>
+__device__ __forceinline__ void update_pairs(RowF row, int nrows, int ncols, int rr, int q, const float *pr,
+                                             float pinv, const float *colq) {
+    constexpr int NW = NT / MPQP_WAVE;
+    const int lane = fresh_lane();
+    const int wave = threadIdx.x / MPQP_WAVE;
+    const int npair = (ncols + 1) >> 1;
+    for (int c0 = 0; c0 < npair; c0 += MPQP_WAVE) {
+        const int c = c0 + lane;
+        const int j = 2 * c;
+        const float2 pv = *reinterpret_cast<const float2 *>(pr + (c < npair ? j : 0));
+        const float p0 = j == q ? pinv : pv.x, p1 = j + 1 == q ? pinv : pv.y;
+        const bool m0 = p0 != 0.f, m1 = p1 != 0.f;
+        if (c < npair && (m0 || m1)) {
+            int i0 = wave * U;
+            for (; i0 + U <= nrows; i0 += NW * U) {
+                if ((unsigned)(rr - i0) < (unsigned)U) {
+                    for (int i = i0; i < i0 + U; ++i)
+                        if (i != rr) {
+                            float *ti = row(i) + j;
+                            const float2 t = *reinterpret_cast<const float2 *>(ti);
+                            const float f = colq[i];
+                            float2 o;
+                            o.x = m0 ? fma(-f, p0, t.x) : t.x;
+                            o.y = m1 ? fma(-f, p1, t.y) : t.y;
+                            *reinterpret_cast<float2 *>(ti) = o;
+                        }
+                    continue;
+                }
+                float2 t[U];
+                float f[U];
+#pragma unroll
+                for (int u = 0; u < U; u += 4) {
+                    const float4 ff = *reinterpret_cast<const float4 *>(colq + i0 + u);
+                    f[u] = ff.x;
+                    f[u + 1] = ff.y;
+                    f[u + 2] = ff.z;
+                    f[u + 3] = ff.w;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) t[u] = *reinterpret_cast<const float2 *>(row(i0 + u) + j);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const float nx = fma(-f[u], p0, t[u].x);
+                    const float ny = fma(-f[u], p1, t[u].y);
+                    float2 o;
+                    o.x = m0 ? nx : t[u].x;
+                    o.y = m1 ? ny : t[u].y;
+                    *reinterpret_cast<float2 *>(row(i0 + u) + j) = o;
+                }
+            }
+            for (int i = i0; i < nrows; ++i)
+                if (i != rr) {
+                    float *ti = row(i) + j;
+                    const float2 t = *reinterpret_cast<const float2 *>(ti);
+                    const float f = colq[i];
+                    float2 o;
+                    o.x = m0 ? fma(-f, p0, t.x) : t.x;
+                    o.y = m1 ? fma(-f, p1, t.y) : t.y;
+                    *reinterpret_cast<float2 *>(ti) = o;
+                }
+        }
+    }
+}
+
+// Same pivot, rows updated four at a time per wave so that several loads
 // are in flight, with two barriers: (A) every thread reads p = T[r][q],
-// copies its column-q entries (rows other than r) to colq and divides its
-// pivot-row entries (columns other than q) — nobody writes T[r][q] or
-// column q here; (B) the other rows are updated, column q from 0 with the
-// factor 1/p (colq[r] is never used), while the first thread of the last
+// copies its column-q entries (rows other than r) to colq and zeroes them,
+// and divides its pivot-row entries (columns other than q); (B) the other
+// rows are updated (update_pairs: eight rows in flight per wave for a split
+// tableau, whose global rows wait on L2; four for an LDS one), column q
+// from its zero with the factor 1/p, while the first thread of the last
 // wave writes T[r][q] = 1/p (no thread of (B) reads it: column q takes the
-// factor from a register) and runs `between` (the basis bookkeeping, which no
-// pivot step reads).  Per-entry arithmetic is identical to pivot_team.
+// factor from a register) and runs `between` (the basis bookkeeping, which
+// no pivot step reads).  Per-entry arithmetic is identical to pivot_team.
 template <int NT, typename TB, typename F>
 __device__ __forceinline__ void pivot_rows(const TB &T, int nrows, int ncols, int r, int q,
                                            typename TB::real_t *colq, F between) {
     typedef typename TB::real_t Real;
     if constexpr (sizeof(Real) == 8) {
-        // FP64 (one workgroup per CU): measured faster with the column copy,
-        // the division and the update in separate barrier phases
         pivot_rows3<NT>(T, nrows, ncols, r, q, colq, between);
         return;
-    }
-    const int tid = threadIdx.x;
-    Real *tr = T.row(r);
-    const Real p = tr[q];
-    for (int i = tid; i < nrows; i += NT)
-        if (i != r) colq[i] = T.row(i)[q];
-    for (int j = tid; j < ncols; j += NT)
-        if (j != q) tr[j] = tr[j] / p;
-    __syncthreads();
-    const Real pinv = Real(1) / p;
-    if (tid == NT - MPQP_WAVE) {
-        tr[q] = pinv;
-        between();
-    }
-    // (B) on column pairs: lane c of a wave step holds columns 2c, 2c + 1
-    // as one 8-byte vector (ld even, T 8-byte aligned; a pair that runs past
-    // ncols updates a padding column of the row, which nothing reads).  A
-    // split tableau keeps eight rows in flight per wave (its global rows
-    // wait on L2), an LDS one four.
-    constexpr int NW = NT / MPQP_WAVE;
-    constexpr int U = TB::split ? 8 : 4;
-    const int lane = fresh_lane();
-    const int wave = tid / MPQP_WAVE;
-    const int npair = (ncols + 1) >> 1;
-    for (int c0 = 0; c0 < npair; c0 += MPQP_WAVE) {
-        const int c = c0 + lane;
-        const int j = 2 * c;
-        const bool q0 = j == q, q1 = j + 1 == q;
-        const float2 pr = *reinterpret_cast<const float2 *>(tr + (c < npair ? j : 0));
-        const Real p0 = q0 ? pinv : pr.x, p1 = q1 ? pinv : pr.y;
-        // a pair of zero pivot-row entries is unchanged (column q never is)
-        if (c < npair && (p0 != Real(0) || p1 != Real(0))) {
-            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
-                float2 t[U];
-                Real f[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u < nrows ? i0 + u : nrows - 1;
-                    t[u] = *reinterpret_cast<const float2 *>(T.row(i) + j);
-                    f[u] = colq[i];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    float2 o;
-                    o.x = p0 != Real(0) ? fma(-f[u], p0, q0 ? Real(0) : t[u].x) : t[u].x;
-                    o.y = p1 != Real(0) ? fma(-f[u], p1, q1 ? Real(0) : t[u].y) : t[u].y;
-                    if (i < nrows && i != r) *reinterpret_cast<float2 *>(T.row(i) + j) = o;
-                }
+    } else {
+        const int tid = threadIdx.x;
+        Real *tr = T.row(r);
+        const Real p = tr[q];
+        for (int i = tid; i < nrows; i += NT)
+            if (i != r) {
+                Real *ti = T.row(i);
+                colq[i] = ti[q];
+                ti[q] = Real(0);
             }
+        for (int j = tid; j < ncols; j += NT)
+            if (j != q) tr[j] = tr[j] / p;
+        __syncthreads();
+        const Real pinv = Real(1) / p;
+        if (tid == NT - MPQP_WAVE) {
+            tr[q] = pinv;
+            between();
         }
+        update_pairs<NT, TB::split ? 8 : 4>([&](int i) { return T.row(i); }, nrows, ncols, r, q, tr, pinv, colq);
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Pivot on row r with an entering column v held outside the tableau (rows

@@ -47,6 +47,7 @@ static_assert(sizeof(DevRecord) == sizeof(MpqpRecord), "record layout");
 static_assert(sizeof(GPUResults) == 48, "GPUResults layout of main.py:12-22");
 static_assert(sizeof(DevRegion) == sizeof(MpqpRegionInfo), "region info layout");
 static_assert(sizeof(MpqpProblem) == 112, "MpqpProblem layout");
+static_assert(sizeof(MpqpStats) == 160, "MpqpStats layout");
 
 // Device memory of mpqp_noalloc(): every call on a GPU uses that GPU's copy
 // of this array (it holds the last call's contents; overlapping calls on
@@ -130,15 +131,14 @@ struct HipError {
 
 // Rows of an optimality tableau too large for LDS that stay in LDS (even,
 // so that a state copy splits on a whole 16-byte vector), next to its cost
-// row and the bookkeeping, within `budget` bytes; the other rows live in
+// row, the bookkeeping and the deferred updates of the other rows (whose
+// size falls as R grows), within `budget` bytes; the other rows live in
 // global memory.
 template <typename Real>
 int split_rows_for(int M, int K, int N2, bool cert, long long budget) {
-    const long long row = (long long)opt_ld<Real>(K) * (long long)sizeof(Real);
-    long long R = (budget - opt_lds_bytes_t<Real>(M, K, N2, cert, 1)) / row;
-    R = std::max(0LL, std::min<long long>(R, M)) & ~1LL;
-    while (R > 0 && opt_lds_bytes_t<Real>(M, K, N2, cert, (int)R + 1) > budget) R -= 2;
-    return (int)R;
+    int R = M & ~1;
+    while (R > 0 && opt_lds_bytes_t<Real>(M, K, N2, cert, R + 1) > budget) R -= 2;
+    return R;
 }
 
 #define HIP_OK(expr)                                                                   \
@@ -1307,6 +1307,9 @@ struct Run {
         }
         C.stats.precision_used = mixed ? MPQP_MIXED : MPQP_FP64;
         C.stats.tableau_global = (gt64 ? 1 : 0) | (gt32 ? 2 : 0);
+        C.stats.split_rows_fp64 = gt64 ? R64 : in.M2;
+        C.stats.split_rows_fp32 = gt32 ? R32 : in.M2;
+        C.stats.lazy_batch = MPQP_LAZY_KE;
         C.last_feas_nt = feas_nt;
         C.last_opt_nt = opt_nt;
         HIP_OK(hipStreamSynchronize(st));
@@ -1409,6 +1412,9 @@ void add_stats(MpqpStats &d, const MpqpStats &s) {
     d.fallbacks += s.fallbacks;
     d.launches_fallback += s.launches_fallback;
     d.tableau_global |= s.tableau_global;
+    d.split_rows_fp64 = s.split_rows_fp64;
+    d.split_rows_fp32 = s.split_rows_fp32;
+    d.lazy_batch = s.lazy_batch;
     d.ms_fallback += s.ms_fallback;
     d.bytes_fallback += s.bytes_fallback;
     d.flops_fallback += s.flops_fallback;

@@ -27,18 +27,39 @@ __host__ __device__ inline int opt_ld(int K) {
 }
 
 // LDS of one workgroup: trows rows of T (M + 1 for the whole tableau; R + 1
-// when it is split: rows [0, R) and the cost row), colq (M+1), vcol (M) in
+// when it is split: rows [0, R) and the cost row), then (16-byte aligned,
+// for the pivot's vector reads of the factors) colq (M+1), vcol (M) in
 // Real; then (8-byte aligned) bcnt, nbcnt, basis, nonb, where, rows in int;
-// for certification (FP32 pivots) x and r in double.  During certification
+// for certification (FP32 pivots) x and r in double; for a split tableau
+// (16-byte aligned) the deferred updates of its global rows.  During certification
 // bcnt/nbcnt hold x by variable (N2 doubles) and colq/vcol the list of
 // nonbasic e-variables.
+// Phase-1 pivots of a split tableau update its global rows [R, M) in
+// batches of MPQP_LAZY_KE (opt_lazy_pivot).
+#define MPQP_LAZY_KE 8
+
+// LDS of those deferred updates (16-byte aligned start): the divided pivot
+// rows (KE rows of ld), per global row its KE factors, the rhs and
+// perturbation columns of the global rows, one brought-up-to-date row (ld);
+// then the applied-update count per global row and the KE pivot columns.
+template <typename Real>
+__host__ __device__ inline long long opt_lazy_bytes(int M, int K, int R) {
+    const long long ld = opt_ld<Real>(K), G = M - R;
+    long long b = (MPQP_LAZY_KE * ld + G * MPQP_LAZY_KE + 2 * G + ld) * (long long)sizeof(Real);
+    b = ((b + 3) & ~3LL) + (G + MPQP_LAZY_KE) * 4;
+    return (b + 15) & ~15LL;
+}
+
 template <typename Real>
 __host__ __device__ inline long long opt_lds_bytes_t(int M, int K, int N2, bool cert, int trows) {
     const int ld = opt_ld<Real>(K);
-    long long b = (long long)((long long)trows * ld + (M + 1) + M) * (long long)sizeof(Real);
+    long long b = (((long long)trows * ld * (long long)sizeof(Real) + 15) & ~15LL) +
+                  (long long)((M + 1) + M) * (long long)sizeof(Real);
     b = ((b + 7) & ~7LL) + (long long)(M + K + 3 * N2) * 4;
     if (cert) b = ((b + 7) & ~7LL) + 2LL * M * 8;
-    return (b + 15) & ~15LL;
+    b = (b + 15) & ~15LL;
+    if (trows < M + 1) b += opt_lazy_bytes<Real>(M, K, trows - 1);
+    return b;
 }
 
 // Global-memory rows [R, M) of a split tableau, one workgroup's share:
@@ -105,13 +126,16 @@ struct OptLds {
     double *cx, *cr;  // certification vectors (FP32 pivots only)
     double *xv;       // certification: x by variable (over bcnt / nbcnt)
     int *elr, *els;   // certification: nonbasic e-list rho / slot (over colq / vcol)
+    // SPLIT: deferred updates of the global rows (opt_lazy_pivot)
+    Real *ep, *ef, *gx, *gp, *prow;
+    int *gver, *eq;
     // SPLIT: rows [R, M) of the tableau at gT (global memory), the rest in LDS
     __device__ void carve(unsigned char *smem, int M, int K, int N2, bool cert, Real *gT, int R) {
         ld = opt_ld<Real>(K);
         Real *lo = reinterpret_cast<Real *>(smem);
         const int trows = SPLIT ? R + 1 : M + 1;
         T = TabRef<Real, SPLIT>{lo, gT, R, M, ld};
-        colq = lo + trows * ld;
+        colq = reinterpret_cast<Real *>(smem + (((size_t)trows * ld * sizeof(Real) + 15) & ~(size_t)15));
         vcol = colq + (M + 1);
         const size_t ib = ((size_t)(reinterpret_cast<unsigned char *>(vcol + M) - smem) + 7) & ~(size_t)7;
         bcnt = reinterpret_cast<int *>(smem + ib);
@@ -124,12 +148,37 @@ struct OptLds {
         cx = cr = xv = nullptr;
         elr = reinterpret_cast<int *>(colq);
         els = reinterpret_cast<int *>(vcol);
+        unsigned char *end = reinterpret_cast<unsigned char *>(where + N2);
         if (cert) {
-            const size_t off = ((size_t)(reinterpret_cast<unsigned char *>(where + N2) - smem) + 7) & ~(size_t)7;
+            const size_t off = ((size_t)(end - smem) + 7) & ~(size_t)7;
             cx = reinterpret_cast<double *>(smem + off);
             cr = cx + M;
             xv = reinterpret_cast<double *>(bcnt);
+            end = reinterpret_cast<unsigned char *>(cr + M);
         }
+        ep = ef = gx = gp = prow = nullptr;
+        gver = eq = nullptr;
+        if (SPLIT) {
+            const size_t off = ((size_t)(end - smem) + 15) & ~(size_t)15;
+            const int G = M - R;
+            ep = reinterpret_cast<Real *>(smem + off);
+            ef = ep + MPQP_LAZY_KE * ld;
+            gx = ef + (size_t)G * MPQP_LAZY_KE;
+            gp = gx + G;
+            prow = gp + G;
+            gver = reinterpret_cast<int *>(prow + ld);
+            eq = gver + G;
+        }
+    }
+    // rhs / perturbation entry of row i < M inside the phase-1 loop (the
+    // global rows' mirrors when the tableau is split)
+    __device__ __forceinline__ const Real *rhs_at(int i, int RHS) const {
+        if constexpr (SPLIT) return i >= T.R ? gx + (i - T.R) : T.lo + i * ld + RHS;
+        else return T.row(i) + RHS;
+    }
+    __device__ __forceinline__ const Real *pert_at(int i, int RHS) const {
+        if constexpr (SPLIT) return i >= T.R ? gp + (i - T.R) : T.lo + i * ld + RHS + 1;
+        else return T.row(i) + RHS + 1;
     }
 };
 
@@ -290,6 +339,244 @@ __device__ __forceinline__ void opt_swap(OptLds<Real, SPLIT> &L, int r, int q) {
 // ---------------------------------------------------------------------
 // phase 1
 // ---------------------------------------------------------------------
+// ---------------------------------------------------------------------
+// Deferred updates of the global rows of a split tableau (phase 1)
+//
+// A pivot updates the LDS rows (and the cost row) at once, as pivot_rows
+// does, and only records itself for the rows [R, M) held in global memory:
+// its divided pivot row (ep) and, per global row, its factor (ef: the row's
+// column-q entry at the time of the pivot).  Within the phase-1 loop the
+// global rows are read only through their rhs and perturbation entries,
+// which are kept current in LDS (gx, gp), through the leaving row (brought
+// up to date when it is chosen, then written back: gver counts the
+// recorded pivots already applied to each row) and through the entering
+// column (brought up to date in the pivot).  Every MPQP_LAZY_KE pivots, and
+// when the loop ends, the recorded pivots are applied to every global row
+// in one pass over it (opt_lazy_flush).  Each entry goes through exactly
+// the fma sequence of the immediate update (column q_e of the other rows
+// from zero with the factor 1/p_e, other columns only where the pivot-row
+// entry is nonzero), so the tableau is bit-identical to pivot_rows' wherever
+// it is read.
+// ---------------------------------------------------------------------
+
+// Entry j of global row g (stored value t, updates v0 .. ne-1 pending).
+template <typename Real>
+__device__ __forceinline__ Real lazy_entry(const Real *ep, const Real *efg, const int *eq, int ld, int j, Real t,
+                                           int v0, int ne) {
+    for (int e = v0; e < ne; ++e) {
+        const Real a = ep[e * ld + j];
+        const Real nt = fma(-efg[e], a, j == eq[e] ? Real(0) : t);
+        t = a != Real(0) ? nt : t;
+    }
+    return t;
+}
+
+// Start of the phase-1 loop: mirrors of the global rows' rhs and
+// perturbation columns, no update pending (the caller's barrier follows).
+template <int NT, typename Real>
+__device__ __forceinline__ void opt_lazy_start(OptLds<Real, true> &L, int M, int RHS) {
+    const int R = L.T.R, G = M - R, ld = L.ld;
+    for (int g = threadIdx.x; g < G; g += NT) {
+        const Real *tg = L.T.hi + (size_t)g * ld;
+        L.gx[g] = tg[RHS];
+        L.gp[g] = tg[RHS + 1];
+        L.gver[g] = 0;
+    }
+}
+
+// All pending updates applied to every global row; ends with a barrier.
+// Lane = column (pair of columns in FP32), U rows in flight per wave; a lane
+// whose entries are zero in every pending pivot row leaves its columns
+// alone.
+template <int NT, typename Real>
+__device__ __forceinline__ void opt_lazy_flush(OptLds<Real, true> &L, int M, int ncols, int ne) {
+    constexpr int KE = MPQP_LAZY_KE, NW = NT / MPQP_WAVE, U = 8;
+    const int R = L.T.R, G = M - R, ld = L.ld;
+    const int wave = threadIdx.x / MPQP_WAVE;
+    Real *hi = L.T.hi;
+    if (ne > 0) {
+        if constexpr (sizeof(Real) == 4) {
+            const int lane = fresh_lane();
+            const int npair = (ncols + 1) >> 1;
+            for (int c0 = 0; c0 < npair; c0 += MPQP_WAVE) {
+                const int c = c0 + lane, j = 2 * c;
+                const int jc = c < npair ? j : 0;
+                float2 a[KE];
+                unsigned zx = 0u, zy = 0u;
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < KE; ++e) {
+                    a[e] = e < ne ? *reinterpret_cast<const float2 *>(L.ep + e * ld + jc) : make_float2(0.f, 0.f);
+                    const int qe = e < ne ? L.eq[e] : -1;
+                    zx |= (j == qe ? 1u : 0u) << e;
+                    zy |= (j + 1 == qe ? 1u : 0u) << e;
+                    any |= a[e].x != 0.f || a[e].y != 0.f;
+                }
+                if (c < npair && any) {
+                    for (int g0 = wave * U; g0 < G; g0 += NW * U) {
+                        float2 t[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int g = g0 + u < G ? g0 + u : G - 1;
+                            t[u] = *reinterpret_cast<const float2 *>(hi + (size_t)g * ld + j);
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int g = g0 + u;
+                            if (g < G) {
+                                const int v0 = __builtin_amdgcn_readfirstlane(L.gver[g]);
+                                const float4 f0 = *reinterpret_cast<const float4 *>(L.ef + (size_t)g * KE);
+                                const float4 f1 = *reinterpret_cast<const float4 *>(L.ef + (size_t)g * KE + 4);
+                                const float f[KE] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+                                float2 v = t[u];
+#pragma unroll
+                                for (int e = 0; e < KE; ++e) {
+                                    if (e >= v0 && e < ne) {
+                                        const float bx = ((zx >> e) & 1u) ? 0.f : v.x;
+                                        const float by = ((zy >> e) & 1u) ? 0.f : v.y;
+                                        const float nx = fma(-f[e], a[e].x, bx);
+                                        const float ny = fma(-f[e], a[e].y, by);
+                                        v.x = a[e].x != 0.f ? nx : v.x;
+                                        v.y = a[e].y != 0.f ? ny : v.y;
+                                    }
+                                }
+                                *reinterpret_cast<float2 *>(hi + (size_t)g * ld + j) = v;
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
+            const int lane = threadIdx.x & (MPQP_WAVE - 1);
+            for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
+                const int j = j0 + lane;
+                const int jc = j < ncols ? j : 0;
+                double a[KE];
+                unsigned z = 0u;
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < KE; ++e) {
+                    a[e] = e < ne ? L.ep[e * ld + jc] : 0.0;
+                    const int qe = e < ne ? L.eq[e] : -1;
+                    z |= (j == qe ? 1u : 0u) << e;
+                    any |= a[e] != 0.0;
+                }
+                if (j < ncols && any) {
+                    for (int g0 = wave * U; g0 < G; g0 += NW * U) {
+                        double t[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int g = g0 + u < G ? g0 + u : G - 1;
+                            t[u] = hi[(size_t)g * ld + j];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int g = g0 + u;
+                            if (g < G) {
+                                const int v0 = __builtin_amdgcn_readfirstlane(L.gver[g]);
+                                double v = t[u];
+#pragma unroll
+                                for (int e = 0; e < KE; ++e) {
+                                    if (e >= v0 && e < ne) {
+                                        const double b = ((z >> e) & 1u) ? 0.0 : v;
+                                        const double nv = fma(-L.ef[(size_t)g * KE + e], a[e], b);
+                                        v = a[e] != 0.0 ? nv : v;
+                                    }
+                                }
+                                hi[(size_t)g * ld + j] = v;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int g = threadIdx.x; g < G; g += NT) L.gver[g] = 0;
+    __syncthreads();
+}
+
+// One pivot on (r, q) of a split tableau with deferred updates of its
+// global rows.  Phase 1 (P1): ncols = K + 2 (rhs = K, perturbation =
+// K + 1), and if r >= R, L.prow holds row r brought up to date; phase 2:
+// ncols = K + 1 (rhs = K; the perturbation column is not updated), and the
+// global rows' column q, brought up to date by the decision, is this
+// pivot's factor ef[g][ne] already.  Returns the new number of pending
+// updates (0 after a flush).
+template <int NT, bool P1, typename Real, typename F>
+__device__ __forceinline__ int opt_lazy_pivot(OptLds<Real, true> &L, int M, int ncols, int r, int q, int ne,
+                                              F between) {
+    constexpr int KE = MPQP_LAZY_KE;
+    const int tid = threadIdx.x;
+    const int R = L.T.R, G = M - R, ld = L.ld, RHS = P1 ? ncols - 2 : ncols - 1;
+    Real *lo = L.T.lo, *hi = L.T.hi;
+    Real *pe = L.ep + ne * ld;
+    const bool rl = r < R;
+    const int gr = rl ? -1 : r - R;
+    const Real *src = rl ? lo + r * ld : L.prow;
+    const Real p = (P1 || rl) ? src[q] : L.ef[(size_t)gr * KE + ne];
+    // (A) LDS rows (physical row R is the cost row): column q to colq and
+    // zeroed; global rows: column q brought up to date = this pivot's
+    // factor; the pivot row divided into pe (and in place, or written back)
+    const int rr = rl ? r : -1;
+    for (int i = tid; i <= R; i += NT)
+        if (i != rr) {
+            Real *ti = lo + i * ld;
+            L.colq[i] = ti[q];
+            ti[q] = Real(0);
+        }
+    if constexpr (P1) {
+        for (int g = tid; g < G; g += NT) {
+            if (g == gr) continue;
+            const Real *efg = L.ef + (size_t)g * KE;
+            L.ef[(size_t)g * KE + ne] = lazy_entry(L.ep, efg, L.eq, ld, q, hi[(size_t)g * ld + q], L.gver[g], ne);
+        }
+    }
+    const int v0r = rl ? 0 : L.gver[gr];
+    for (int j = tid; j < ncols; j += NT) {
+        Real a;
+        if (P1 || rl) a = src[j];
+        else a = lazy_entry(L.ep, L.ef + (size_t)gr * KE, L.eq, ld, j, hi[(size_t)gr * ld + j], v0r, ne);
+        const Real v = j != q ? a / p : Real(1) / p;
+        pe[j] = v;
+        if (rl) {
+            if (j != q) lo[r * ld + j] = v;
+        } else {
+            hi[(size_t)gr * ld + j] = v;
+        }
+    }
+    __syncthreads();
+    // (B) the LDS rows updated at once (row r excluded), the mirrors of the
+    // global rows' rhs / perturbation entries; T[r][q] = 1/p, the update
+    // count of a global pivot row and the bookkeeping by one thread
+    const Real pinv = Real(1) / p;
+    if (tid == NT - MPQP_WAVE) {
+        if (rl) lo[r * ld + q] = pinv;
+        else L.gver[gr] = ne + 1;
+        L.eq[ne] = q;
+        between();
+    }
+    auto lrow = [lo, ld](int i) { return lo + i * ld; };
+    if constexpr (sizeof(Real) == 4) update_pairs<NT, 4>(lrow, R + 1, ncols, rr, q, pe, pinv, L.colq);
+    else update_cols<NT, 4>(lrow, R + 1, ncols, rr, pe, L.colq);
+    const Real ax = pe[RHS], ap = P1 ? pe[RHS + 1] : Real(0);
+    for (int g = tid; g < G; g += NT) {
+        const Real f = L.ef[(size_t)g * KE + ne];
+        const Real x = L.gx[g];
+        L.gx[g] = g == gr ? ax : (ax != Real(0) ? fma(-f, ax, x) : x);
+        if constexpr (P1) {
+            const Real y = L.gp[g];
+            L.gp[g] = g == gr ? ap : (ap != Real(0) ? fma(-f, ap, y) : y);
+        }
+    }
+    __syncthreads();
+    if (++ne == KE) {
+        opt_lazy_flush<NT>(L, M, ncols, ne);
+        ne = 0;
+    }
+    return ne;
+}
+
 struct P1Result {
     int status, iters, kkt;
     double z1;
@@ -306,7 +593,9 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
         T, M, K + 2, L.rows, sh.nrows, [&](int) { return (Real)1; },
         [&](int j) { return j < K ? (L.nonb[j] >= Nn ? (Real)1 : (Real)0) : (Real)0; });
     for (int v = tid; v < N2; v += NT) { L.bcnt[v] = 0; L.nbcnt[v] = 0; }
+    if constexpr (SPLIT) opt_lazy_start<NT>(L, M, RHS);
     __syncthreads();
+    int ne = 0;  // pending updates of the global rows (split tableau)
 
     P1Result res{ST_ITER_CAP, 0, 0, 0.0};
     int n_same = 0, no_pert = 0;
@@ -333,9 +622,8 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 const int i = lane + MPQP_WAVE * s;
                 const int ic = i < M ? i : 0;
                 const int kc = i < K ? i : 0;
-                const Real *tic = T.row(ic);
-                x0[s] = tic[RHS];
-                xp[s] = tic[PERT];
+                x0[s] = *L.rhs_at(ic, RHS);
+                xp[s] = *L.pert_at(ic, RHS);
                 bv[s] = L.basis[ic];
                 cl[s] = T.row(M)[kc];
                 nv[s] = L.nonb[kc];
@@ -400,17 +688,39 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 if (pert) {
                     bool inf = false;
                     for (int i = lane; i < M; i += MPQP_WAVE)
-                        inf |= (double)T.row(i)[RHS] < tl.dual_stop_tol;
+                        inf |= (double)*L.rhs_at(i, RHS) < tl.dual_stop_tol;
                     if (__any(inf)) action = ACT_UNPERT;
                 }
             } else {
                 const int mode = n_same < 10 ? 0 : (n_same < 100 ? 1 : 2);
-                const Real *tr = T.row(r);
                 Real al[S];
+                bool eager_r = true;
+                if constexpr (SPLIT) {
+                    if (r >= T.R) {
+                        // a global row: brought up to date (registers, and
+                        // L.prow for the pivot; K + 2 <= 64 S columns)
+                        eager_r = false;
+                        const int g = r - T.R, ld = L.ld;
+                        const int v0 = L.gver[g];
+                        const Real *src = T.hi + (size_t)g * ld;
+                        const Real *efg = L.ef + (size_t)g * MPQP_LAZY_KE;
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int kk = lane + MPQP_WAVE * s;
-                    al[s] = tr[kk < K ? kk : 0];
+                        for (int s = 0; s < S; ++s) {
+                            const int j = lane + MPQP_WAVE * s;
+                            const int jc = j < K + 2 ? j : 0;
+                            const Real t = lazy_entry(L.ep, efg, L.eq, ld, jc, src[jc], v0, ne);
+                            if (j < K + 2) L.prow[j] = t;
+                            al[s] = j < K ? t : (Real)0;
+                        }
+                    }
+                }
+                if (eager_r) {
+                    const Real *tr = T.row(r);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int kk = lane + MPQP_WAVE * s;
+                        al[s] = tr[kk < K ? kk : 0];
+                    }
                 }
                 Real dlim = (Real)tl.tol_minus;
                 if (tl.pivot_rel > 0.0) {
@@ -465,8 +775,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
                 const int ic = i < M ? i : 0;
-                const Real *tic = T.row(ic);
-                const Real x0 = tic[RHS], xp = tic[PERT];
+                const Real x0 = *L.rhs_at(ic, RHS), xp = *L.pert_at(ic, RHS);
                 const Real x = (pert && i < M) ? x0 + xp : x0;
                 ps[s] = (i < M && x < (Real)0) ? x : (Real)0;
             }
@@ -497,13 +806,18 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
         // (the pivot below has barriers of its own)
         if (action != ACT_PIVOT) __syncthreads();
         if (action == ACT_STOP) { stopped = true; break; }
-        if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; return res; }
+        if (action == ACT_UNBOUNDED) { res.status = ST_UNBOUNDED; break; }
         if (action == ACT_UNPERT) continue;
-        pivot_rows<NT>(T, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
+        if constexpr (SPLIT) ne = opt_lazy_pivot<NT, true>(L, M, K + 2, r, q, ne, [&] { opt_swap(L, r, q); });
+        else pivot_rows<NT>(T, M + 1, K + 2, r, q, L.colq, [&] { opt_swap(L, r, q); });
         OPT_PROF(sh, 2);
         ++res.iters;
-        if (stall) { res.status = ST_STALL; return res; }
+        if (stall) { res.status = ST_STALL; break; }
     }
+    // the global rows of a split tableau brought up to date before
+    // anything else reads them
+    if constexpr (SPLIT) opt_lazy_flush<NT>(L, M, K + 2, ne);
+    if (res.status == ST_UNBOUNDED || res.status == ST_STALL) return res;
     if (!stopped) { res.status = ST_ITER_CAP; return res; }
 
     // Restore dual feasibility if the counter modes broke it: primal simplex
@@ -595,6 +909,59 @@ __device__ __forceinline__ int opt_drive_out(OptLds<Real, SPLIT> &L, const DevPr
 // ---------------------------------------------------------------------
 // phase 2: min -eps over the first N_ columns
 // ---------------------------------------------------------------------
+// primal_ratio_wave for a split tableau inside the phase-2 loop (wave 0):
+// the global rows' column q is brought up to date (and kept as this
+// pivot's factors ef[g][ne]), their rhs read from the mirror.  Each lane
+// takes its rows in ascending order, as primal_ratio_wave does.
+template <int S, typename Real>
+__device__ __forceinline__ int lazy_ratio_wave(OptLds<Real, true> &L, int M, int q, int RHS, const DevTol &tl,
+                                               int ne) {
+    constexpr int KE = MPQP_LAZY_KE;
+    const int lane = threadIdx.x & (MPQP_WAVE - 1);
+    const int R = L.T.R, ld = L.ld;
+    Real al[S], xr[S];
+    double cm = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = lane + MPQP_WAVE * s;
+        Real a = (Real)0, x = (Real)0;
+        if (i < M) {
+            if (i >= R) {
+                const int g = i - R;
+                a = lazy_entry(L.ep, L.ef + (size_t)g * KE, L.eq, ld, q, L.T.hi[(size_t)g * ld + q], L.gver[g], ne);
+                L.ef[(size_t)g * KE + ne] = a;
+                x = L.gx[g];
+            } else {
+                const Real *ti = L.T.lo + i * ld;
+                a = ti[q];
+                x = ti[RHS];
+            }
+            const double fa = fabs((double)a);
+            cm = fa > cm ? fa : cm;
+        }
+        al[s] = a;
+        xr[s] = x;
+    }
+    Real lim = (Real)tl.tol_plus;
+    if (tl.pivot_rel > 0.0) {
+        const Real cmax = (Real)wave_max(cm);
+        const Real rl = (Real)tl.pivot_rel * cmax;
+        lim = rl > lim ? rl : lim;
+    }
+    double rv = 0.0;
+    int rk = -1, dummy = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = lane + MPQP_WAVE * s;
+        if (i < M && al[s] > lim) {
+            const Real rt = xr[s] / al[s];
+            if (rk < 0 || (double)rt < rv) { rv = (double)rt; rk = i; }
+        }
+    }
+    lex_min(rv, rk, dummy);
+    return rk;
+}
+
 struct P2Result {
     int status, iters, final_set;
     double z2;
@@ -611,7 +978,9 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const Dev
         T, M, K + 1, L.rows, sh.nrows, [&](int) { return (Real)-1; },
         [&](int j) { return j < K ? (L.nonb[j] == EPS ? (Real)-1 : (Real)0) : (Real)0; });
     for (int v = tid; v < N2; v += NT) L.bcnt[v] = 0;
+    if constexpr (SPLIT) opt_lazy_start<NT>(L, M, RHS);
     __syncthreads();
+    int ne = 0;  // pending updates of the global rows (split tableau)
     P2Result res{ST_ITER_CAP, 0, 0, 0.0};
     for (int it = 0; it < O.max_iter_p2; ++it) {
         if (wave == 0) {
@@ -640,13 +1009,14 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const Dev
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int i = lane + MPQP_WAVE * s;
-                    zs[s] = (i < M) ? (L.basis[i] == EPS ? (Real)-1 : (Real)0) * T.row(i)[RHS] : (Real)0;
+                    zs[s] = (i < M) ? (L.basis[i] == EPS ? (Real)-1 : (Real)0) * *L.rhs_at(i, RHS) : (Real)0;
                 }
                 zz = (double)tree_sum_s<S, Real>(zs, M, lane);
                 action = ACT_STOP;
             } else {
                 q = -L.where[wv] - 1;
-                r = primal_ratio_wave(T, M, q, RHS, tl);
+                if constexpr (SPLIT) r = lazy_ratio_wave<S>(L, M, q, RHS, tl, ne);
+                else r = primal_ratio_wave(T, M, q, RHS, tl);
                 if (r < 0) action = ACT_UNBOUNDED;
             }
             if (lane == 0) {
@@ -669,9 +1039,11 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const Dev
             res.status = ST_UNBOUNDED;
             break;
         }
-        pivot_rows<NT>(T, M + 1, K + 1, r, q, L.colq, [&] { opt_swap(L, r, q); });
+        if constexpr (SPLIT) ne = opt_lazy_pivot<NT, false>(L, M, K + 1, r, q, ne, [&] { opt_swap(L, r, q); });
+        else pivot_rows<NT>(T, M + 1, K + 1, r, q, L.colq, [&] { opt_swap(L, r, q); });
         ++res.iters;
     }
+    if constexpr (SPLIT) opt_lazy_flush<NT>(L, M, K + 1, ne);
     res.final_set = (res.status == ST_OPTIMAL && -res.z2 > tl.final_tol) ? 1 : 0;
     return res;
 }

@@ -149,6 +149,10 @@ class MpqpStats(ctypes.Structure):
         ("ms_fallback", ctypes.c_double),
         ("bytes_fallback", ctypes.c_double),
         ("flops_fallback", ctypes.c_double),
+        ("split_rows_fp64", ctypes.c_int),
+        ("split_rows_fp32", ctypes.c_int),
+        ("lazy_batch", ctypes.c_int),
+        ("reserved_stats", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -50,7 +50,7 @@ def test_gpuresults_layout_matches_reference():
 def test_struct_sizes():
     assert ctypes.sizeof(MpqpRecord) == 80
     assert ctypes.sizeof(MpqpOptions) == 48 + 16 * 8 + 8 + 8
-    assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8 + 8 + 3 * 8
+    assert ctypes.sizeof(MpqpStats) == 16 + 6 * 8 + 6 * 8 + 8 + 3 * 8 + 16
     assert ctypes.sizeof(MpqpProblem) == 112
     assert ctypes.sizeof(MpqpRegionInfo) == 24
 
