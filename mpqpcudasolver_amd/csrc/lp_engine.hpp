@@ -601,28 +601,43 @@ __device__ __forceinline__ void pivot_rows_narrow(double *T, int ld, int nrows, 
     const int j = 2 * c;
     const double2 pr = *reinterpret_cast<const double2 *>(tr + (dr < rpw ? j : 0));
     // a pair that runs past ncols updates the row's padding column; a pair
-    // of zero pivot-row entries is unchanged
+    // of zero pivot-row entries is unchanged.  While every lane's rows of a
+    // block exist (the wave's first row + rpw - 1 + (U - 1) stride < nrows)
+    // the block is stored without a branch (row r keeps its values); the
+    // rest goes row by row.
     if (dr < rpw && (pr.x != 0.0 || pr.y != 0.0)) {
         const int stride = NW * rpw;
-        for (int i0 = wave * rpw + dr; i0 < nrows; i0 += stride * U) {
+        int b0 = wave * rpw;
+        for (; b0 + (rpw - 1) + (U - 1) * stride < nrows; b0 += stride * U) {
+            const int i0 = b0 + dr;
             double2 t[U];
             double f[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * stride;
-                const int ic = i < nrows ? i : nrows - 1;
-                t[u] = *reinterpret_cast<const double2 *>(T + ic * ld + j);
-                f[u] = colq[ic];
+                t[u] = *reinterpret_cast<const double2 *>(T + (i0 + u * stride) * ld + j);
+                f[u] = colq[i0 + u * stride];
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int i = i0 + u * stride;
+                const bool isr = i0 + u * stride == r;
+                const double nx = fma(-f[u], pr.x, t[u].x);
+                const double ny = fma(-f[u], pr.y, t[u].y);
                 double2 o;
-                o.x = pr.x != 0.0 ? fma(-f[u], pr.x, t[u].x) : t[u].x;
-                o.y = pr.y != 0.0 ? fma(-f[u], pr.y, t[u].y) : t[u].y;
-                if (i < nrows && i != r) *reinterpret_cast<double2 *>(T + i * ld + j) = o;
+                o.x = (pr.x != 0.0 && !isr) ? nx : t[u].x;
+                o.y = (pr.y != 0.0 && !isr) ? ny : t[u].y;
+                *reinterpret_cast<double2 *>(T + (i0 + u * stride) * ld + j) = o;
             }
         }
+        for (int i = b0 + dr; i < nrows; i += stride)
+            if (i != r) {
+                double *ti = T + i * ld + j;
+                const double2 t = *reinterpret_cast<const double2 *>(ti);
+                const double fi = colq[i];
+                double2 o;
+                o.x = pr.x != 0.0 ? fma(-fi, pr.x, t.x) : t.x;
+                o.y = pr.y != 0.0 ? fma(-fi, pr.y, t.y) : t.y;
+                *reinterpret_cast<double2 *>(ti) = o;
+            }
     }
     __syncthreads();
 }

@@ -36,7 +36,10 @@ __host__ __device__ inline int opt_ld(int K) {
 // nonbasic e-variables.
 // Phase-1 pivots of a split tableau update its global rows [R, M) in
 // batches of MPQP_LAZY_KE (opt_lazy_pivot).
+#ifndef MPQP_LAZY_KE
 #define MPQP_LAZY_KE 8
+#endif
+static_assert(MPQP_LAZY_KE % 4 == 0 && MPQP_LAZY_KE <= 32, "batch of deferred updates");
 
 // LDS of those deferred updates (16-byte aligned start): the divided pivot
 // rows (KE rows of ld), per global row its KE factors, the rhs and
@@ -359,14 +362,25 @@ __device__ __forceinline__ void opt_swap(OptLds<Real, SPLIT> &L, int r, int q) {
 // it is read.
 // ---------------------------------------------------------------------
 
-// Entry j of global row g (stored value t, updates v0 .. ne-1 pending).
+// Entry j of global row g (stored value t, updates v0 .. ne-1 pending;
+// efg = the row's factors, 16-byte aligned).  Every slot's operands are
+// loaded at once and the updates outside [v0, ne) are masked.
 template <typename Real>
 __device__ __forceinline__ Real lazy_entry(const Real *ep, const Real *efg, const int *eq, int ld, int j, Real t,
                                            int v0, int ne) {
-    for (int e = v0; e < ne; ++e) {
-        const Real a = ep[e * ld + j];
-        const Real nt = fma(-efg[e], a, j == eq[e] ? Real(0) : t);
-        t = a != Real(0) ? nt : t;
+    constexpr int KE = MPQP_LAZY_KE;
+    Real a[KE], f[KE];
+    int qe[KE];
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+        a[e] = ep[e * ld + j];
+        f[e] = efg[e];
+        qe[e] = eq[e];
+    }
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+        const Real nt = fma(-f[e], a[e], j == qe[e] ? Real(0) : t);
+        t = (e >= v0 && e < ne && a[e] != Real(0)) ? nt : t;
     }
     return t;
 }
@@ -425,9 +439,15 @@ __device__ __forceinline__ void opt_lazy_flush(OptLds<Real, true> &L, int M, int
                             const int g = g0 + u;
                             if (g < G) {
                                 const int v0 = __builtin_amdgcn_readfirstlane(L.gver[g]);
-                                const float4 f0 = *reinterpret_cast<const float4 *>(L.ef + (size_t)g * KE);
-                                const float4 f1 = *reinterpret_cast<const float4 *>(L.ef + (size_t)g * KE + 4);
-                                const float f[KE] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+                                float f[KE];
+#pragma unroll
+                                for (int e = 0; e < KE; e += 4) {
+                                    const float4 fv = *reinterpret_cast<const float4 *>(L.ef + (size_t)g * KE + e);
+                                    f[e] = fv.x;
+                                    f[e + 1] = fv.y;
+                                    f[e + 2] = fv.z;
+                                    f[e + 3] = fv.w;
+                                }
                                 float2 v = t[u];
 #pragma unroll
                                 for (int e = 0; e < KE; ++e) {
@@ -707,9 +727,12 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
                             const int j = lane + MPQP_WAVE * s;
-                            const int jc = j < K + 2 ? j : 0;
-                            const Real t = lazy_entry(L.ep, efg, L.eq, ld, jc, src[jc], v0, ne);
-                            if (j < K + 2) L.prow[j] = t;
+                            Real t = (Real)0;
+                            if (MPQP_WAVE * s < K + 2) {
+                                const int jc = j < K + 2 ? j : 0;
+                                t = lazy_entry(L.ep, efg, L.eq, ld, jc, src[jc], v0, ne);
+                                if (j < K + 2) L.prow[j] = t;
+                            }
                             al[s] = j < K ? t : (Real)0;
                         }
                     }
