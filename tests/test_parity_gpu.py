@@ -506,6 +506,31 @@ def test_forced_status_paths(hip, orc, name, opts, field, status, batching):
     assert (g.records[field] == status).sum() > 0
 
 
+SPLIT_FORCED = [f for f in FORCED if f[0] in ("p1_iter_cap", "p1_unbounded", "p1_stall", "p2_iter_cap",
+                                               "no_perturb_delay")] + [("defaults", dict(), "p1_status", 1)]
+
+
+@pytest.mark.parametrize("name,opts,field,status", SPLIT_FORCED, ids=[f[0] for f in SPLIT_FORCED])
+@pytest.mark.parametrize("precision,batching", [(0, 0), (0, 1), (1, 1)], ids=["fp64-plain", "fp64-pers", "mixed"])
+def test_split_tableau_exits(hip, orc, name, opts, field, status, precision, batching):
+    """A tableau larger than LDS (n=40 m=150 p=6: M2 = 353, 286 KB in FP32,
+    569 KB in FP64), its global rows updated in batches: loops that end in
+    the middle of a batch (caps, empty ratio tests, stall exit), the
+    perturbed rhs from the first pivot, and the defaults.  Every record and
+    basis equal the oracle's; the forced status occurs (FP64)."""
+    _log(f"case: split tableau n=40 m=150 p=6 |S|<=1, {name} {opts}, precision {precision}, batching {batching}")
+    lp = assemble_lps(synthetic_mpqp(40, 150, 6, 21))
+    g, o = _run_pair(hip, orc, lp, 1, precision=precision, batching=batching, **opts)
+    st = hip.stats()
+    hip.set_options()
+    orc.set_options()
+    assert st["tableau_global"] & (2 if precision else 1)
+    assert st["split_rows_fp64" if precision == 0 else "split_rows_fp32"] < lp.M2
+    assert compare_runs(g, o) == []
+    if precision == 0:
+        assert (g.records[field] == status).sum() > 0
+
+
 def test_duplicated_rows(hip, orc):
     """Two identical constraint rows (an exactly dependent pair)."""
     _log("case: exactly duplicated constraint rows, n=8 m=24 p=3 |S|<=3")
