@@ -167,7 +167,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
     int status = ST_ITER_CAP, iters = 0, feasible = 0;
     double z = 0.0;
     for (int it = 0; it < O.max_iter_feas; ++it) {
-        if (tid < MPQP_WAVE) {
+        if (tid < MPQP_WAVE) { MPQP_PRIO_HI();
             // Dantzig pricing, lowest variable id on ties (min_idx_kernel :256-316)
             double v = 0.0;
             int key = -1, aux = -1;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(NT) void feas_kernel(FeasArgs a) {
                 sh_feas = fe;
             }
         }
-        __syncthreads();
+        MPQP_PRIO_LO(); __syncthreads();
         if (sh_done) {
             status = sh_done - 1;
             z = sh_z;

@@ -27,6 +27,17 @@
 #define MPQP_WAVE 64
 #define MPQP_MAXS 8  // in-lane slots per reduction: n <= 512
 
+// Scheduling priority of the wave that runs an LP's serial decision (the
+// other waves of the workgroup wait at the next barrier, and the SIMD is
+// shared with another workgroup's waves): raised for the decision, back to
+// the default before the barrier.
+// (measured: 0.6% on n=60 m=120 p=12, 1.5% on n=30 m=62 p=8)
+#ifndef MPQP_DECISION_PRIO
+#define MPQP_DECISION_PRIO 3
+#endif
+#define MPQP_PRIO_HI() __builtin_amdgcn_s_setprio(MPQP_DECISION_PRIO)
+#define MPQP_PRIO_LO() __builtin_amdgcn_s_setprio(0)
+
 struct DevOpts {
     int max_iter_feas, max_iter_p1, max_iter_p2, stall_perturb, stall_exit, warm;
     double tol_plus, tol_minus, tol_minus_1, tol_minus_opt;

@@ -404,7 +404,10 @@ __device__ __forceinline__ void opt_lazy_start(OptLds<Real, true> &L, int M, int
 // alone.
 template <int NT, typename Real>
 __device__ __forceinline__ void opt_lazy_flush(OptLds<Real, true> &L, int M, int ncols, int ne) {
-    constexpr int KE = MPQP_LAZY_KE, NW = NT / MPQP_WAVE, U = 8;
+#ifndef MPQP_FLUSH_ROWS
+#define MPQP_FLUSH_ROWS 8
+#endif
+    constexpr int KE = MPQP_LAZY_KE, NW = NT / MPQP_WAVE, U = MPQP_FLUSH_ROWS;
     const int R = L.T.R, G = M - R, ld = L.ld;
     const int wave = threadIdx.x / MPQP_WAVE;
     Real *hi = L.T.hi;
@@ -623,7 +626,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
     int it = 0;
     bool stopped = false;
     for (; it < O.max_iter_p1; ++it) {
-        if (wave == 0) {
+        if (wave == 0) { MPQP_PRIO_HI();
             const int lane = sizeof(Real) == 4 ? fresh_lane() : (int)(threadIdx.x & (MPQP_WAVE - 1));
             const bool pert = !no_pert && n_same >= O.stall_perturb;
             Real xv[S];
@@ -633,10 +636,12 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
             // every load is issued unconditionally (index clamped into the
             // tableau) and masked afterwards: one wait instead of a branch
             // and a wait per slot
-            // (the ratio test's cost row and nonbasic use counters do not
-            // depend on the leaving row: read here, with the rows')
+            // (the ratio test's cost row does not depend on the leaving row:
+            // read here, with the rows'; the use counters only matter to the
+            // trees, which read them when they run; FP64, whose leaving row
+            // always runs its tree, reads the basic ones here)
             Real x0[S], xp[S], cl[S];
-            int bv[S], nv[S], nc[S];
+            int bv[S], nv[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int i = lane + MPQP_WAVE * s;
@@ -648,10 +653,9 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 cl[s] = T.row(M)[kc];
                 nv[s] = L.nonb[kc];
             }
+            if constexpr (sizeof(Real) == 8) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                cn[s] = L.bcnt[bv[s]];
-                nc[s] = L.nbcnt[nv[s]];
+                for (int s = 0; s < S; ++s) cn[s] = L.bcnt[bv[s]];
             }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
@@ -662,7 +666,6 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 xv[s] = x;
                 v[s] = (double)x;
                 id[s] = i;
-                cn[s] = valid ? cn[s] : 0;
                 neg |= valid && (double)x < tl.dual_stop_tol;
             }
             // Leaving row.  In the plain mode (no counter preference) the
@@ -696,6 +699,11 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 if (!__any(nan) && close == 1) r = at;
             }
             if (r < 0) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if constexpr (sizeof(Real) == 4) cn[s] = L.bcnt[bv[s]];
+                    cn[s] = lane + MPQP_WAVE * s < M ? cn[s] : 0;
+                }
                 tree_argmin_s<S>(v, id, cn, M, lane, DminPred{mode_b, tl.dual_stop_tol});
                 r = rfl(id[0]);
             }
@@ -770,10 +778,44 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                     const double ratio = (double)(-cl[s] / al[s]);
                     id[s] = take ? kk : -1;
                     v[s] = take ? ratio : 100000.0;
-                    cn[s] = take ? nc[s] : 0;
                 }
-                tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
-                const int qk = rfl(id[0]);
+                // Entering column.  Without counter preference (mode 0) the
+                // predicate keeps the strictly smaller of two taken entries
+                // and never an untaken one over a taken one: a unique minimum
+                // among the taken ratios (no NaN) is the tree's answer
+                // whatever the order, and none taken leaves -1.  Anything
+                // else runs the tree.
+                int qk = -2;
+                if (mode == 0) {
+                    double ml = INFINITY;
+                    bool nan = false, any_t = false;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const bool t = id[s] >= 0;
+                        any_t |= t;
+                        nan |= t && v[s] != v[s];
+                        ml = (t && v[s] < ml) ? v[s] : ml;
+                    }
+                    if (!__any(any_t)) {
+                        qk = -1;
+                    } else {
+                        const double m = -wave_max(-ml);
+                        int cnt = 0, at = -1;
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            const unsigned long long hit = __ballot(id[s] >= 0 && v[s] == m);
+                            cnt += __popcll(hit);
+                            at = (at < 0 && hit) ? (__ffsll((long long)hit) - 1) + MPQP_WAVE * s : at;
+                        }
+                        if (!__any(nan) && cnt == 1) qk = at;
+                    }
+                }
+                if (qk == -2) {
+#pragma unroll
+                    for (int s = 0; s < S; ++s) cn[s] = id[s] >= 0 ? L.nbcnt[nv[s]] : 0;
+                    tree_argmin_s<S>(v, id, cn, K, lane, DratioPred{mode});
+                    qk = rfl(id[0]);
+                }
                 OPT_PROF_DEC(sh, 10);
                 if (qk < 0) {
                     action = ACT_UNBOUNDED;
@@ -788,7 +830,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 sh.q = q;
                 sh.action = action;
             }
-        } else if (wave == 1) {
+        } else if (wave == 1) { MPQP_PRIO_HI();
             // meanwhile: the sum of the negative x_B (the same values as the
             // decision's), used by the stall counter after a pivot
             const int lane = sizeof(Real) == 4 ? fresh_lane() : (int)(threadIdx.x & (MPQP_WAVE - 1));
@@ -805,7 +847,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
             tree_sum_lane0<S, Real>(ps, M, lane);
             if (lane == 0) sh.pinf = (double)ps[0];
         }
-        __syncthreads();
+        MPQP_PRIO_LO(); __syncthreads();
         OPT_PROF(sh, 1);
         const int action = sh.action, r = sh.r, q = sh.q;
         // stall bookkeeping, kept by every thread: a pivot without progress
@@ -846,7 +888,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
     // Restore dual feasibility if the counter modes broke it: primal simplex
     // on the phase-1 cost, Dantzig pricing with lowest variable on ties.
     for (; it < O.max_iter_p1; ++it) {
-        if (wave == 0) {
+        if (wave == 0) { MPQP_PRIO_HI();
             const Real *tc = T.row(M);
             double dv = 0.0;
             int key = -1, aux = -1;
@@ -867,7 +909,7 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                 sh.q = aux;
             }
         }
-        __syncthreads();
+        MPQP_PRIO_LO(); __syncthreads();
         const int action = sh.action, r = sh.r, q = sh.q;
         if (action != ACT_PIVOT) __syncthreads();
         if (action == ACT_STOP) break;
@@ -1006,7 +1048,7 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const Dev
     int ne = 0;  // pending updates of the global rows (split tableau)
     P2Result res{ST_ITER_CAP, 0, 0, 0.0};
     for (int it = 0; it < O.max_iter_p2; ++it) {
-        if (wave == 0) {
+        if (wave == 0) { MPQP_PRIO_HI();
             double v[S];
             int id[S], cn[S];
 #pragma unroll
@@ -1049,7 +1091,7 @@ __device__ __forceinline__ P2Result opt_phase2(OptLds<Real, SPLIT> &L, const Dev
                 sh.z = zz;
             }
         }
-        __syncthreads();
+        MPQP_PRIO_LO(); __syncthreads();
         const int action = sh.action, r = sh.r, q = sh.q;
         const double zz = sh.z;
         if (action != ACT_PIVOT) __syncthreads();
