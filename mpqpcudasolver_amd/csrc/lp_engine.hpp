@@ -771,7 +771,8 @@ __device__ __forceinline__ void pivot_rows(const TB &T, int nrows, int ncols, in
 
 // Pivot on row r with an entering column v held outside the tableau (rows
 // [0, nrows) of v); the leaving column is discarded.  Retained entries get
-// exactly the arithmetic of pivot_team.
+// exactly the arithmetic of pivot_team (the update of update_cols, v as the
+// factors).
 template <int NT, typename TB>
 __device__ __forceinline__ void pivot_ext(const TB &T, int nrows, int ncols, int r,
                                           const typename TB::real_t *v) {
@@ -781,41 +782,35 @@ __device__ __forceinline__ void pivot_ext(const TB &T, int nrows, int ncols, int
     Real *tr = T.row(r);
     for (int j = tid; j < ncols; j += NT) tr[j] = tr[j] / p;
     __syncthreads();
-    constexpr int NW = NT / MPQP_WAVE;
-    constexpr int U = 4;
-    const int lane = tid & (MPQP_WAVE - 1);
-    const int wave = tid / MPQP_WAVE;
-    for (int j0 = 0; j0 < ncols; j0 += MPQP_WAVE) {
-        const int j = j0 + lane;
-        const Real prj = j < ncols ? tr[j] : Real(0);
-        if (prj != Real(0)) {
-            for (int i0 = wave * U; i0 < nrows; i0 += NW * U) {
-                Real t[U], f[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    t[u] = i < nrows ? T.row(i)[j] : Real(0);
-                    f[u] = i < nrows ? v[i] : Real(0);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int i = i0 + u;
-                    if (i < nrows && i != r) T.row(i)[j] = fma(-f[u], prj, t[u]);
-                }
-            }
-        }
-    }
+    update_cols<NT, TB::split ? 8 : 4>([&](int i) { return T.row(i); }, nrows, ncols, r, tr, v);
     __syncthreads();
 }
 
-// Reduced-cost row from a compact ascending list of the rows with cb != 0.
+// Reduced-cost row from a compact ascending list of the rows with cb != 0:
+// per column one fma chain in list order, the loads of eight list entries
+// in flight ahead of it.
 template <int NT, typename TB, typename CbF, typename CnF>
 __device__ __forceinline__ void cost_row_list(const TB &T, int M, int ncols, const int *rows, int nrows_cb,
                                               CbF cb, CnF cN) {
     typedef typename TB::real_t Real;
+    constexpr int B = 8;
     for (int j = threadIdx.x; j < ncols; j += NT) {
         Real acc = cN(j);
-        for (int t = 0; t < nrows_cb; ++t) {
+        int t = 0;
+        for (; t + B <= nrows_cb; t += B) {
+            int ii[B];
+            Real c[B], tv[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) ii[u] = rows[t + u];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                c[u] = cb(ii[u]);
+                tv[u] = T.row(ii[u])[j];
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u) acc = fma(-c[u], tv[u], acc);
+        }
+        for (; t < nrows_cb; ++t) {
             const int i = rows[t];
             acc = fma(-cb(i), T.row(i)[j], acc);
         }
