@@ -120,7 +120,7 @@ typedef struct MpqpOptions {
     double pivot_rel;          /* ratio tests also require |entry| > pivot_rel * max|entry| (0: off) */
     double tol32;              /* mixed precision: pivot / stop tolerance of the FP32 pass (1e-5) */
     double licq32;             /* mixed precision: FP32 LICQ screen; smaller pivots go to FP64 (1e-6) */
-    double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass (1e-3) */
+    double pivot_rel32;        /* mixed precision: relative pivot threshold of the FP32 pass (5e-3) */
     double licq_screen;        /* LICQ screen of each candidate before its LPs: S fails when a
                                   pivot d_t of the Cholesky factorisation of (A Q^-1 A^T)_SS (rows
                                   of S in ascending order) has d_t <= licq_screen * G_tt, i.e. row

@@ -337,7 +337,7 @@ void default_opts(MpqpOptions *o) {
     o->pivot_rel = 1e-9;
     o->tol32 = 1e-5;
     o->licq32 = 1e-6;
-    o->pivot_rel32 = 1e-3;
+    o->pivot_rel32 = 5e-3;
     o->licq_screen = 1e-14;
     o->max_chunk = 0;
     o->split_wgs = 0;
