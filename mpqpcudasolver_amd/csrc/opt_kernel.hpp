@@ -34,8 +34,8 @@ __host__ __device__ inline int opt_ld(int K) {
 // (16-byte aligned) the deferred updates of its global rows.  During certification
 // bcnt/nbcnt hold x by variable (N2 doubles) and colq/vcol the list of
 // nonbasic e-variables.
-// Phase-1 pivots of a split tableau update its global rows [R, M) in
-// batches of MPQP_LAZY_KE (opt_lazy_pivot).
+// Phase-1 and phase-2 pivots of a split tableau update its global rows
+// [R, M) in batches of MPQP_LAZY_KE (opt_lazy_pivot).
 #ifndef MPQP_LAZY_KE
 #define MPQP_LAZY_KE 8
 #endif
@@ -173,8 +173,8 @@ struct OptLds {
             eq = gver + G;
         }
     }
-    // rhs / perturbation entry of row i < M inside the phase-1 loop (the
-    // global rows' mirrors when the tableau is split)
+    // rhs / perturbation entry of row i < M inside the phase-1 and phase-2
+    // loops (the global rows' mirrors when the tableau is split)
     __device__ __forceinline__ const Real *rhs_at(int i, int RHS) const {
         if constexpr (SPLIT) return i >= T.R ? gx + (i - T.R) : T.lo + i * ld + RHS;
         else return T.row(i) + RHS;
@@ -343,18 +343,19 @@ __device__ __forceinline__ void opt_swap(OptLds<Real, SPLIT> &L, int r, int q) {
 // phase 1
 // ---------------------------------------------------------------------
 // ---------------------------------------------------------------------
-// Deferred updates of the global rows of a split tableau (phase 1)
+// Deferred updates of the global rows of a split tableau (phases 1 and 2)
 //
 // A pivot updates the LDS rows (and the cost row) at once, as pivot_rows
 // does, and only records itself for the rows [R, M) held in global memory:
 // its divided pivot row (ep) and, per global row, its factor (ef: the row's
-// column-q entry at the time of the pivot).  Within the phase-1 loop the
-// global rows are read only through their rhs and perturbation entries,
-// which are kept current in LDS (gx, gp), through the leaving row (brought
-// up to date when it is chosen, then written back: gver counts the
-// recorded pivots already applied to each row) and through the entering
-// column (brought up to date in the pivot).  Every MPQP_LAZY_KE pivots, and
-// when the loop ends, the recorded pivots are applied to every global row
+// column-q entry at the time of the pivot).  Within the phase-1 and phase-2
+// loops the global rows are read only through their rhs and perturbation
+// entries, which are kept current in LDS (gx, gp), through the leaving row
+// (brought up to date when it is chosen, or in the pivot's division in phase
+// 2, then written back: gver counts the recorded pivots already applied to
+// each row) and through the entering column (brought up to date in the
+// pivot, or by the ratio test in phase 2).  Every MPQP_LAZY_KE pivots, and
+// when a loop ends, the recorded pivots are applied to every global row
 // in one pass over it (opt_lazy_flush).  Each entry goes through exactly
 // the fma sequence of the immediate update (column q_e of the other rows
 // from zero with the factor 1/p_e, other columns only where the pivot-row
