@@ -386,6 +386,20 @@ __device__ __forceinline__ Real lazy_entry(const Real *ep, const Real *efg, cons
     return t;
 }
 
+// The same with the operands shared by several entries loaded once: the
+// pivot-row entries a[e] of the entry's column, its row's factors f[e] and
+// the pivot columns qe[e] (j = the entry's column).
+template <typename Real>
+__device__ __forceinline__ Real lazy_apply(const Real (&a)[MPQP_LAZY_KE], const Real (&f)[MPQP_LAZY_KE],
+                                           const int (&qe)[MPQP_LAZY_KE], int j, Real t, int v0, int ne) {
+#pragma unroll
+    for (int e = 0; e < MPQP_LAZY_KE; ++e) {
+        const Real nt = fma(-f[e], a[e], j == qe[e] ? Real(0) : t);
+        t = (e >= v0 && e < ne && a[e] != Real(0)) ? nt : t;
+    }
+    return t;
+}
+
 // Start of the phase-1 loop: mirrors of the global rows' rhs and
 // perturbation columns, no update pending (the caller's barrier follows).
 template <int NT, typename Real>
@@ -733,13 +747,23 @@ __device__ __forceinline__ P1Result opt_phase1(OptLds<Real, SPLIT> &L, const Dev
                         const int v0 = L.gver[g];
                         const Real *src = T.hi + (size_t)g * ld;
                         const Real *efg = L.ef + (size_t)g * MPQP_LAZY_KE;
+                        Real f[MPQP_LAZY_KE];
+                        int qe[MPQP_LAZY_KE];
+#pragma unroll
+                        for (int e = 0; e < MPQP_LAZY_KE; ++e) {
+                            f[e] = efg[e];
+                            qe[e] = L.eq[e];
+                        }
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
                             const int j = lane + MPQP_WAVE * s;
                             Real t = (Real)0;
                             if (MPQP_WAVE * s < K + 2) {
                                 const int jc = j < K + 2 ? j : 0;
-                                t = lazy_entry(L.ep, efg, L.eq, ld, jc, src[jc], v0, ne);
+                                Real a[MPQP_LAZY_KE];
+#pragma unroll
+                                for (int e = 0; e < MPQP_LAZY_KE; ++e) a[e] = L.ep[e * ld + jc];
+                                t = lazy_apply(a, f, qe, jc, src[jc], v0, ne);
                                 if (j < K + 2) L.prow[j] = t;
                             }
                             al[s] = j < K ? t : (Real)0;
@@ -986,6 +1010,13 @@ __device__ __forceinline__ int lazy_ratio_wave(OptLds<Real, true> &L, int M, int
     const int lane = threadIdx.x & (MPQP_WAVE - 1);
     const int R = L.T.R, ld = L.ld;
     Real al[S], xr[S];
+    Real pa[KE];
+    int qe[KE];
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+        pa[e] = L.ep[e * ld + q];
+        qe[e] = L.eq[e];
+    }
     double cm = 0.0;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -994,7 +1025,10 @@ __device__ __forceinline__ int lazy_ratio_wave(OptLds<Real, true> &L, int M, int
         if (i < M) {
             if (i >= R) {
                 const int g = i - R;
-                a = lazy_entry(L.ep, L.ef + (size_t)g * KE, L.eq, ld, q, L.T.hi[(size_t)g * ld + q], L.gver[g], ne);
+                Real f[KE];
+#pragma unroll
+                for (int e = 0; e < KE; ++e) f[e] = L.ef[(size_t)g * KE + e];
+                a = lazy_apply(pa, f, qe, q, L.T.hi[(size_t)g * ld + q], L.gver[g], ne);
                 L.ef[(size_t)g * KE + ne] = a;
                 x = L.gx[g];
             } else {
